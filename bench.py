@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Woven-field throughput on one MI355X: Mpts/s of a 512^3 FP32 woven field.
+
+One step = one whole woven-field generation from the tube table and centerline points already
+resident in device memory to normalised vorticity and velocity resident in device memory:
+geometry (cze, splines, turning angle, sub-points), binning, deposition, forward FFT,
+Biot-Savart, inverse FFT, normalisation (tube.f90:392-1043). Workload: 6 levels, r = 0.8*2^-l,
+4*4^l tubes per level (5460 tubes), 256*2^l..256*2^(l+1) points per tube (55M points), seeded
+fBb centerlines with H = 0.75, synthetic. Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpts/s woven field at 512^3 FP32; time-to-target-spectrum (±5%) vs sweep"
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+SLOT_NAMES = ["theta", "subpoints", "bin_count", "deposit", "fixed_to_float", "fft_x", "fft_y",
+              "fft_z_biot_savart", "fft_y_inv", "fft_x_inv", "spectral_reduce"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name: str):
+    from fastweavturb_amd import fbb
+    if name == "c2":
+        return "batched_512_fp32", 32, fbb.config_batched_512(seed=1234)
+    if name == "c1":
+        return "plain_128_fp64", 64, fbb.config_plain_128(seed=1234)
+    if name == "small":
+        return "small_64_fp32", 32, fbb.woven_config(64, [fbb.Level(0.5, 4, (256, 512)), fbb.Level(0.25, 16, (512, 1024))], 1234)
+    raise SystemExit(f"unknown config {name}")
+
+
+def algorithmic_cost(slot: str, N: int, seg: dict, dep: dict, p):
+    """(amount, unit, bound, peak) of one launch of a kernel: bytes for memory-bound kernels,
+    FLOPs for arithmetic-bound ones (per-unit figures documented in DESIGN.md)."""
+    H = N // 2 + 1
+    half = 3 * N * N * H * 8          # three FP32 half spectra, bytes
+    real = 3 * N ** 3 * 4             # three FP32 fields, bytes
+    if slot in ("fft_y", "fft_z_biot_savart", "fft_y_inv"):
+        return 2 * half, "GB/s", "hbm", PEAK_HBM_GBS
+    if slot == "fft_x":
+        return real + half, "GB/s", "hbm", PEAK_HBM_GBS
+    if slot == "fft_x_inv":
+        return half + real + 2 * real, "GB/s", "hbm", PEAK_HBM_GBS
+    if slot == "fixed_to_float":
+        return 3 * N ** 3 * (8 + 4), "GB/s", "hbm", PEAK_HBM_GBS
+    if slot == "theta":
+        # 1001 samples per section; per sample 108 FP64 arithmetic ops + 2 sqrt + 1 divide,
+        # counted as 120 FLOP
+        return seg["nsec_all"] * (p.ntf_pre + 1) * 120, "TFLOP/s", "valu-fp64", PEAK_FP64_TFLOPS
+    if slot == "deposit":
+        # 24 FP64 FLOP per column scanned, 30 per candidate test, 250 per hit
+        return dep["columns"] * 24 + dep["candidates"] * 30 + dep["hits"] * 250, "TFLOP/s", "valu", PEAK_FP32_TFLOPS
+    return None
+
+
+def cpu_baseline(p, t, budget_s: float):
+    """Time the FP64 oracle (single-threaded C restatement of the reference) on a bounded
+    sample of the same workload and extrapolate to the whole field."""
+    from oracle import oracle_py as O
+    from fastweavturb_amd.woven_io import Tubes
+    rng = np.random.default_rng(0)
+    N = p.N
+    # level boundaries by sigma
+    sig = t.sigma
+    levels = sorted(set(np.round(sig, 12)), reverse=True)
+    pt_base = np.concatenate([[0], np.cumsum(t.npoint)])
+    t_geo = 0.0
+    t_dep = 0.0
+    notes = []
+    per_level_budget = budget_s * 0.8 / max(1, len(levels))
+    for lv in levels:
+        idx = np.flatnonzero(np.round(sig, 12) == lv)
+        tube = int(rng.choice(idx))
+        a, b = pt_base[tube], pt_base[tube + 1]
+        sub = Tubes(t.npoint[tube:tube + 1].copy(), t.cx[a:b].copy(), t.cy[a:b].copy(), t.cz[a:b].copy(),
+                    t.sigma[tube:tube + 1].copy(), t.gamma[tube:tube + 1].copy(), t.n_T[tube:tube + 1].copy(),
+                    t.if_filter[tube:tube + 1].copy())
+        P = O.Params(N, N, 1, 1, p.sigma_ratio, p.eta_t, p.ntf_times, p.ntf_pre, p.Rtube_ratio, p.dzeta_sup)
+        t0 = time.perf_counter()
+        geo = O.Geometry(P, sub.npoint, sub.cx, sub.cy, sub.cz)
+        tg = time.perf_counter() - t0
+        level_points = int(t.npoint[idx].sum())
+        t_geo += tg * level_points / int(sub.npoint[0])
+        # deposit a growing chunk of sub-segments until the per-level budget is used
+        nseg = geo.nseg
+        chunk = 1
+        spent, done, secs = 0.0, 0, 0.0
+        while done < nseg and spent < per_level_budget:
+            hi = min(nseg, done + chunk)
+            dt, _ = O.deposit_range_timed(P, geo, sub.sigma, sub.gamma, sub.n_T, done, hi)
+            secs += dt
+            spent += dt
+            done = hi
+            chunk = min(chunk * 2, 1 << 20)
+        seg_per_point = nseg / int(sub.npoint[0])
+        t_dep += secs * (seg_per_point * level_points) / done
+        notes.append(f"r={lv * np.sqrt(2):.4g}: tube {tube}, {done}/{nseg} sub-segments")
+        geo.close()
+    # spectral part: 6 3-D c2c transforms of N^3 = 18 N^2 lines of length N, timed on a sample
+    line = np.random.default_rng(1).standard_normal(N) + 0j
+    nl = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < min(2.0, budget_s * 0.1):
+        O.fft1d(line, True)
+        nl += 1
+    t_line = (time.perf_counter() - t0) / nl
+    t_fft = 18 * N * N * t_line
+    total = t_geo + t_dep + t_fft
+    sample = ("FP64 oracle, 1 thread: per level one random tube's geometry (turning angle, "
+              "splines, sub-points) and a leading run of its sub-segments deposited, scaled by "
+              "points/sub-segments to the whole level; 3-D FFTs from timed 1-D lines; "
+              + "; ".join(notes))
+    return {"value": N ** 3 / total / 1e6, "unit": "Mpts/s", "cores": 1, "kind": "port",
+            "sample": sample, "seconds_estimated": total,
+            "seconds_parts": {"geometry": t_geo, "deposition": t_dep, "fft": t_fft}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=["c2", "c1", "small"])
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU decomposition is described in DESIGN.md and not built.", file=sys.stderr)
+        sys.exit(2)
+
+    from fastweavturb_amd.api import WovenGPU
+    t0 = time.time()
+    wname, precision, (p, t) = make_workload(args.config)
+    log(f"[bench] workload {wname}: {t.nline} tubes, {t.npoint_all} points, N={p.N} "
+        f"(generated in {time.time() - t0:.1f}s)")
+    g = WovenGPU(p.N, precision)
+    g.set_params(p)
+    g.set_tubes(t)
+    for i in range(args.warmup):
+        g.generate()
+        log(f"[bench] warmup {i} done, stages ms {np.round(g.stage_times()[:4], 2).tolist()}")
+    # every library call is stream-synchronous on return, so the clock brackets finished work
+    t0 = time.perf_counter()
+    stage_ms = np.zeros(4)
+    for i in range(args.steps):
+        g.generate()
+        stage_ms += g.stage_times()[:4]
+    dt = time.perf_counter() - t0
+    ms_per_step = dt / args.steps * 1e3
+    value = p.N ** 3 / (ms_per_step * 1e-3) / 1e6
+    times = g.stage_times()
+    kernels = {SLOT_NAMES[i]: float(times[4 + i]) for i in range(len(SLOT_NAMES)) if times[4 + i] > 0}
+    seg = g.seg_stats
+    dep = g.deposit()  # counters of one more deposition (same inputs), outside the timed region
+    log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step; stages {np.round(stage_ms / args.steps, 2).tolist()}")
+    log(f"[bench] kernels ms {json.dumps({k: round(v, 3) for k, v in kernels.items()})}")
+    log(f"[bench] geometry {json.dumps(seg)}; deposit {json.dumps(dep)}")
+    dom = max(kernels, key=kernels.get)
+    roof = None
+    cost = algorithmic_cost(dom, p.N, seg, dep, p)
+    if cost is not None:
+        amount, unit, bound, peak = cost
+        sec = kernels[dom] * 1e-3
+        achieved = amount / sec / (1e9 if unit == "GB/s" else 1e12)
+        roof = {"kernel": dom, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+                "frac": achieved / peak, "traffic": None, "ms": kernels[dom]}
+    cpu = None
+    if not args.no_cpu_baseline:
+        t0 = time.time()
+        cpu = cpu_baseline(p, t, args.cpu_budget)
+        log(f"[bench] cpu baseline {cpu['value']:.4g} Mpts/s (sampled in {time.time() - t0:.1f}s)")
+    out = {
+        "metric": METRIC, "value": value, "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if precision == 32 else "f64", "data": "synthetic",
+        "config": {"workload": wname, "N": p.N, "tubes": t.nline, "points": int(t.npoint_all),
+                   "sub_segments": int(seg["nseg_all"]), "levels": 6 if args.config == "c2" else None},
+        "stages_ms": dict(zip(["segments", "bin", "deposit", "biot_savart"], (stage_ms / args.steps).tolist())),
+        "kernels_ms": kernels, "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

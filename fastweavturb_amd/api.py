@@ -1,0 +1,268 @@
+"""Python host interface over the wovengpu C ABI (include/wovengpu.h), through ctypes.
+
+The reference's interface for this path is its two input files plus inline Fortran
+(tube.f90:392-1043) and external subroutines (subroutine.f90). `WovenGPU` keeps the same
+stages and names: build segments, bin, deposit, Biot-Savart + normalisation, spectrum and
+structure functions, each one call into the HIP library. There is no CPU fallback: if the
+library is missing the import of the library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from .woven_io import RunParams, Tubes
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwovengpu.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "wovengpu.h")
+
+WT_FIELD = {"wx": 0, "wy": 1, "wz": 2, "ux": 3, "uy": 4, "uz": 5}
+ERRORS = {-1: "WT_EINVAL", -2: "WT_ESTATE", -3: "WT_EDEVICE", -4: "WT_ENOMEM", -5: "WT_ERANGE"}
+
+
+class WovenError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Params(C.Structure):
+    _fields_ = [("N", C.c_int32), ("n_d", C.c_int32), ("line_s", C.c_int32), ("line_e", C.c_int32),
+                ("sigma_ratio", C.c_double), ("eta_t", C.c_double), ("ntf_times", C.c_int32),
+                ("ntf_pre", C.c_int32), ("Rtube_ratio", C.c_double), ("dzeta_sup", C.c_double)]
+
+
+class _SegStats(C.Structure):
+    _fields_ = [("npoint_all", C.c_int64), ("nsec_all", C.c_int64), ("nsub_all", C.c_int64),
+                ("nseg_all", C.c_int64), ("ntf_max", C.c_int32), ("reserved", C.c_int32),
+                ("dtf_min", C.c_double), ("ntf_sum", C.c_int64)]
+
+
+class _DepStats(C.Structure):
+    _fields_ = [("columns", C.c_int64), ("candidates", C.c_int64), ("hits", C.c_int64),
+                ("fixed_point_shift", C.c_int32), ("overflow", C.c_int32)]
+
+
+_lib = None
+
+
+def header_functions() -> list[str]:
+    """Names of every function the public header declares."""
+    with open(HEADER_PATH) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(wt_[a-z_0-9]+)\s*\(", text)))
+
+
+def load_library():
+    """Load libwovengpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "or python -m fastweavturb_amd.build")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, dp = C.c_void_p, C.c_int32, C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    L.wt_last_error.restype = C.c_char_p
+    L.wt_last_error.argtypes = [vp]
+    L.wt_create.argtypes = [C.POINTER(vp), i32, i32, i32]
+    L.wt_destroy.argtypes = [vp]
+    L.wt_destroy.restype = None
+    L.wt_set_params.argtypes = [vp, C.POINTER(_Params)]
+    L.wt_set_tubes.argtypes = [vp, i32, ip, dp, dp, ip, ip, dp, dp, dp]
+    L.wt_build_segments.argtypes = [vp, C.POINTER(_SegStats)]
+    L.wt_bin.argtypes = [vp]
+    L.wt_deposit.argtypes = [vp, C.POINTER(_DepStats)]
+    L.wt_biot_savart.argtypes = [vp, dp, dp, dp]
+    L.wt_generate.argtypes = [vp, dp, dp, dp]
+    L.wt_spectrum.argtypes = [vp, i32, dp, i32]
+    L.wt_nek.argtypes = [i32]
+    L.wt_nek.restype = i32
+    L.wt_structure_functions.argtypes = [vp, i32, i32, i32, i32, ip, dp]
+    L.wt_sp_separations.argtypes = [i32, ip, i32]
+    L.wt_sp_separations.restype = i32
+    L.wt_get_field.argtypes = [vp, i32, vp, i32]
+    L.wt_set_field.argtypes = [vp, i32, vp, i32]
+    L.wt_get_sections.argtypes = [vp, ip, dp]
+    L.wt_get_subpoints.argtypes = [vp, dp, dp]
+    L.wt_get_bins.argtypes = [vp, ip, dp]
+    L.wt_stage_times.argtypes = [vp, dp, i32]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def nek(N: int) -> int:
+    return int(load_library().wt_nek(N))
+
+
+def sp_separations(N: int) -> np.ndarray:
+    L = load_library()
+    n = L.wt_sp_separations(N, None, 0)
+    r = np.zeros(n, dtype=np.int32)
+    L.wt_sp_separations(N, _ip(r), n)
+    return r
+
+
+class WovenGPU:
+    """One device context: an N^3 grid in FP32 or FP64 on one GPU."""
+
+    def __init__(self, N: int, precision: int = 32, device: int = 0):
+        self._L = load_library()
+        h = C.c_void_p()
+        rc = self._L.wt_create(C.byref(h), N, precision, device)
+        if rc != 0:
+            raise WovenError(rc, self._L.wt_last_error(None).decode())
+        self._h = h
+        self.N = N
+        self.precision = precision
+        self.dtype = np.float64 if precision == 64 else np.float32
+        self.nline = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.wt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc < 0:
+            raise WovenError(rc, self._L.wt_last_error(self._h).decode())
+        return rc
+
+    # --- inputs --------------------------------------------------------------------------
+    def set_params(self, p: RunParams):
+        self._params = _Params(p.N, p.n_d, p.line_s, p.line_e, p.sigma_ratio, p.eta_t,
+                               p.ntf_times, p.ntf_pre, p.Rtube_ratio, p.dzeta_sup)
+        self._check(self._L.wt_set_params(self._h, C.byref(self._params)))
+
+    def set_tubes(self, t: Tubes):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (t.sigma, t.gamma, t.cx, t.cy, t.cz)]
+        npt = np.ascontiguousarray(t.npoint, dtype=np.int32)
+        nT = np.ascontiguousarray(t.n_T, dtype=np.int32)
+        filt = np.ascontiguousarray(t.if_filter, dtype=np.int32)
+        self._check(self._L.wt_set_tubes(self._h, len(npt), _ip(npt), _dp(a[0]), _dp(a[1]), _ip(nT),
+                                         _ip(filt), _dp(a[2]), _dp(a[3]), _dp(a[4])))
+        self.nline = len(npt)
+
+    # --- stages --------------------------------------------------------------------------
+    def build_segments(self) -> dict:
+        s = _SegStats()
+        self._check(self._L.wt_build_segments(self._h, C.byref(s)))
+        self.seg_stats = {k: getattr(s, k) for k, _ in _SegStats._fields_ if k != "reserved"}
+        return self.seg_stats
+
+    def bin(self):
+        self._check(self._L.wt_bin(self._h))
+
+    def deposit(self) -> dict:
+        s = _DepStats()
+        self._check(self._L.wt_deposit(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _DepStats._fields_}
+
+    def biot_savart(self):
+        e, u, en = C.c_double(), C.c_double(), C.c_double()
+        self._check(self._L.wt_biot_savart(self._h, C.byref(e), C.byref(u), C.byref(en)))
+        return e.value, u.value, en.value
+
+    def generate(self):
+        e, u, en = C.c_double(), C.c_double(), C.c_double()
+        self._check(self._L.wt_generate(self._h, C.byref(e), C.byref(u), C.byref(en)))
+        return e.value, u.value, en.value
+
+    def stage_times(self) -> np.ndarray:
+        ms = np.zeros(4)
+        self._L.wt_stage_times(self._h, _dp(ms), 4)
+        return ms
+
+    # --- statistics ----------------------------------------------------------------------
+    def spectrum(self, field: int = 1) -> np.ndarray:
+        E = np.zeros(nek(self.N))
+        n = self._check(self._L.wt_spectrum(self._h, field, _dp(E), len(E)))
+        return E[:n]
+
+    def structure_functions(self, mode: int, pmin: int, pmax: int, seps) -> np.ndarray:
+        s = np.ascontiguousarray(seps, dtype=np.int32)
+        S = np.zeros((pmax - pmin + 1) * len(s))
+        self._check(self._L.wt_structure_functions(self._h, mode, pmin, pmax, len(s), _ip(s), _dp(S)))
+        return S.reshape(pmax - pmin + 1, len(s))
+
+    # --- fields --------------------------------------------------------------------------
+    def get_field(self, which, fp64: bool = True) -> np.ndarray:
+        w = WT_FIELD[which] if isinstance(which, str) else int(which)
+        out = np.empty((self.N,) * 3, dtype=np.float64 if fp64 else np.float32)
+        self._check(self._L.wt_get_field(self._h, w, out.ctypes.data_as(C.c_void_p), int(fp64)))
+        return out
+
+    def get_fields(self, name: str, fp64: bool = True) -> np.ndarray:
+        base = 0 if name == "omega" else 3
+        return np.stack([self.get_field(base + c, fp64) for c in range(3)])
+
+    def set_field(self, which, arr: np.ndarray):
+        w = WT_FIELD[which] if isinstance(which, str) else int(which)
+        a = np.ascontiguousarray(arr)
+        if a.shape != (self.N,) * 3:
+            raise ValueError(f"field must have shape {(self.N,) * 3}")
+        fp64 = a.dtype == np.float64
+        if not fp64:
+            a = a.astype(np.float32, copy=False)
+        self._check(self._L.wt_set_field(self._h, w, a.ctypes.data_as(C.c_void_p), int(fp64)))
+
+    def set_fields(self, name: str, arr: np.ndarray):
+        base = 0 if name == "omega" else 3
+        for c in range(3):
+            self.set_field(base + c, arr[c])
+
+    # --- intermediate arrays (parity inspection) -------------------------------------------
+    def sections(self):
+        S = self.seg_stats["nsec_all"]
+        ntf = np.zeros(S, dtype=np.int32)
+        dtf = np.zeros(S)
+        self._check(self._L.wt_get_sections(self._h, _ip(ntf), _dp(dtf)))
+        return ntf, dtf
+
+    def subpoints(self):
+        n = self.seg_stats["nsub_all"]
+        ci = np.zeros((n, 3))
+        s = np.zeros(n)
+        self._check(self._L.wt_get_subpoints(self._h, _dp(ci), _dp(s)))
+        return ci, s
+
+    def bins(self):
+        n = self.seg_stats["nsub_all"]
+        lohi = np.zeros((n, 6), dtype=np.int32)
+        R = np.zeros(n)
+        self._check(self._L.wt_get_bins(self._h, _ip(lohi), _dp(R)))
+        return lohi, R
+
+
+def run_woven(p: RunParams, t: Tubes, precision: int = 32, device: int = 0):
+    """Whole path on one GPU; returns the context with omega and u resident, and stats."""
+    g = WovenGPU(p.N, precision, device)
+    g.set_params(p)
+    g.set_tubes(t)
+    etot, up, ens = g.generate()
+    return g, {"etot": etot, "up": up, "enstrophy": ens}

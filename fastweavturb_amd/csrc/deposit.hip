@@ -1,0 +1,460 @@
+// Vorticity deposition (tube.f90:800-892) on the GPU.
+//
+// The reference walks every cell of each sub-segment's integer bounding box and keeps the
+// cells that pass three tests (t0 >= 0, t1 < 0, rho_c < Rtube). Its sub-segments are far
+// thinner than a cell, so almost every box cell fails. Here each work item is one sub-segment
+// and up to 256 grid columns of its box, the columns running along the chord's dominant axis.
+// For each column the two end planes give, in closed form, the short index range that can
+// pass; only those candidates get the reference's exact FP64 test, so the set of depositing
+// (cell, sub-segment) pairs is exactly the reference's. Hits are compacted per wavefront
+// through an LDS queue and evaluated 64 at a time with every lane busy.
+//
+// Precision modes:
+//   double: the reference formulas in the reference order; accumulation by FP64 atomics.
+//   float : geometry, tests and the spline parameter zeta in FP64; the Gaussian core, the
+//           spline evaluation and the Frenet terms in FP32 in coordinates local to the
+//           sub-segment start; accumulation in 64-bit fixed point (exact and
+//           order-independent, so results are bitwise reproducible), converted once at the end.
+// This file is compiled with -ffp-contract=off; the FP32 evaluation re-enables contraction.
+#include "wt_device.h"
+
+namespace wt {
+
+constexpr int kWaves = 4;             // waves per block
+constexpr int kQCap = 128;            // LDS hit queue entries per wave
+constexpr int kColsPerItemDep = 256;  // must match the binning layout
+
+struct Hit {
+    uint32_t q;
+    int16_t ip, jp, kp, pad;
+};
+
+struct DepArgs {
+    Geo g;
+    Run r;
+    Bins b;
+    void *omega;                  // double* in FP64 mode
+    unsigned long long *acc;      // fixed-point accumulators in FP32 mode
+    double fx_scale;              // 2^shift
+    double fx_limit;              // |value * scale| bound before overflow
+    DeviceScalars *scal;
+    unsigned long long *counters; // [0] columns [1] candidates [2] hits [3] overflow
+};
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+    unsigned lane = __lane_id();
+    return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        int w = __shfl_xor(v, o, 64);
+        v = v > w ? v : w;
+    }
+    return v;
+}
+
+// ---- hit evaluation, FP64: tube.f90:819-892 in the reference order ----------------------
+__device__ void eval_hit_f64(const DepArgs &A, const Hit &h) {
+    const Geo &g = A.g;
+    const Run &r = A.r;
+    SubSeg sg;
+    subseg_load(g, r, h.q, sg);
+    const int l = sg.tube;
+    const double LD = g.Len_Disc[l], len = g.length[l], Gamma = g.gamma[l];
+    const double smin = g.sigma[l], smax = r.sigma_ratio * smin;
+    const int nT = g.n_T[l];
+    const double *coe = g.coe + 18 * sg.sec;
+    const double sfirst = g.czl[sg.sec];
+    const double dzeta = r.dzeta_sup < g.dtf[sg.sec] ? r.dzeta_sup : g.dtf[sg.sec];
+    double X[3] = {mesh(h.ip, r), mesh(h.jp, r), mesh(h.kp, r)};
+    double a0 = X[0] - sg.ci0[0], a1 = X[1] - sg.ci0[1], a2 = X[2] - sg.ci0[2];
+    double t0 = (a0 * sg.tci0[0] + a1 * sg.tci0[1]) + a2 * sg.tci0[2];
+    double t1 = ((X[0] - sg.ci1[0]) * sg.tci0[0] + (X[1] - sg.ci1[1]) * sg.tci0[1]) +
+                (X[2] - sg.ci1[2]) * sg.tci0[2];
+    double zeta = (sg.s1 * t0 - sg.s0 * t1) / sg.ndci0;
+    double zsec = zeta - sfirst;
+    double cz3[3], dc[3], ddc[3], cr[3], xc[3], T[3];
+    spline_pos(zsec * LD, coe, cz3);
+    spline_d1(zsec * LD, coe, dc);
+    double ndc = norm3(dc);
+#pragma unroll
+    for (int c = 0; c < 3; c++) T[c] = dc[c] / ndc;
+    spline_d2(zsec * LD, coe, ddc);
+    cr[0] = dc[1] * ddc[2] - dc[2] * ddc[1];
+    cr[1] = dc[2] * ddc[0] - dc[0] * ddc[2];
+    cr[2] = dc[0] * ddc[1] - dc[1] * ddc[0];
+    double ncr = norm3(cr);
+    cminus(X, cz3, xc);
+    double nxc = norm3(xc);
+    double rho = nxc;
+    double sigma = sigma_func(zeta, len, smax, smin, nT);
+    double tp = exp(-(p2(rho - 0.0) / 2.0 / p2(sigma))) / (kPi2 * p2(sigma));
+    double ftd = -exp(-(p2(sg.R) / 2.0 / p2(sigma))) / (kPi2 * p2(sigma));
+    tp = tp + ftd;
+    double sl = sigma_func(zeta - dzeta, len, smax, smin, nT);
+    double sr = sigma_func(zeta + dzeta, len, smax, smin, nT);
+    double dsds = (sr - sl) / (2.0 * dzeta);
+    dsds = dsds / ndc / LD;
+    const double eta = r.eta;
+    double tg = tp * Gamma;
+    double v[3];
+    if (ncr < 1.0e-8) {
+        double er[3], eth[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) er[c] = xc[c] / (nxc + 1.0e-15);
+        eth[0] = er[1] * T[2] - er[2] * T[1];
+        eth[1] = er[2] * T[0] - er[0] * T[2];
+        eth[2] = er[0] * T[1] - er[1] * T[0];
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] = tg * ((T[c] + eta * rho * eth[c]) + (rho * dsds / sigma) * er[c]);
+    } else {
+        double B[3], Nv[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) B[c] = cr[c] / ncr;
+        Nv[0] = B[1] * T[2] - B[2] * T[1];
+        Nv[1] = B[2] * T[0] - B[0] * T[2];
+        Nv[2] = B[0] * T[1] - B[1] * T[0];
+        double kap = ncr / p3(ndc);
+        double costh = ((xc[0] * Nv[0] + xc[1] * Nv[1]) + xc[2] * Nv[2]) / (nxc + 1.0e-15);
+        double sinth = ((xc[0] * B[0] + xc[1] * B[1]) + xc[2] * B[2]) / (nxc + 1.0e-15);
+        double D = 1.0 - kap * rho * costh;
+        double t5 = -(sinth * rho / D);
+        double t6 = costh * rho / D;
+        double t7 = costh * rho * dsds / sigma / D;
+        double t8 = sinth * rho * dsds / sigma / D;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            v[c] = tg * ((T[c] + eta * (t5 * Nv[c] + t6 * B[c])) + (t7 * Nv[c] + t8 * B[c]));
+    }
+    const int N = r.N;
+    int i = wrap_idx(h.ip, N), j = wrap_idx(h.jp, N), k = wrap_idx(h.kp, N);
+    int64_t idx = (int64_t)(i - 1) + (int64_t)N * (j - 1) + (int64_t)N * N * (k - 1);
+    int64_t N3 = (int64_t)N * N * N;
+    double *om = (double *)A.omega;
+    unsafeAtomicAdd(om + idx, v[0]);
+    unsafeAtomicAdd(om + N3 + idx, v[1]);
+    unsafeAtomicAdd(om + 2 * N3 + idx, v[2]);
+}
+
+// ---- hit evaluation, FP32 core in local coordinates -------------------------------------
+__device__ void eval_hit_f32(const DepArgs &A, const Hit &h) {
+    const Geo &g = A.g;
+    const Run &r = A.r;
+    SubSeg sg;
+    subseg_load(g, r, h.q, sg);
+    const int l = sg.tube;
+    const double LD = g.Len_Disc[l];
+    const double *coe = g.coe + 18 * sg.sec;
+    // FP64: cell offset from the sub-segment start, spline parameter
+    double X[3] = {mesh(h.ip, r), mesh(h.jp, r), mesh(h.kp, r)};
+    double a[3] = {X[0] - sg.ci0[0], X[1] - sg.ci0[1], X[2] - sg.ci0[2]};
+    double t0 = (a[0] * sg.tci0[0] + a[1] * sg.tci0[1]) + a[2] * sg.tci0[2];
+    double t1 = ((X[0] - sg.ci1[0]) * sg.tci0[0] + (X[1] - sg.ci1[1]) * sg.tci0[1]) +
+                (X[2] - sg.ci1[2]) * sg.tci0[2];
+    double zeta = (sg.s1 * t0 - sg.s0 * t1) / sg.ndci0;
+    double zz = (zeta - g.czl[sg.sec]) * LD;
+    float d0[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {  // spline constant term relative to ci0, nearest periodic image
+        double d = coe[6 * c] - sg.ci0[c];
+        d = d - kPi2 * rint(d / kPi2);
+        d0[c] = (float)d;
+    }
+    float xl[3] = {(float)a[0], (float)a[1], (float)a[2]};
+    const float Gamma = (float)g.gamma[l];
+    const double smin = g.sigma[l];
+    float v[3];
+    {
+#pragma clang fp contract(fast)
+        float z = (float)zz;
+        float cl[3], dc[3], ddc[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float a1 = (float)coe[6 * c + 1], a2 = (float)coe[6 * c + 2], a3 = (float)coe[6 * c + 3];
+            float a4 = (float)coe[6 * c + 4], a5 = (float)coe[6 * c + 5];
+            cl[c] = d0[c] + z * (a1 + z * (a2 + z * (a3 + z * (a4 + z * a5))));
+            dc[c] = a1 + z * (2.0f * a2 + z * (3.0f * a3 + z * (4.0f * a4 + z * (5.0f * a5))));
+            ddc[c] = 2.0f * a2 + z * (6.0f * a3 + z * (12.0f * a4 + z * (20.0f * a5)));
+        }
+        float xc[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float d = xl[c] - cl[c];
+            if (d > (float)kPi) d -= (float)kPi2;
+            if (d < -(float)kPi) d += (float)kPi2;
+            xc[c] = d;
+        }
+        float rho2 = xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2];
+        float ndc = sqrtf(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]);
+        float indc = 1.0f / ndc;
+        float T[3] = {dc[0] * indc, dc[1] * indc, dc[2] * indc};
+        float sigma;
+        double sigma_d = smin;
+        if (r.sigma_ratio != 1.0) {
+            sigma_d = sigma_func(zeta, g.length[l], r.sigma_ratio * smin, smin, g.n_T[l]);
+        }
+        sigma = (float)sigma_d;
+        float is2 = 1.0f / (sigma * sigma);
+        float R = (float)sg.R;
+        float norm = (float)(1.0 / (kPi2 * sigma_d * sigma_d));
+        float tp = (__expf(-0.5f * rho2 * is2) - __expf(-0.5f * R * R * is2)) * norm;
+        float tg = tp * Gamma;
+        if (r.eta == 0.0 && r.sigma_ratio == 1.0) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) v[c] = tg * T[c];
+        } else {
+            float dsds = 0.0f;
+            if (r.sigma_ratio != 1.0) {
+                const double dzeta = r.dzeta_sup < g.dtf[sg.sec] ? r.dzeta_sup : g.dtf[sg.sec];
+                double sl = sigma_func(zeta - dzeta, g.length[l], r.sigma_ratio * smin, smin, g.n_T[l]);
+                double sr = sigma_func(zeta + dzeta, g.length[l], r.sigma_ratio * smin, smin, g.n_T[l]);
+                dsds = (float)((sr - sl) / (2.0 * dzeta) / LD) * indc;
+            }
+            const float eta = (float)r.eta;
+            float rho = sqrtf(rho2);
+            float cr[3] = {dc[1] * ddc[2] - dc[2] * ddc[1], dc[2] * ddc[0] - dc[0] * ddc[2],
+                           dc[0] * ddc[1] - dc[1] * ddc[0]};
+            float ncr = sqrtf(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+            float inx = 1.0f / (rho + 1.0e-15f);
+            float sr_ = rho * dsds / sigma;
+            if (ncr < 1.0e-8f) {
+                float er[3] = {xc[0] * inx, xc[1] * inx, xc[2] * inx};
+                float eth[3] = {er[1] * T[2] - er[2] * T[1], er[2] * T[0] - er[0] * T[2],
+                                er[0] * T[1] - er[1] * T[0]};
+#pragma unroll
+                for (int c = 0; c < 3; c++) v[c] = tg * (T[c] + eta * rho * eth[c] + sr_ * er[c]);
+            } else {
+                float incr = 1.0f / ncr;
+                float B[3] = {cr[0] * incr, cr[1] * incr, cr[2] * incr};
+                float Nv[3] = {B[1] * T[2] - B[2] * T[1], B[2] * T[0] - B[0] * T[2], B[0] * T[1] - B[1] * T[0]};
+                float kap = ncr * indc * indc * indc;
+                float costh = (xc[0] * Nv[0] + xc[1] * Nv[1] + xc[2] * Nv[2]) * inx;
+                float sinth = (xc[0] * B[0] + xc[1] * B[1] + xc[2] * B[2]) * inx;
+                float iD = 1.0f / (1.0f - kap * rho * costh);
+                float t5 = -sinth * rho * iD, t6 = costh * rho * iD;
+                float t7 = costh * sr_ * iD, t8 = sinth * sr_ * iD;
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    v[c] = tg * (T[c] + eta * (t5 * Nv[c] + t6 * B[c]) + (t7 * Nv[c] + t8 * B[c]));
+            }
+        }
+    }
+    const int N = r.N;
+    int i = wrap_idx(h.ip, N), j = wrap_idx(h.jp, N), k = wrap_idx(h.kp, N);
+    int64_t idx = (int64_t)(i - 1) + (int64_t)N * (j - 1) + (int64_t)N * N * (k - 1);
+    int64_t N3 = (int64_t)N * N * N;
+    bool ovf = false;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        double s = (double)v[c] * A.fx_scale;
+        if (!(fabs(s) < A.fx_limit)) { ovf = true; s = 0.0; }
+        long long fx = (long long)rint(s);
+        atomicAdd(A.acc + c * N3 + idx, (unsigned long long)fx);
+    }
+    if (ovf) atomicAdd(A.counters + 3, 1ull);
+}
+
+template <bool F64>
+__device__ __forceinline__ void drain(const DepArgs &A, Hit *queue, int &qn, int lane) {
+    // evaluate the top 64 entries (all lanes busy)
+    Hit h = queue[qn - 64 + lane];
+    __builtin_amdgcn_wave_barrier();
+    if (F64) eval_hit_f64(A, h); else eval_hit_f32(A, h);
+    qn -= 64;
+}
+
+template <bool F64>
+__global__ __launch_bounds__(kWaves * 64) void k_deposit(DepArgs A) {
+    __shared__ Hit qbuf[kWaves][kQCap];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    Hit *queue = qbuf[wv];
+    int qn = 0;
+    const Geo &g = A.g;
+    const Run &r = A.r;
+    unsigned long long ncols_seen = 0, ncand = 0, nhit = 0;
+    const int64_t items = A.b.items;
+    const int64_t stride = (int64_t)gridDim.x * kWaves * 64;
+    for (int64_t base = ((int64_t)blockIdx.x * kWaves + wv) * 64; base < items; base += stride) {
+        int64_t it = base + lane;
+        int ncl = 0;
+        SubSeg sg;
+        int ax = 0, bx = 1, cxs = 2;
+        int64_t c_begin = 0;
+        int32_t nb = 1;
+        double ia0 = 0, ia1 = 0, A0b = 0, A0c = 0, A1b = 0, A1c = 0;
+        double s0sgn = 1.0, s1sgn = 0.0;
+        if (it < items) {
+            uint32_t q = A.b.item_q[it];
+            int64_t blk = it - A.b.item_off[q];
+            int64_t ncol = A.b.col_count[q];
+            subseg_load(g, r, q, sg);
+            c_begin = blk * kColsPerItemDep;
+            int64_t c_end = c_begin + kColsPerItemDep < ncol ? c_begin + kColsPerItemDep : ncol;
+            ncl = (int)(c_end - c_begin);
+            ax = dominant_axis(sg.tci0);
+            bx = (ax + 1) % 3;
+            cxs = (ax + 2) % 3;
+            nb = sg.hi[bx] - sg.lo[bx] + 1;
+            // plane index functions: ia*(b, c) = ia0 - (A0b*(Xb - ci0_b) + A0c*(Xc - ci0_c))
+            double t0a = sg.tci0[ax], t1a = sg.tci1[ax];
+            ia0 = (sg.ci0[ax] - r.start) / r.dx + 1.0;
+            A0b = sg.tci0[bx] / (t0a * r.dx);
+            A0c = sg.tci0[cxs] / (t0a * r.dx);
+            s0sgn = t0a > 0.0 ? 1.0 : -1.0;
+            if (t1a != 0.0) {
+                ia1 = (sg.ci1[ax] - r.start) / r.dx + 1.0;
+                A1b = sg.tci1[bx] / (t1a * r.dx);
+                A1c = sg.tci1[cxs] / (t1a * r.dx);
+                s1sgn = t1a > 0.0 ? 1.0 : -1.0;
+            } else {
+                s1sgn = 0.0;
+            }
+        }
+        ncols_seen += (unsigned long long)ncl;
+        const int maxc = wave_max_i(ncl);
+        int64_t kcol = c_begin;
+        for (int cc = 0; cc < maxc; cc++, kcol++) {
+            int alo = 1, ahi = 0;
+            int32_t bi = 0, ci_ = 0;
+            if (cc < ncl) {
+                bi = sg.lo[bx] + (int32_t)(kcol % nb);
+                ci_ = sg.lo[cxs] + (int32_t)(kcol / nb);
+                double Xb = mesh(bi, r), Xc = mesh(ci_, r);
+                const double lo_a = sg.lo[ax], hi_a = sg.hi[ax];
+                double flo = lo_a, fhi = hi_a;
+                double p0 = ia0 - (A0b * (Xb - sg.ci0[bx]) + A0c * (Xc - sg.ci0[cxs]));
+                p0 = fmin(fmax(p0, lo_a - 2.0), hi_a + 2.0);
+                if (s0sgn > 0.0) flo = fmax(flo, ceil(p0 - 1e-6));
+                else fhi = fmin(fhi, floor(p0 + 1e-6));
+                if (s1sgn != 0.0) {
+                    double p1 = ia1 - (A1b * (Xb - sg.ci1[bx]) + A1c * (Xc - sg.ci1[cxs]));
+                    p1 = fmin(fmax(p1, lo_a - 2.0), hi_a + 2.0);
+                    if (s1sgn > 0.0) fhi = fmin(fhi, floor(p1 + 1e-6));
+                    else flo = fmax(flo, ceil(p1 - 1e-6));
+                }
+                alo = (int)flo;
+                ahi = (int)fhi;
+            }
+            int nc = ahi >= alo ? ahi - alo + 1 : 0;
+            ncand += (unsigned long long)nc;
+            const int maxn = wave_max_i(nc);
+            for (int t = 0; t < maxn; t++) {
+                bool hit = false;
+                int32_t ijk[3];
+                if (t < nc) {
+                    ijk[ax] = alo + t;
+                    ijk[bx] = bi;
+                    ijk[cxs] = ci_;
+                    hit = pair_hits(sg, r, ijk[0], ijk[1], ijk[2]);
+                }
+                unsigned long long m = __ballot(hit);
+                if (m) {
+                    int pos = qn + __popcll(m & lanemask_lt());
+                    if (hit) {
+                        Hit h;
+                        h.q = A.b.item_q[it];
+                        h.ip = (int16_t)ijk[0];
+                        h.jp = (int16_t)ijk[1];
+                        h.kp = (int16_t)ijk[2];
+                        h.pad = 0;
+                        queue[pos] = h;
+                    }
+                    int cnt = __popcll(m);
+                    qn += cnt;
+                    nhit += (lane == 0) ? (unsigned long long)cnt : 0ull;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (qn >= 64) drain<F64>(A, queue, qn, lane);
+                }
+            }
+        }
+    }
+    // flush the partial queue
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < qn) {
+        Hit h = queue[lane];
+        if (F64) eval_hit_f64(A, h); else eval_hit_f32(A, h);
+    }
+    // counters: one atomic per wave
+    unsigned long long cols = ncols_seen, cands = ncand;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        cols += __shfl_xor(cols, o, 64);
+        cands += __shfl_xor(cands, o, 64);
+    }
+    if (lane == 0) {
+        atomicAdd(A.counters + 0, cols);
+        atomicAdd(A.counters + 1, cands);
+        atomicAdd(A.counters + 2, nhit);
+    }
+}
+
+__global__ void k_fixed_to_float(const unsigned long long *__restrict__ acc, float *__restrict__ out,
+                                 int64_t n, double inv_scale) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = (float)((double)(long long)acc[i] * inv_scale);
+}
+
+hipError_t launch_deposit(wt_ctx *c) {
+    hipStream_t st = c->stream;
+    const int64_t N3 = (int64_t)c->N * c->N * c->N;
+    const bool f64 = c->precision == 64;
+    unsigned long long *cnt = nullptr;
+    WT_HIP(hipMalloc(&cnt, 4 * sizeof(unsigned long long)));
+    WT_HIP(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st));
+    DepArgs A{};
+    A.g = c->geo;
+    A.r = c->run;
+    A.b = c->bins;
+    A.omega = c->omega;
+    A.acc = (unsigned long long *)c->acc;
+    A.scal = c->scal;
+    A.counters = cnt;
+    if (f64) {
+        WT_HIP(hipMemsetAsync(c->omega, 0, sizeof(double) * 3 * N3, st));
+    } else {
+        WT_HIP(hipMemsetAsync(c->acc, 0, sizeof(long long) * 3 * N3, st));
+        A.fx_scale = ldexp(1.0, c->fixed_shift);
+        A.fx_limit = ldexp(1.0, 62) / 64.0;
+    }
+    int dev_cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) dev_cus = prop.multiProcessorCount;
+    int64_t waves_needed = (c->bins.items + 63) / 64;
+    int64_t blocks = (waves_needed + kWaves - 1) / kWaves;
+    int64_t cap = (int64_t)dev_cus * 16;
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+    kt_begin(c, kSlotDeposit);
+    if (f64) hipLaunchKernelGGL(k_deposit<true>, dim3((unsigned)blocks), dim3(kWaves * 64), 0, st, A);
+    else hipLaunchKernelGGL(k_deposit<false>, dim3((unsigned)blocks), dim3(kWaves * 64), 0, st, A);
+    kt_end(c, kSlotDeposit);
+    WT_HIP(hipGetLastError());
+    if (!f64) {
+        kt_begin(c, kSlotFixedToFloat);
+        hipLaunchKernelGGL(k_fixed_to_float, dim3(4096), dim3(256), 0, st,
+                           (const unsigned long long *)c->acc, (float *)c->omega, 3 * N3,
+                           ldexp(1.0, -c->fixed_shift));
+        kt_end(c, kSlotFixedToFloat);
+        WT_HIP(hipGetLastError());
+    }
+    unsigned long long h[4];
+    WT_HIP(hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, st));
+    WT_HIP(hipStreamSynchronize(st));
+    (void)hipFree(cnt);
+    c->dep_stats.columns = (int64_t)h[0];
+    c->dep_stats.candidates = (int64_t)h[1];
+    c->dep_stats.hits = (int64_t)h[2];
+    c->dep_stats.overflow = (int32_t)(h[3] > 0);
+    c->dep_stats.fixed_point_shift = f64 ? 0 : c->fixed_shift;
+    return hipSuccess;
+}
+
+}  // namespace wt

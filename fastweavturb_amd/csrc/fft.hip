@@ -1,0 +1,536 @@
+// Spectral Biot-Savart inversion (tube.f90:985-1043, subroutine.f90:888-919, 983-1084,
+// 1550-1569, 1604-1638) with a hand-written pencil FFT.
+//
+// Transform layout: the half spectrum (kx = 0..N/2) of each component, spec[c][z][y][kx]
+// with row stride N/2+1, complex<T>. Passes:
+//   x : R2C of two real lines at once packed as one complex line, scaled by 1/N^3
+//   y : C2C along y on tiles of W kx-columns staged in LDS
+//   z : C2C forward along z, then u_hat = i k~ x w_hat / k^2 (k~ = k with Nyquist components
+//       zeroed, k^2 unchanged; this equals the real part the reference keeps, see below),
+//       spectra of w_hat and u_hat into shells, Parseval sums, C2C inverse along z
+//   y : inverse;  x : C2R, scaled by 1/up, and omega scaled by 1/up in the same pass.
+// Why the Nyquist rule: the reference transforms real omega with full complex FFTs
+// (wavenumber index N/2 maps to -N/2) and keeps dreal() of the inverse (subroutine.f90:1083).
+// The real part equals the Hermitian projection (X(k) + conj X(-k))/2 of the spectrum; at a
+// Nyquist index the partner -k has the same wavenumber -N/2, and the projection works out to
+// zeroing that wavenumber component in the numerator while keeping it in k^2.
+//
+// The 1-D FFTs run in LDS: iterative in-place decimation in time over radices 8/4/3/2, data
+// loaded in mixed-radix digit-reversed order, twiddles from a table exp(-2 pi i k / N).
+#include "wt_internal.h"
+
+namespace wt {
+
+template <typename T> struct C2 { T x, y; };
+
+template <typename T>
+__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <typename T>
+__device__ __forceinline__ C2<T> cadd(C2<T> a, C2<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ C2<T> csub(C2<T> a, C2<T> b) { return {a.x - b.x, a.y - b.y}; }
+// multiply by -i (forward) or +i (inverse)
+template <typename T, bool INV>
+__device__ __forceinline__ C2<T> mul_mi(C2<T> a) { return INV ? C2<T>{-a.y, a.x} : C2<T>{a.y, -a.x}; }
+
+struct FftPlan {
+    int N;
+    int npass;
+    int radix[12];
+    const int32_t *rev;   // digit-reversal permutation, N entries
+    const void *tw;       // N complex twiddles exp(-2 pi i k / N) of type C2<T>
+};
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft4(C2<T> *v) {
+    C2<T> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    C2<T> s13 = cadd(v[1], v[3]), d13 = mul_mi<T, INV>(csub(v[1], v[3]));
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = cadd(d02, d13);
+    v[3] = csub(d02, d13);
+}
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft_small(C2<T> *v) {
+    if (R == 2) {
+        C2<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if (R == 3) {
+        const T c1 = (T)-0.5, s1 = (T)(INV ? 0.86602540378443864676 : -0.86602540378443864676);
+        C2<T> a = v[0], b = v[1], c = v[2];
+        C2<T> s = cadd(b, c), d = csub(b, c);
+        v[0] = cadd(a, s);
+        C2<T> m = {a.x + c1 * s.x, a.y + c1 * s.y};
+        C2<T> r = {-s1 * d.y, s1 * d.x};  // i*s1*d
+        v[1] = cadd(m, r);
+        v[2] = csub(m, r);
+    } else if (R == 4) {
+        dft4<T, INV>(v);
+    } else if (R == 8) {
+        C2<T> e[4] = {v[0], v[2], v[4], v[6]};
+        C2<T> o[4] = {v[1], v[3], v[5], v[7]};
+        dft4<T, INV>(e);
+        dft4<T, INV>(o);
+        const T h = (T)0.70710678118654752440;
+        // w8^k for k = 0..3 (forward: exp(-i pi k / 4))
+        C2<T> w1 = {h, INV ? h : -h};
+        C2<T> o1 = cmul(o[1], w1);
+        C2<T> o2 = mul_mi<T, INV>(o[2]);
+        C2<T> w3 = {-h, INV ? h : -h};
+        C2<T> o3 = cmul(o[3], w3);
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+}
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void fft_pass(C2<T> *buf, int nlines, int ldb, int N, int L,
+                                         const C2<T> *__restrict__ tw) {
+    const int nbf = N / R;
+    const int total = nlines * nbf;
+    const int step = N / (L * R);
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        int line = t / nbf;
+        int b = t - line * nbf;
+        int g = b / L;
+        int k = b - g * L;
+        C2<T> *p = buf + line * ldb + g * L * R + k;
+        C2<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = p[r * L];
+        if (L > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                C2<T> w = tw[k * r * step];
+                if (INV) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        dft_small<T, R, INV>(v);
+#pragma unroll
+        for (int r = 0; r < R; r++) p[r * L] = v[r];
+    }
+}
+
+// In-place FFT of nlines lines stored at buf + line*ldb, input already digit-reversed.
+template <typename T, bool INV>
+__device__ void fft_lds(C2<T> *buf, int nlines, int ldb, const FftPlan &pl) {
+    const C2<T> *tw = (const C2<T> *)pl.tw;
+    int L = 1;
+    for (int p = 0; p < pl.npass; p++) {
+        int R = pl.radix[p];
+        switch (R) {
+            case 8: fft_pass<T, 8, INV>(buf, nlines, ldb, pl.N, L, tw); break;
+            case 4: fft_pass<T, 4, INV>(buf, nlines, ldb, pl.N, L, tw); break;
+            case 3: fft_pass<T, 3, INV>(buf, nlines, ldb, pl.N, L, tw); break;
+            default: fft_pass<T, 2, INV>(buf, nlines, ldb, pl.N, L, tw); break;
+        }
+        __syncthreads();
+        L *= R;
+    }
+}
+
+// Decimation in frequency: natural-order input, digit-reversed output (read back through
+// rev). The same butterflies as fft_lds in reverse pass order, twiddles after the small DFT.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void fft_pass_dif(C2<T> *buf, int nlines, int ldb, int N, int L,
+                                             const C2<T> *__restrict__ tw) {
+    const int nbf = N / R;
+    const int total = nlines * nbf;
+    const int step = N / (L * R);
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        int line = t / nbf;
+        int b = t - line * nbf;
+        int g = b / L;
+        int k = b - g * L;
+        C2<T> *p = buf + line * ldb + g * L * R + k;
+        C2<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = p[r * L];
+        dft_small<T, R, INV>(v);
+        if (L > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                C2<T> w = tw[k * r * step];
+                if (INV) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) p[r * L] = v[r];
+    }
+}
+
+template <typename T, bool INV>
+__device__ void fft_lds_dif(C2<T> *buf, int nlines, int ldb, const FftPlan &pl) {
+    const C2<T> *tw = (const C2<T> *)pl.tw;
+    int L = pl.N;
+    for (int p = pl.npass - 1; p >= 0; p--) {
+        int R = pl.radix[p];
+        L /= R;
+        switch (R) {
+            case 8: fft_pass_dif<T, 8, INV>(buf, nlines, ldb, pl.N, L, tw); break;
+            case 4: fft_pass_dif<T, 4, INV>(buf, nlines, ldb, pl.N, L, tw); break;
+            case 3: fft_pass_dif<T, 3, INV>(buf, nlines, ldb, pl.N, L, tw); break;
+            default: fft_pass_dif<T, 2, INV>(buf, nlines, ldb, pl.N, L, tw); break;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- x pass, forward: two real lines -> two half spectra --------------------------------
+template <typename T>
+__global__ void k_fft_x_r2c(const T *__restrict__ f, C2<T> *__restrict__ spec, FftPlan pl, int pairs_per_block,
+                            int64_t npairs, T scale) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<T> *buf = (C2<T> *)smem_raw;
+    const int N = pl.N, H = N / 2 + 1;
+    const int64_t pair0 = (int64_t)blockIdx.x * pairs_per_block;
+    int np = (int)((npairs - pair0) < pairs_per_block ? (npairs - pair0) : pairs_per_block);
+    for (int i = threadIdx.x; i < np * N; i += blockDim.x) {
+        int pp = i / N, x = i - pp * N;
+        int64_t line = 2 * (pair0 + pp);
+        T a = f[line * N + x], b = f[(line + 1) * N + x];
+        buf[pp * N + pl.rev[x]] = C2<T>{a, b};
+    }
+    __syncthreads();
+    fft_lds<T, false>(buf, np, N, pl);
+    for (int i = threadIdx.x; i < np * H; i += blockDim.x) {
+        int pp = i / H, k = i - pp * H;
+        C2<T> z = buf[pp * N + (k == N / 2 ? N / 2 : k)];
+        C2<T> zc = buf[pp * N + ((N - k) % N)];
+        // A = (Z(k) + conj Z(N-k)) / 2, B = (Z(k) - conj Z(N-k)) / (2i)
+        C2<T> A = {(z.x + zc.x) * (T)0.5, (z.y - zc.y) * (T)0.5};
+        C2<T> B = {(z.y + zc.y) * (T)0.5, (zc.x - z.x) * (T)0.5};
+        int64_t line = 2 * (pair0 + pp);
+        spec[line * H + k] = C2<T>{A.x * scale, A.y * scale};
+        spec[(line + 1) * H + k] = C2<T>{B.x * scale, B.y * scale};
+    }
+}
+
+// ---- x pass, inverse: two half spectra -> two real lines, times inv_up; omega too --------
+template <typename T>
+__global__ void k_fft_x_c2r(const C2<T> *__restrict__ spec, T *__restrict__ u, T *__restrict__ omega, FftPlan pl,
+                            int pairs_per_block, int64_t npairs, const DeviceScalars *__restrict__ sc) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<T> *buf = (C2<T> *)smem_raw;
+    const int N = pl.N, H = N / 2 + 1;
+    const int64_t pair0 = (int64_t)blockIdx.x * pairs_per_block;
+    int np = (int)((npairs - pair0) < pairs_per_block ? (npairs - pair0) : pairs_per_block);
+    for (int i = threadIdx.x; i < np * N; i += blockDim.x) {
+        int pp = i / N, k = i - pp * N;
+        int64_t line = 2 * (pair0 + pp);
+        int kk = k <= N / 2 ? k : N - k;
+        C2<T> A = spec[line * H + kk], B = spec[(line + 1) * H + kk];
+        if (k == 0 || k == N / 2) { A.y = 0; B.y = 0; }
+        if (k > N / 2) { A.y = -A.y; B.y = -B.y; }   // Hermitian extension
+        // Z = A + i B
+        buf[pp * N + k] = C2<T>{A.x - B.y, A.y + B.x};
+    }
+    __syncthreads();
+    fft_lds_dif<T, true>(buf, np, N, pl);
+    const T s = (T)sc->inv_up;
+    for (int i = threadIdx.x; i < np * N; i += blockDim.x) {
+        int pp = i / N, x = i - pp * N;
+        int64_t line = 2 * (pair0 + pp);
+        C2<T> z = buf[pp * N + pl.rev[x]];
+        u[line * N + x] = z.x * s;
+        u[(line + 1) * N + x] = z.y * s;
+        if (omega) {
+            omega[line * N + x] *= s;
+            omega[(line + 1) * N + x] *= s;
+        }
+    }
+}
+
+// ---- y pass: tiles of W kx columns x N y values ------------------------------------------
+template <typename T, bool INV>
+__global__ void k_fft_y(C2<T> *__restrict__ spec, FftPlan pl, int W, int nchunk) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<T> *buf = (C2<T> *)smem_raw;
+    const int N = pl.N, H = N / 2 + 1;
+    int chunk = blockIdx.x % nchunk;
+    int64_t plane = blockIdx.x / nchunk;  // c*N + z
+    int kx0 = chunk * W;
+    int w_valid = H - kx0 < W ? H - kx0 : W;
+    C2<T> *base = spec + plane * (int64_t)N * H;
+    for (int i = threadIdx.x; i < W * N; i += blockDim.x) {
+        int w = i % W, y = i / W;
+        if (w < w_valid) buf[w * N + (INV ? y : pl.rev[y])] = base[(int64_t)y * H + kx0 + w];
+    }
+    __syncthreads();
+    if (INV) fft_lds_dif<T, true>(buf, w_valid, N, pl);
+    else fft_lds<T, false>(buf, w_valid, N, pl);
+    for (int i = threadIdx.x; i < W * N; i += blockDim.x) {
+        int w = i % W, y = i / W;
+        if (w < w_valid) base[(int64_t)y * H + kx0 + w] = buf[w * N + (INV ? pl.rev[y] : y)];
+    }
+}
+
+__device__ __forceinline__ int wavenum(int i, int N) { return ((i + N / 2) % N) - N / 2; }
+
+// Shell of integer k2: ik = int(sqrt(k2 + 1e-8) + 0.5) = the m with (m-1/2)^2 <= k2 < (m+1/2)^2
+__device__ __forceinline__ int shell_of(int64_t k2) {
+    int m = (int)sqrt((double)k2);
+    while ((int64_t)(2 * m + 1) * (2 * m + 1) <= 4 * k2) m++;
+    while (m > 0 && (int64_t)(2 * m - 1) * (2 * m - 1) > 4 * k2) m--;
+    return m;
+}
+
+// ---- z pass: forward, Biot-Savart (mode 0) or spectrum only (mode 1), inverse ------------
+// partials row per block: [0, nek) shells of w_hat (mode 0) | [nek, 2nek) shells of u_hat |
+// [2nek] sum |w_hat|^2 weighted | [2nek+1] sum |u_hat|^2 weighted
+template <typename T>
+__global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl, int W, int nchunk, int mode, int nek,
+                        double *__restrict__ partials) {
+    extern __shared__ unsigned char smem_raw[];
+    C2<T> *buf = (C2<T> *)smem_raw;                       // 3 * W * N
+    double *shell = (double *)(buf + 3 * W * pl.N);       // 2 * nek + 2
+    const int N = pl.N, H = N / 2 + 1;
+    const int64_t N3h = (int64_t)N * N * H;
+    int chunk = blockIdx.x % nchunk;
+    int y = blockIdx.x / nchunk;
+    int kx0 = chunk * W;
+    int w_valid = H - kx0 < W ? H - kx0 : W;
+    for (int i = threadIdx.x; i < 2 * nek + 2; i += blockDim.x) shell[i] = 0.0;
+    for (int i = threadIdx.x; i < 3 * W * N; i += blockDim.x) {
+        int w = i % W, rest = i / W;
+        int z = rest % N, c = rest / N;
+        C2<T> v = {0, 0};
+        if (w < w_valid) v = spec[c * N3h + ((int64_t)z * N + y) * H + kx0 + w];
+        buf[(c * W + w) * N + pl.rev[z]] = v;
+    }
+    __syncthreads();
+    fft_lds<T, false>(buf, 3 * W, N, pl);
+    // pointwise: (w, kz) pairs
+    const int ky = wavenum(y, N);
+    double e_w = 0.0, e_u = 0.0;
+    for (int i = threadIdx.x; i < w_valid * N; i += blockDim.x) {
+        int w = i / N, z = i - w * N;
+        int kxi = kx0 + w;
+        int kx = wavenum(kxi, N), kz = wavenum(z, N);
+        int64_t k2 = (int64_t)kx * kx + (int64_t)ky * ky + (int64_t)kz * kz;
+        double wt = (kxi == 0 || kxi == N / 2) ? 1.0 : 2.0;   // half-spectrum multiplicity
+        C2<T> a = buf[(0 * W + w) * N + z], b = buf[(1 * W + w) * N + z], c = buf[(2 * W + w) * N + z];
+        double ew = (double)a.x * a.x + (double)a.y * a.y + (double)b.x * b.x + (double)b.y * b.y +
+                    (double)c.x * c.x + (double)c.y * c.y;
+        int sh = shell_of(k2);
+        if (mode == 0) {
+            e_w += wt * ew;
+            if (sh >= 1 && sh <= nek) atomicAdd(shell + sh - 1, 0.5 * wt * ew);
+            // u_hat = i (k~ x w_hat) / k2, Nyquist components of the numerator zeroed
+            T qx = (kxi == N / 2) ? (T)0 : (T)kx;
+            T qy = (y == N / 2) ? (T)0 : (T)ky;
+            T qz = (z == N / 2) ? (T)0 : (T)kz;
+            C2<T> ux, uy, uz;
+            if (k2 == 0) {
+                ux = uy = uz = C2<T>{0, 0};
+            } else {
+                T ik2 = (T)(1.0 / (double)k2);
+                // cross = q x w
+                C2<T> cx = {qy * c.x - qz * b.x, qy * c.y - qz * b.y};
+                C2<T> cy = {qz * a.x - qx * c.x, qz * a.y - qx * c.y};
+                C2<T> cz = {qx * b.x - qy * a.x, qx * b.y - qy * a.y};
+                ux = C2<T>{-cx.y * ik2, cx.x * ik2};
+                uy = C2<T>{-cy.y * ik2, cy.x * ik2};
+                uz = C2<T>{-cz.y * ik2, cz.x * ik2};
+            }
+            double eu = (double)ux.x * ux.x + (double)ux.y * ux.y + (double)uy.x * uy.x + (double)uy.y * uy.y +
+                        (double)uz.x * uz.x + (double)uz.y * uz.y;
+            e_u += wt * eu;
+            if (sh >= 1 && sh <= nek) atomicAdd(shell + nek + sh - 1, 0.5 * wt * eu);
+            buf[(0 * W + w) * N + z] = ux;
+            buf[(1 * W + w) * N + z] = uy;
+            buf[(2 * W + w) * N + z] = uz;
+        } else {
+            e_u += wt * ew;
+            if (sh >= 1 && sh <= nek) atomicAdd(shell + nek + sh - 1, 0.5 * wt * ew);
+        }
+    }
+    // block sums of the two energies
+    atomicAdd(shell + 2 * nek, e_w);
+    atomicAdd(shell + 2 * nek + 1, e_u);
+    __syncthreads();
+    double *row = partials + (int64_t)blockIdx.x * (2 * nek + 2);
+    for (int i = threadIdx.x; i < 2 * nek + 2; i += blockDim.x) row[i] = shell[i];
+    if (mode != 0) return;
+    __syncthreads();
+    fft_lds_dif<T, true>(buf, 3 * W, N, pl);
+    for (int i = threadIdx.x; i < 3 * W * N; i += blockDim.x) {
+        int w = i % W, rest = i / W;
+        int z = rest % N, c = rest / N;
+        if (w < w_valid)
+            spec[c * N3h + ((int64_t)z * N + y) * H + kx0 + w] = buf[(c * W + w) * N + pl.rev[z]];
+    }
+}
+
+// Reduce the per-block partial rows (columns are independent), then derive up and the
+// normalised spectra.
+__global__ void k_reduce_columns(const double *__restrict__ partials, int64_t rows, int cols,
+                                 double *__restrict__ out) {
+    int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= cols) return;
+    double s = 0.0;
+    for (int64_t r = 0; r < rows; r++) s += partials[r * cols + col];
+    out[col] = s;
+}
+
+__global__ void k_finalize(const double *__restrict__ sums, int nek, int mode, DeviceScalars *__restrict__ sc,
+                           double *__restrict__ E_w, double *__restrict__ E_u) {
+    // sums: [0,nek) w shells, [nek,2nek) u shells, [2nek] sum|w|^2, [2nek+1] sum|u|^2
+    if (mode == 0) {
+        double etot = 0.5 * sums[2 * nek + 1];
+        double up = sqrt(2 * etot / 3);
+        sc->etot_raw = etot;
+        sc->up = up;
+        sc->inv_up = 1.0 / up;
+        sc->ens_raw = 0.5 * sums[2 * nek];
+        sc->enstrophy = 0.5 * sums[2 * nek] / (up * up);
+        sc->etot_norm = etot / (up * up);
+        for (int i = threadIdx.x; i < nek; i += blockDim.x) {
+            E_w[i] = sums[i];
+            E_u[i] = sums[nek + i] / (up * up);
+        }
+    } else {
+        for (int i = threadIdx.x; i < nek; i += blockDim.x) E_u[i] = sums[nek + i];
+    }
+}
+
+// ---- host side ----------------------------------------------------------------------------
+static void make_plan(int N, FftPlan &pl) {
+    pl.N = N;
+    pl.npass = 0;
+    int n = N;
+    while (n % 8 == 0) { pl.radix[pl.npass++] = 8; n /= 8; }
+    while (n % 4 == 0) { pl.radix[pl.npass++] = 4; n /= 4; }
+    while (n % 2 == 0) { pl.radix[pl.npass++] = 2; n /= 2; }
+    while (n % 3 == 0) { pl.radix[pl.npass++] = 3; n /= 3; }
+}
+
+bool fft_size_supported(int N) {
+    if (N < 4 || N > 1024 || N % 2) return false;
+    int n = N;
+    while (n % 2 == 0) n /= 2;
+    while (n % 3 == 0) n /= 3;
+    return n == 1;
+}
+
+// Digit-reversal permutation matching fft_lds's pass order (radix[0] first).
+void fft_reversal(int N, std::vector<int32_t> &rev) {
+    FftPlan pl;
+    make_plan(N, pl);
+    rev.resize(N);
+    for (int i = 0; i < N; i++) {
+        int x = i, pos = 0, mult = N;
+        for (int p = pl.npass - 1; p >= 0; p--) {
+            int R = pl.radix[p];
+            mult /= R;
+            pos += (x % R) * mult;
+            x /= R;
+        }
+        rev[i] = pos;
+    }
+}
+
+template <typename T>
+static hipError_t biot_savart_t(wt_ctx *c, int mode) {
+    hipStream_t st = c->stream;
+    const int N = c->N, H = N / 2 + 1;
+    FftPlan pl;
+    make_plan(N, pl);
+    pl.rev = (const int32_t *)((char *)c->twiddle + sizeof(C2<T>) * N);
+    pl.tw = c->twiddle;
+    int dev_lds = 65536;
+    (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+    const size_t cs = sizeof(C2<T>);
+    T *field = (T *)(mode == 0 ? c->omega : c->u);
+    C2<T> *spec = (C2<T> *)c->spec;
+    // x forward
+    int ppb = (int)(8192 / N) > 1 ? (int)(8192 / N) : 1;  // pairs per block
+    while ((size_t)ppb * N * cs > (size_t)dev_lds) ppb /= 2;
+    if (ppb < 1) ppb = 1;
+    int64_t npairs = (int64_t)3 * N * N / 2;
+    unsigned gx = (unsigned)((npairs + ppb - 1) / ppb);
+    T scale = (T)(1.0 / ((double)N * N * N));
+    size_t shx = (size_t)ppb * N * cs;
+    WT_HIP(hipFuncSetAttribute((const void *)k_fft_x_r2c<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shx));
+    kt_begin(c, kSlotFftX);
+    hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(gx), dim3(256), shx, st, (const T *)field, spec, pl, ppb, npairs, scale);
+    kt_end(c, kSlotFftX);
+    WT_HIP(hipGetLastError());
+    // y forward
+    int Wy = 1;
+    while (Wy < 16 && (size_t)(2 * Wy) * N * cs <= (size_t)dev_lds) Wy *= 2;
+    int ncy = (H + Wy - 1) / Wy;
+    size_t shy = (size_t)Wy * N * cs;
+    WT_HIP(hipFuncSetAttribute((const void *)k_fft_y<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shy));
+    WT_HIP(hipFuncSetAttribute((const void *)k_fft_y<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shy));
+    kt_begin(c, kSlotFftY);
+    hipLaunchKernelGGL((k_fft_y<T, false>), dim3((unsigned)(3 * N * ncy)), dim3(256), shy, st, spec, pl, Wy, ncy);
+    kt_end(c, kSlotFftY);
+    WT_HIP(hipGetLastError());
+    // z fused
+    const int nek = c->nek;
+    size_t shell_bytes = sizeof(double) * (2 * nek + 2);
+    int Wz = 1;
+    while (Wz < 16 && 3 * (size_t)(2 * Wz) * N * cs + shell_bytes <= (size_t)dev_lds) Wz *= 2;
+    int ncz = (H + Wz - 1) / Wz;
+    size_t shz = 3 * (size_t)Wz * N * cs + shell_bytes;
+    int64_t zblocks = (int64_t)N * ncz;
+    size_t need = sizeof(double) * (size_t)zblocks * (2 * nek + 2);
+    if (need > c->partials_cap) {
+        if (c->partials) (void)hipFree(c->partials);
+        c->partials = nullptr;
+        c->partials_cap = 0;
+        WT_HIP(hipMalloc(&c->partials, need));
+        c->partials_cap = need;
+    }
+    WT_HIP(hipFuncSetAttribute((const void *)k_fft_z<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shz));
+    kt_begin(c, kSlotFftZ);
+    hipLaunchKernelGGL(k_fft_z<T>, dim3((unsigned)zblocks), dim3(256), shz, st, spec, pl, Wz, ncz, mode, nek,
+                       c->partials);
+    kt_end(c, kSlotFftZ);
+    WT_HIP(hipGetLastError());
+    int cols = 2 * nek + 2;
+    double *colsum = nullptr;
+    WT_HIP(hipMallocAsync((void **)&colsum, sizeof(double) * cols, st));
+    kt_begin(c, kSlotReduce);
+    hipLaunchKernelGGL(k_reduce_columns, dim3((unsigned)((cols + 63) / 64)), dim3(64), 0, st,
+                       (const double *)c->partials, zblocks, cols, colsum);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (const double *)colsum, nek, mode, c->scal, c->E_w,
+                       c->E_u);
+    kt_end(c, kSlotReduce);
+    WT_HIP(hipGetLastError());
+    WT_HIP(hipFreeAsync(colsum, st));
+    if (mode != 0) return hipSuccess;
+    // y inverse
+    kt_begin(c, kSlotFftYInv);
+    hipLaunchKernelGGL((k_fft_y<T, true>), dim3((unsigned)(3 * N * ncy)), dim3(256), shy, st, spec, pl, Wy, ncy);
+    kt_end(c, kSlotFftYInv);
+    WT_HIP(hipGetLastError());
+    // x inverse (C2R), scaled by 1/up, omega scaled too
+    WT_HIP(hipFuncSetAttribute((const void *)k_fft_x_c2r<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shx));
+    kt_begin(c, kSlotFftXInv);
+    hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(gx), dim3(256), shx, st, (const C2<T> *)spec, (T *)c->u, (T *)c->omega, pl,
+                       ppb, npairs, (const DeviceScalars *)c->scal);
+    kt_end(c, kSlotFftXInv);
+    return hipGetLastError();
+}
+
+hipError_t launch_biot_savart(wt_ctx *c) {
+    return c->precision == 64 ? biot_savart_t<double>(c, 0) : biot_savart_t<float>(c, 0);
+}
+
+hipError_t launch_spectrum_of_u(wt_ctx *c) {
+    return c->precision == 64 ? biot_savart_t<double>(c, 1) : biot_savart_t<float>(c, 1);
+}
+
+}  // namespace wt

@@ -1,0 +1,415 @@
+// Geometry of the woven field on the GPU, FP64 (compiled with -ffp-contract=off):
+//   points   -> cze, Len_Disc, ck, ckk                   (tube.f90:392-515)
+//   sections -> quintic spline coefficients, ntf, dtf   (tube.f90:543-642,
+//                                                         subroutine.f90:440-488)
+//   sub-points with the seam fix                         (tube.f90:706-723)
+//   per sub-segment bounding box -> column work items    (tube.f90:766-798)
+// Every value is produced by the same IEEE operation sequence as the reference statement it
+// restates, so integers (ntf, boxes) and coordinates match the FP64 oracle bit for bit.
+#include "wt_device.h"
+
+namespace wt {
+
+constexpr int kColsPerItem = 256;
+
+// Fill out[i] = l for every i in [base[l], base[l+1]); one block per tube.
+__global__ void k_fill_owner(const int64_t *__restrict__ base, int32_t n, int32_t *__restrict__ out) {
+    int l = blockIdx.x;
+    if (l >= n) return;
+    for (int64_t i = base[l] + threadIdx.x; i < base[l + 1]; i += blockDim.x) out[i] = l;
+}
+
+__device__ __forceinline__ double wrapped_absdiff(double a, double b) {
+    double d = fabs(a - b);
+    if (d > kPi) d = d - kPi2;
+    return d;
+}
+
+// tube.f90:419-435: chord to the previous point; at the first point, the closing chord.
+__global__ void k_chords(Geo g) {
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= g.npoint_all) return;
+    int l = g.pt_tube[p];
+    int64_t b = g.pt_base[l];
+    int64_t e = g.pt_base[l + 1] - 1;
+    int64_t a = (p == b) ? e : p - 1;
+    int64_t c = (p == b) ? b : p;
+    // reference: first point pairs (cx(1), cx(npoint)); others (cx(i), cx(i-1))
+    double dx = wrapped_absdiff(g.cx[c], g.cx[a]);
+    double dy = wrapped_absdiff(g.cy[c], g.cy[a]);
+    double dz = wrapped_absdiff(g.cz[c], g.cz[a]);
+    g.chord[p] = sqrt((p2(dx) + p2(dy)) + p2(dz));
+}
+
+// tube.f90:418-437 sequential cumulative sum, one thread per tube (kept sequential so the
+// rounding matches the reference), then Len_Disc and length.
+__global__ void k_cumsum(Geo g, double *__restrict__ tot_out) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= g.nline) return;
+    int64_t b = g.pt_base[l], e = g.pt_base[l + 1];
+    double acc = 0.0;
+    g.cze[b] = 0.0;
+    for (int64_t i = b + 1; i < e; i++) {
+        acc = acc + g.chord[i];
+        g.cze[i] = acc;
+    }
+    double tot = acc + g.chord[b];
+    double LD = tot / kPi2;
+    g.Len_Disc[l] = LD;
+    g.length[l] = LD * kPi2;
+    tot_out[l] = tot;
+}
+
+// tube.f90:437: cze = cze*pi2/(cze(npoint) + dist)
+__global__ void k_cze_norm(Geo g, const double *__restrict__ tot) {
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= g.npoint_all) return;
+    g.cze[p] = g.cze[p] * kPi2 / tot[g.pt_tube[p]];
+}
+
+// tube.f90:439-466 (ck, with min-image wrap) and 468-489 (ckk, no wrap)
+template <bool SECOND>
+__global__ void k_derivative(Geo g) {
+    int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= g.npoint_all) return;
+    int l = g.pt_tube[p];
+    int64_t b = g.pt_base[l], e = g.pt_base[l + 1] - 1;
+    int64_t ip = (p == e) ? b : p + 1;
+    int64_t im = (p == b) ? e : p - 1;
+    double ds;
+    if (p == b) ds = g.cze[b + 1] - g.cze[e] + kPi2;
+    else if (p == e) ds = g.cze[b] - g.cze[e - 1] + kPi2;
+    else ds = g.cze[p + 1] - g.cze[p - 1];
+    double LD = g.Len_Disc[l];
+    if (!SECOND) {
+        double d[3] = {g.cx[ip] - g.cx[im], g.cy[ip] - g.cy[im], g.cz[ip] - g.cz[im]};
+#pragma unroll
+        for (int c = 0; c < 3; c++) if (d[c] > kPi) d[c] = d[c] - kPi2;
+#pragma unroll
+        for (int c = 0; c < 3; c++) if (d[c] < -kPi) d[c] = d[c] + kPi2;
+#pragma unroll
+        for (int c = 0; c < 3; c++) g.ck[3 * p + c] = d[c] / (ds * LD);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) g.ckk[3 * p + c] = (g.ck[3 * ip + c] - g.ck[3 * im + c]) / (ds * LD);
+    }
+}
+
+// get_spline_5th_coe (subroutine.f90:440-461)
+__device__ __forceinline__ void spline5_coe(double *coe, double zr, double xl, double xr, double vl,
+                                            double vr, double al, double ar) {
+    double xpl = xl, xpr = xr;
+    if (xl - xr > kPi) xpr = xr + kPi2;
+    else if (xl - xr < -kPi) xpl = xl + kPi2;
+    double d = xpr - xpl;
+    coe[0] = xl;
+    coe[1] = vl;
+    coe[2] = al / 2.0;
+    coe[3] = ((20.0 * d - (12.0 * vl + 8.0 * vr) * zr) - (3.0 * al - ar) * p2(zr)) / (2.0 * p3(zr));
+    coe[4] = ((-30.0 * d + (16.0 * vl + 14.0 * vr) * zr) + (3.0 * al - 2.0 * ar) * p2(zr)) / (2.0 * p4(zr));
+    coe[5] = ((12.0 * d - 6.0 * (vl + vr) * zr) - (al - ar) * p2(zr)) / (2.0 * p5(zr));
+}
+
+// Section i of a tube (0-based, i = n_sec is the periodic ghost): end points and their
+// parameters with the ghost rule of tube.f90:571-597.
+struct SecEnds { int64_t p0, p1; double cze0, cze1; };
+__device__ __forceinline__ SecEnds section_ends(const Geo &g, int l, int64_t s) {
+    int64_t b = g.pt_base[l];
+    int64_t n_sec = g.pt_base[l + 1] - b;
+    int64_t i = s - g.sec_base[l];
+    SecEnds e;
+    auto pl = [&](int64_t k, int64_t *pt, double *cz) {
+        if (k < n_sec) { *pt = b + k; *cz = g.cze[b + k]; }
+        else if (k == n_sec) { *pt = b; *cz = kPi2; }
+        else { *pt = b + 1; *cz = kPi2 + g.cze[b + 1]; }
+    };
+    pl(i, &e.p0, &e.cze0);
+    pl(i + 1, &e.p1, &e.cze1);
+    return e;
+}
+
+// curve_spline_5th (subroutine.f90:463-488) with abscissae cze_pl*Len_Disc (tube.f90:598)
+__global__ void k_spline(Geo g) {
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= g.nsec_all) return;
+    int l = g.sec_tube[s];
+    SecEnds e = section_ends(g, l, s);
+    double LD = g.Len_Disc[l];
+    double zr = e.cze1 * LD - e.cze0 * LD;
+    const double *X[3] = {g.cx, g.cy, g.cz};
+    double coe[18];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        spline5_coe(coe + 6 * c, zr, X[c][e.p0], X[c][e.p1], g.ck[3 * e.p0 + c], g.ck[3 * e.p1 + c],
+                    g.ckk[3 * e.p0 + c], g.ckk[3 * e.p1 + c]);
+#pragma unroll
+    for (int k = 0; k < 18; k++) g.coe[18 * s + k] = coe[k];
+    g.czl[s] = e.cze0;
+    g.czr[s] = e.cze1;
+}
+
+// tube.f90:625-642: turning angle from ntf_pre+1 samples (sequential sum, reference order),
+// ntf = ceiling(ntf_times*theta/pi2) + 1, dtf = (cze_pl(t_i+1)-cze_pl(t_i))/ntf.
+__global__ void k_theta(Geo g, Run r, int32_t *__restrict__ err) {
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= g.nsec_all) return;
+    int l = g.sec_tube[s];
+    double LD = g.Len_Disc[l];
+    double coe[18];
+#pragma unroll
+    for (int k = 0; k < 18; k++) coe[k] = g.coe[18 * s + k];
+    double czl = g.czl[s], czr = g.czr[s];
+    double dtfp = (czr - czl) / (1.0 * r.ntf_pre);
+    double theta = 0.0;
+    for (int tj = 1; tj <= r.ntf_pre + 1; tj++) {
+        double sp = czl + dtfp * (double)(tj - 1);
+        double zz = (sp - czl) * LD;
+        double dc[3], ddc[3], cr[3];
+        spline_d1(zz, coe, dc);
+        double ndc = norm3(dc);
+        spline_d2(zz, coe, ddc);
+        cr[0] = dc[1] * ddc[2] - dc[2] * ddc[1];
+        cr[1] = dc[2] * ddc[0] - dc[0] * ddc[2];
+        cr[2] = dc[0] * ddc[1] - dc[1] * ddc[0];
+        double ncr = norm3(cr);
+        theta = theta + ncr / p2(ndc) * dtfp * LD;
+    }
+    double q = ceil((double)r.ntf_times * theta / kPi2);
+    int32_t ntf;
+    if (!(q >= 0.0 && q < 1.0e7)) { atomicOr(err, 1); ntf = 1; }
+    else ntf = (int32_t)q + 1;
+    g.ntf[s] = ntf;
+    g.dtf[s] = (czr - czl) / (1.0 * ntf);
+}
+
+__global__ void k_fill_sub_sec(Geo g) {
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= g.nsec_all) return;
+    for (int64_t q = g.sp_base[s]; q < g.sp_base[s + 1]; q++) g.sub_sec[q] = (int32_t)s;
+}
+
+// tube.f90:706-713 sub-points; tube.f90:719-723 seam: the last sub-point of a real section is
+// the first sub-point of the next section (evaluated with that section's spline at 0).
+__global__ void k_subpoints(Geo g) {
+    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= g.nsub_all) return;
+    int32_t s = g.sub_sec[q];
+    int l = g.sec_tube[s];
+    int32_t j = (int32_t)(q - g.sp_base[s]);
+    bool real = (int64_t)s < g.sec_base[l + 1] - 1;
+    int32_t src = s;
+    int32_t jj = j;
+    if (real && j == g.ntf[s]) { src = s + 1; jj = 0; }
+    double LD = g.Len_Disc[l];
+    double czl = g.czl[src];
+    double sv = czl + g.dtf[src] * (double)jj;
+    double coe[18];
+#pragma unroll
+    for (int k = 0; k < 18; k++) coe[k] = g.coe[18 * src + k];
+    double c3[3];
+    spline_pos((sv - czl) * LD, coe, c3);
+#pragma unroll
+    for (int c = 0; c < 3; c++) g.ci[3 * q + c] = c3[c];
+}
+
+// Per sub-point: number of grid columns of its sub-segment's box (0 if it starts none).
+// Columns run along the chord's dominant axis. Also counts the work items (column blocks).
+__global__ void k_bin_count(Geo g, Run r, Bins b, int32_t *__restrict__ nitems,
+                            int32_t *__restrict__ err) {
+    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= g.nsub_all) return;
+    SubSeg sg;
+    int64_t ncol = 0;
+    if (subseg_load(g, r, q, sg)) {
+        int ax = dominant_axis(sg.tci0);
+        int bx = (ax + 1) % 3, cxs = (ax + 2) % 3;
+        ncol = (int64_t)(sg.hi[bx] - sg.lo[bx] + 1) * (int64_t)(sg.hi[cxs] - sg.lo[cxs] + 1);
+#pragma unroll
+        for (int c = 0; c < 3; c++)  // wrap_idx needs ip + N - 1 >= 0 for every visited index
+            if (sg.lo[c] < 1 - r.N || sg.hi[c] > 2 * r.N) atomicOr(err, 2);
+    }
+    b.col_count[q] = ncol;
+    nitems[q] = (int32_t)((ncol + kColsPerItem - 1) / kColsPerItem);
+}
+
+__global__ void k_fill_items(Geo g, Bins b) {
+    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= g.nsub_all) return;
+    for (int64_t it = b.item_off[q]; it < b.item_off[q + 1]; it++) b.item_q[it] = (uint32_t)q;
+}
+
+__global__ void k_get_bins(Geo g, Run r, int32_t *__restrict__ lohi, double *__restrict__ R) {
+    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= g.nsub_all) return;
+    SubSeg sg;
+    if (subseg_load(g, r, q, sg)) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) { lohi[6 * q + c] = sg.lo[c]; lohi[6 * q + 3 + c] = sg.hi[c]; }
+        R[q] = sg.R;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 6; c++) lohi[6 * q + c] = 0;
+        R[q] = -1.0;
+    }
+}
+
+static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename T>
+static hipError_t dalloc(wt_ctx *c, std::vector<void *> &list, T **p, size_t count) {
+    void *v = nullptr;
+    hipError_t e = hipMalloc(&v, sizeof(T) * (count ? count : 1));
+    if (e != hipSuccess) return e;
+    list.push_back(v);
+    *p = (T *)v;
+    return hipSuccess;
+}
+
+static void dfree(std::vector<void *> &list) {
+    for (void *p : list) (void)hipFree(p);
+    list.clear();
+}
+
+hipError_t launch_segments(wt_ctx *c) {
+    Geo &g = c->geo;
+    hipStream_t st = c->stream;
+    dfree(c->seg_allocs);
+    const int bs = 256;
+    // per point
+    WT_HIP(hipMemsetAsync(c->geo.cze, 0, sizeof(double) * g.npoint_all, st));
+    hipLaunchKernelGGL(k_fill_owner, dim3(g.nline), dim3(256), 0, st, g.pt_base, g.nline, g.pt_tube);
+    hipLaunchKernelGGL(k_fill_owner, dim3(g.nline), dim3(256), 0, st, g.sec_base, g.nline, g.sec_tube);
+    hipLaunchKernelGGL(k_chords, dim3(grid_for(g.npoint_all, bs)), dim3(bs), 0, st, g);
+    double *tot = nullptr;
+    WT_HIP(dalloc(c, c->seg_allocs, &tot, g.nline));
+    hipLaunchKernelGGL(k_cumsum, dim3(grid_for(g.nline, 64)), dim3(64), 0, st, g, tot);
+    hipLaunchKernelGGL(k_cze_norm, dim3(grid_for(g.npoint_all, bs)), dim3(bs), 0, st, g, tot);
+    hipLaunchKernelGGL(k_derivative<false>, dim3(grid_for(g.npoint_all, bs)), dim3(bs), 0, st, g);
+    hipLaunchKernelGGL(k_derivative<true>, dim3(grid_for(g.npoint_all, bs)), dim3(bs), 0, st, g);
+    // per section
+    hipLaunchKernelGGL(k_spline, dim3(grid_for(g.nsec_all, bs)), dim3(bs), 0, st, g);
+    int32_t *err = nullptr;
+    WT_HIP(dalloc(c, c->seg_allocs, &err, 1));
+    WT_HIP(hipMemsetAsync(err, 0, sizeof(int32_t), st));
+    kt_begin(c, kSlotTheta);
+    hipLaunchKernelGGL(k_theta, dim3(grid_for(g.nsec_all, 128)), dim3(128), 0, st, g, c->run, err);
+    kt_end(c, kSlotTheta);
+    // sub-point layout
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    WT_HIP(exclusive_scan_i32_to_i64(g.ntf, 1, g.sp_base, g.nsec_all, st, &tmp, &tmp_bytes));
+    int32_t herr = 0;
+    WT_HIP(hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    int64_t nsub = 0;
+    WT_HIP(hipMemcpyAsync(&nsub, g.sp_base + g.nsec_all, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    WT_HIP(hipStreamSynchronize(st));
+    if (tmp) (void)hipFree(tmp);
+    if (herr) return hipErrorInvalidValue;
+    g.nsub_all = nsub;
+    WT_HIP(dalloc(c, c->seg_allocs, &g.sub_sec, nsub));
+    WT_HIP(dalloc(c, c->seg_allocs, &g.ci, 3 * nsub));
+    hipLaunchKernelGGL(k_fill_sub_sec, dim3(grid_for(g.nsec_all, bs)), dim3(bs), 0, st, g);
+    kt_begin(c, kSlotSubpoints);
+    hipLaunchKernelGGL(k_subpoints, dim3(grid_for(nsub, bs)), dim3(bs), 0, st, g);
+    kt_end(c, kSlotSubpoints);
+    return hipGetLastError();
+}
+
+hipError_t launch_bins(wt_ctx *c) {
+    Geo &g = c->geo;
+    Bins &b = c->bins;
+    hipStream_t st = c->stream;
+    dfree(c->bin_allocs);
+    const int bs = 256;
+    int32_t *nitems = nullptr, *err = nullptr;
+    WT_HIP(dalloc(c, c->bin_allocs, &b.col_count, g.nsub_all));
+    WT_HIP(dalloc(c, c->bin_allocs, &b.item_off, g.nsub_all + 1));
+    WT_HIP(dalloc(c, c->bin_allocs, &nitems, g.nsub_all));
+    WT_HIP(dalloc(c, c->bin_allocs, &err, 1));
+    WT_HIP(hipMemsetAsync(err, 0, sizeof(int32_t), st));
+    kt_begin(c, kSlotBinCount);
+    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(g.nsub_all, bs)), dim3(bs), 0, st, g, c->run, b,
+                       nitems, err);
+    kt_end(c, kSlotBinCount);
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    WT_HIP(exclusive_scan_i32_to_i64(nitems, 0, b.item_off, g.nsub_all, st, &tmp, &tmp_bytes));
+    int64_t *colsum = nullptr;
+    WT_HIP(dalloc(c, c->bin_allocs, &colsum, g.nsub_all + 1));
+    // total columns (for statistics): scan of the column counts
+    WT_HIP(exclusive_scan_i64(b.col_count, colsum, g.nsub_all, st, &tmp, &tmp_bytes));
+    int64_t items = 0, sumcol = 0;
+    int32_t herr = 0;
+    WT_HIP(hipMemcpyAsync(&items, b.item_off + g.nsub_all, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    WT_HIP(hipMemcpyAsync(&sumcol, colsum + g.nsub_all, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    WT_HIP(hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    WT_HIP(hipStreamSynchronize(st));
+    if (tmp) (void)hipFree(tmp);
+    if (herr) return hipErrorInvalidValue;
+    b.items = items;
+    b.columns = sumcol;
+    WT_HIP(dalloc(c, c->bin_allocs, &b.item_q, items));
+    hipLaunchKernelGGL(k_fill_items, dim3(grid_for(g.nsub_all, bs)), dim3(bs), 0, st, g, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_get_bins(wt_ctx *c, int32_t *d_lohi, double *d_R) {
+    hipLaunchKernelGGL(k_get_bins, dim3(grid_for(c->geo.nsub_all, 256)), dim3(256), 0, c->stream,
+                       c->geo, c->run, d_lohi, d_R);
+    return hipGetLastError();
+}
+
+}  // namespace wt
+
+namespace wt {
+
+// Counters of wt_seg_stats: max ntf, min dtf (all sections), sum of ntf (all sections) and
+// number of sub-segments (real sections only).
+__global__ void k_seg_stats(Geo g, unsigned long long *__restrict__ out) {
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int mx = 0;
+    unsigned long long sum = 0, nseg = 0, dmin = ~0ull;
+    if (s < g.nsec_all) {
+        int l = g.sec_tube[s];
+        mx = g.ntf[s];
+        sum = (unsigned long long)g.ntf[s];
+        if ((int64_t)s < g.sec_base[l + 1] - 1) nseg = (unsigned long long)g.ntf[s];
+        double d = g.dtf[s];
+        dmin = (unsigned long long)__double_as_longlong(d);  // dtf > 0: bit order = value order
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        int m2 = __shfl_xor(mx, o, 64);
+        mx = m2 > mx ? m2 : mx;
+        sum += __shfl_xor(sum, o, 64);
+        nseg += __shfl_xor(nseg, o, 64);
+        unsigned long long d2 = __shfl_xor(dmin, o, 64);
+        dmin = d2 < dmin ? d2 : dmin;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(out + 0, (unsigned long long)mx);
+        atomicAdd(out + 1, sum);
+        atomicAdd(out + 2, nseg);
+        atomicMin(out + 3, dmin);
+    }
+}
+
+hipError_t launch_seg_stats(wt_ctx *c, wt_seg_stats *o) {
+    unsigned long long *d = nullptr, h[4] = {0, 0, 0, ~0ull};
+    WT_HIP(hipMalloc(&d, sizeof h));
+    WT_HIP(hipMemcpyAsync(d, h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_seg_stats, dim3(grid_for(c->geo.nsec_all, 256)), dim3(256), 0, c->stream, c->geo, d);
+    WT_HIP(hipGetLastError());
+    WT_HIP(hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    WT_HIP(hipStreamSynchronize(c->stream));
+    (void)hipFree(d);
+    o->ntf_max = (int32_t)h[0];
+    o->ntf_sum = (int64_t)h[1];
+    o->nseg_all = (int64_t)h[2];
+    union { unsigned long long u; double d; } bits;
+    bits.u = h[3];
+    o->dtf_min = bits.d;
+    return hipSuccess;
+}
+
+}  // namespace wt

@@ -1,0 +1,166 @@
+"""Seeded fractional-Brownian-bridge (fBb) centerline generator for the woven-field workloads.
+
+The reference builds its centerlines offline in MATLAB (FBB_with_parameters.m) with the
+Wavelet Toolbox's wfbm, which is not reproducible here. This module restates the same
+construction with an exact fBm sampler:
+  - fractional Gaussian noise by circulant embedding (Davies & Harte 1987), cumulated to fBm;
+  - the bridge B(t) - f(t) B(1), f(t) = (t^2H + 1 - (1-t)^2H)/2 (FBB_with_parameters.m:92-102);
+  - step-rms normalisation of the 3-D increments to dB (FBB_with_parameters.m:104-112);
+  - a uniform random shift and a wrap into [-pi, pi) (FBB_with_parameters.m:117-120).
+A tube also carries a winding drift 2*pi*w*t along an integer lattice vector w nearest to its
+level's mean axis direction, so that a closed periodic centerline has a mean direction.
+Everything is seeded per tube (seed + global tube index), so subsets are reproducible.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from .woven_io import RunParams, Tubes
+
+
+_EIG_CACHE: dict = {}
+
+
+def _smooth_at_least(n: int) -> int:
+    """Smallest 2^a 3^b >= n (fast FFT length)."""
+    best = 1 << max(0, (n - 1).bit_length())
+    p3 = 1
+    while p3 <= best:
+        v = p3
+        while v < n:
+            v *= 2
+        best = min(best, v)
+        p3 *= 3
+    return best
+
+
+def _circulant_sqrt_eigs(n: int, H: float) -> np.ndarray:
+    """sqrt(lambda / 2n) of the circulant embedding of the fGn covariance (cached per n, H)."""
+    key = (n, float(H))
+    v = _EIG_CACHE.get(key)
+    if v is None:
+        k = np.arange(0, n + 1, dtype=np.float64)
+        gam = 0.5 * (np.abs(k + 1) ** (2 * H) - 2 * np.abs(k) ** (2 * H) + np.abs(k - 1) ** (2 * H))
+        row = np.concatenate([gam, gam[-2:0:-1]])          # length 2n circulant first row
+        lam = np.maximum(np.fft.fft(row).real, 0.0)        # exact method: lam >= 0 up to round-off
+        v = np.sqrt(lam / len(row))
+        if len(_EIG_CACHE) > 64:
+            _EIG_CACHE.clear()
+        _EIG_CACHE[key] = v
+    return v
+
+
+def fgn_davies_harte(n: int, H: float, rng: np.random.Generator, count: int = 1) -> np.ndarray:
+    """(count, n) samples of unit-variance fractional Gaussian noise with Hurst index H.
+
+    Generated at a smooth length n' >= n and truncated (a prefix of fGn is fGn). The real and
+    imaginary parts of one complex circulant draw are two independent samples."""
+    if n <= 1:
+        return rng.standard_normal((count, max(n, 0)))
+    s = _circulant_sqrt_eigs(_smooth_at_least(n), H)
+    m = len(s)
+    pairs = (count + 1) // 2
+    z = rng.standard_normal((pairs, m)) + 1j * rng.standard_normal((pairs, m))
+    w = np.fft.fft(s[None, :] * z, axis=1)[:, :n]
+    out = np.concatenate([w.real, w.imag], axis=0)
+    return out[:count]
+
+
+def fbb_path(npts: int, H: float, rng: np.random.Generator) -> np.ndarray:
+    """(3, npts) closed fractional Brownian bridge with unnormalised amplitude."""
+    j = np.arange(1, npts + 1, dtype=np.float64) / npts
+    f = 0.5 * (j ** (2 * H) + 1.0 - (1.0 - j) ** (2 * H))
+    g = fgn_davies_harte(npts - 1, H, rng, 3)
+    B = np.concatenate([np.zeros((3, 1)), np.cumsum(g, axis=1)], axis=1)
+    return B - f[None, :] * B[:, -1:]
+
+
+def lattice_winding(direction: np.ndarray) -> np.ndarray:
+    """Integer lattice vector in {-1,0,1}^3 nearest in direction to a unit vector."""
+    d = np.asarray(direction, dtype=np.float64)
+    w = np.rint(d / np.max(np.abs(d))).astype(np.int64)
+    if not np.any(w):
+        w[np.argmax(np.abs(d))] = 1 if d[np.argmax(np.abs(d))] >= 0 else -1
+    return w
+
+
+def tube_centerline(npts: int, H: float, winding: np.ndarray, wiggle: float,
+                    rng: np.random.Generator) -> np.ndarray:
+    """(3, npts) centerline wrapped into [-pi, pi): shift + drift + normalised fBb."""
+    bb = fbb_path(npts, H, rng)
+    steps = np.diff(bb, axis=1)
+    rms = float(np.sum(steps ** 2) / (npts - 1))
+    drift_step = 2.0 * np.pi * float(np.linalg.norm(winding)) / npts
+    dB = wiggle * drift_step
+    bb *= dB / math.sqrt(rms) if rms > 0 else 0.0
+    t = np.arange(npts, dtype=np.float64) / npts
+    x0 = (rng.random(3) - 0.5) * 2.0 * np.pi
+    x = x0[:, None] + 2.0 * np.pi * winding[:, None] * t[None, :] + bb
+    return np.mod(x + np.pi, 2.0 * np.pi) - np.pi
+
+
+@dataclasses.dataclass
+class Level:
+    radius: float          # core radius r; sigma = r / sqrt(2)
+    ntubes: int
+    npoints: tuple         # (min, max) points per tube; max > min draws log-uniformly
+    hurst: float = 0.75
+
+
+def woven_config(N: int, levels: list[Level], seed: int, gamma_exp: float = 4.0 / 3.0,
+                 gamma_jitter: float = 0.2, wiggle: float = 0.35, sigma_ratio: float = 1.0,
+                 Rtube_ratio: float = 3.0, ntf_times: int = 50, ntf_pre: int = 1000,
+                 dzeta_sup: float = 1e-4) -> tuple[RunParams, Tubes]:
+    """Build the tube table and centerlines of one woven field.
+
+    Circulation Gamma = r^gamma_exp * U(1 - jitter, 1 + jitter); sigma = r / sqrt(2) (so the
+    leading Gaussian term exp(-rho^2 / 2 sigma^2) is exp(-rho^2 / r^2)); n_T = 1; no filter.
+    """
+    npoint, xs, sig, gam = [], [], [], []
+    tube = 0
+    for li, lev in enumerate(levels):
+        lrng = np.random.default_rng(seed + 1_000_000 + li)
+        axis = lrng.standard_normal(3)
+        axis /= np.linalg.norm(axis)
+        w = lattice_winding(axis)
+        for _ in range(lev.ntubes):
+            rng = np.random.default_rng(seed + tube)
+            lo, hi = lev.npoints
+            n = lo if hi <= lo else int(math.floor(lo * (hi / lo) ** rng.random()))
+            n = max(n, 8)
+            xs.append(tube_centerline(n, lev.hurst, w, wiggle, rng))
+            npoint.append(n)
+            sig.append(lev.radius / math.sqrt(2.0))
+            gam.append(lev.radius ** gamma_exp * rng.uniform(1 - gamma_jitter, 1 + gamma_jitter))
+            tube += 1
+    X = np.concatenate(xs, axis=1)
+    t = Tubes(np.array(npoint, dtype=np.int32), X[0].copy(), X[1].copy(), X[2].copy(),
+              np.array(sig), np.array(gam), np.ones(tube, dtype=np.int32),
+              np.zeros(tube, dtype=np.int32))
+    p = RunParams(N=N, n_d=N, line_s=1, line_e=tube, sigma_ratio=sigma_ratio, eta_t=0.0,
+                  ntf_times=ntf_times, ntf_pre=ntf_pre, Rtube_ratio=Rtube_ratio, dzeta_sup=dzeta_sup)
+    t.validate()
+    return p, t
+
+
+def config_plain_128(seed: int = 1234) -> tuple[RunParams, Tubes]:
+    """128^3 plain method: 3 levels, r = 0.4/0.2/0.1, 8/32/128 tubes of 1024 points, H = 0.75."""
+    levels = [Level(0.4, 8, (1024, 1024)), Level(0.2, 32, (1024, 1024)), Level(0.1, 128, (1024, 1024))]
+    return woven_config(128, levels, seed)
+
+
+def config_batched_512(seed: int = 1234, density: float = 1.0) -> tuple[RunParams, Tubes]:
+    """512^3: 6 levels, r = 0.8 * 2^-l, round(density * 4 * 4^l) tubes, 256*2^l..256*2^(l+1) points."""
+    levels = [Level(0.8 * 2.0 ** -l, max(1, int(round(density * 4 * 4 ** l))),
+                    (256 * 2 ** l, 256 * 2 ** (l + 1))) for l in range(6)]
+    return woven_config(512, levels, seed)
+
+
+def config_small(N: int = 32, seed: int = 7, ntubes=(1, 2), npts=(64, 48), radii=(0.5, 0.3),
+                 sigma_ratio: float = 1.0, ntf_pre: int = 100, **kw) -> tuple[RunParams, Tubes]:
+    """Tiny multi-level case for parity tests (seconds on the CPU oracle)."""
+    levels = [Level(r, n, (p, p)) for r, n, p in zip(radii, ntubes, npts)]
+    return woven_config(N, levels, seed, sigma_ratio=sigma_ratio, ntf_pre=ntf_pre, **kw)

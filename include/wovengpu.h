@@ -1,0 +1,165 @@
+/*
+ * wovengpu: C ABI of the MI355X (gfx950) woven-turbulence field generator.
+ *
+ * The reference (YYgroup/FastWeavTurb) is one Fortran program with no plugin API; its hot
+ * path is inline code in tube.f90 plus external subroutines in subroutine.f90. Each entry
+ * point below replaces one stretch of that path and cites it. A Fortran host calls these
+ * through ISO_C_BINDING (fastweavturb_amd/fortran/wovengpu_mod.f90); Python calls them
+ * through ctypes (fastweavturb_amd/api.py).
+ *
+ * Conventions
+ *  - Return 0 on success, a negative WT_E* code on failure; wt_last_error() gives the text.
+ *    Nothing aborts or throws across the ABI.
+ *  - Scalars by value, arrays as plain pointers to contiguous host memory owned by the caller.
+ *  - Fields are N^3 values per component in Fortran order (x fastest), like the reference's
+ *    vorx(nx, ny, nz). The context owns all device memory and one HIP stream; every call is
+ *    stream-ordered and synchronous on return.
+ *  - Indices in results are the reference's: 1-based grid indices, extended (unwrapped)
+ *    indices for bounding boxes (tube.f90:793-798).
+ *  - One context = one GPU; single host thread per context.
+ */
+#ifndef WOVENGPU_H
+#define WOVENGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WT_OK 0
+#define WT_EINVAL (-1)     /* bad argument or inconsistent input                */
+#define WT_ESTATE (-2)     /* call out of order (e.g. deposit before segments)  */
+#define WT_EDEVICE (-3)    /* HIP runtime error                                 */
+#define WT_ENOMEM (-4)     /* device allocation failed or memory cap exceeded   */
+#define WT_ERANGE (-5)     /* value outside what the kernels support            */
+
+typedef struct wt_ctx wt_ctx;
+
+/* Scalar header of parameters_input.dat, same order and meaning (tube.f90:191-216). */
+typedef struct {
+    int32_t N;            /* grid points per axis, even                       */
+    int32_t n_d;          /* N_downsample (field_wh.dat resolution)           */
+    int32_t line_s;       /* first tube deposited, 1-based                    */
+    int32_t line_e;       /* last tube deposited, 1-based, inclusive          */
+    double sigma_ratio;   /* sigma_max / sigma_min                            */
+    double eta_t;         /* twist rate eta0                                  */
+    int32_t ntf_times;    /* sub-segments per 2*pi of turning angle           */
+    int32_t ntf_pre;      /* samples per section for the turning angle        */
+    double Rtube_ratio;   /* cut-off radius / sigma                           */
+    double dzeta_sup;     /* upper bound of the dsigma/dzeta step             */
+} wt_params;
+
+/* Geometry counters after wt_build_segments (tube.f90:623-738 prints/writes these). */
+typedef struct {
+    int64_t npoint_all;   /* centerline points                                */
+    int64_t nsec_all;     /* sections incl. one periodic ghost per tube       */
+    int64_t nsub_all;     /* sub-points: sum over sections of (ntf + 1)       */
+    int64_t nseg_all;     /* sub-segments deposited from: sum of ntf, real sections */
+    int32_t ntf_max;      /* max over all sections                             */
+    int32_t reserved;
+    double dtf_min;       /* min over all sections (tube.f90:737)              */
+    int64_t ntf_sum;      /* sum of ntf over all sections                      */
+} wt_seg_stats;
+
+/* Counters of the deposition (wt_deposit). */
+typedef struct {
+    int64_t columns;      /* grid columns scanned                                */
+    int64_t candidates;   /* cells tested with the reference predicate          */
+    int64_t hits;         /* (cell, sub-segment) pairs that deposited            */
+    int32_t fixed_point_shift; /* FP32 mode: accumulator scale 2^shift; 0 in FP64 */
+    int32_t overflow;     /* nonzero if a fixed-point contribution overflowed    */
+} wt_deposit_stats;
+
+/* Field selectors for wt_get_field / wt_set_field. */
+#define WT_FIELD_WX 0
+#define WT_FIELD_WY 1
+#define WT_FIELD_WZ 2
+#define WT_FIELD_UX 3
+#define WT_FIELD_UY 4
+#define WT_FIELD_UZ 5
+
+/* Create a context on HIP device `device` for an N^3 grid computing in FP64 (precision=64,
+ * reference arithmetic) or FP32 (precision=32; geometry, binning and hit tests stay FP64).
+ * Replaces: set_fft_plans + wavenumber + field allocation (tube.f90:279-306;
+ * subroutine.f90:860-919). */
+int wt_create(wt_ctx **ctx, int32_t n, int32_t precision, int32_t device);
+void wt_destroy(wt_ctx *ctx);
+const char *wt_last_error(const wt_ctx *ctx);
+
+/* Replaces the header read of parameters_input.dat (tube.f90:191-216). p->N must equal the
+ * context's N. */
+int wt_set_params(wt_ctx *ctx, const wt_params *p);
+
+/* Replaces the per-tube rows of parameters_input.dat (tube.f90:218-221) and the centerline
+ * read + broadcast (tube.f90:341-379): nline tubes, npoint[l] points each, coordinates as
+ * three flat FP64 arrays of sum(npoint) entries. if_filter must be all zero (filter_spec is
+ * not part of this library). */
+int wt_set_tubes(wt_ctx *ctx, int32_t nline, const int32_t *npoint, const double *sigma,
+                 const double *gamma, const int32_t *n_T, const int32_t *if_filter,
+                 const double *cx, const double *cy, const double *cz);
+
+/* Replaces tube.f90:392-751: cze/Len_Disc/ck/ckk, quintic splines, turning angle -> ntf/dtf,
+ * sub-points with the seam fix. FP64, bit-identical to the reference's operation order. */
+int wt_build_segments(wt_ctx *ctx, wt_seg_stats *out);
+
+/* Replaces tube.f90:766-798 (unwrap, Rtube, integer bounding box) for every sub-segment of
+ * tubes line_s..line_e, and lays out the column work of the deposition. Integer-exact. */
+int wt_bin(wt_ctx *ctx);
+
+/* Replaces tube.f90:800-892: vorticity of every (cell, sub-segment) pair that passes the
+ * reference's tests, summed over sub-segments. Leaves omega resident on the device. */
+int wt_deposit(wt_ctx *ctx, wt_deposit_stats *out);
+
+/* Replaces tube.f90:985-1043 (+ subroutine.f90:983-1084, 1550-1569): forward FFT of omega,
+ * u_hat = i k x w_hat / k^2 with the DC mode zeroed, inverse FFT, then normalisation of omega
+ * and u by up = sqrt(2 Etot / 3). Outputs (any may be NULL): Etot before normalisation, up,
+ * and Enstrophy after normalisation. Also prepares the spectra of raw omega and of u. */
+int wt_biot_savart(wt_ctx *ctx, double *etot, double *up, double *enstrophy);
+
+/* Whole woven-field path: wt_build_segments + wt_bin + wt_deposit + wt_biot_savart. */
+int wt_generate(wt_ctx *ctx, double *etot, double *up, double *enstrophy);
+
+/* Replaces spectrum/get_spectrum (subroutine.f90:1604-1638), accumulated in FP64:
+ * E[i-1] for integer shells i = 1..nek, nek = wt_nek(N). field 0 = omega before
+ * normalisation (spectrum_vor_temp.dat, tube.f90:972-978), 1 = normalised u (spectrum.dat,
+ * tube.f90:1158-1164). Returns nek (>0) or an error code. */
+int wt_spectrum(wt_ctx *ctx, int32_t field, double *E, int32_t cap);
+int32_t wt_nek(int32_t n);
+
+/* Structure functions of the resident velocity.
+ * mode 0 replaces Sp_calculate (subroutine.f90:64-129): S[(p-1)*nsep + m] for p = 1..pmax,
+ *   longitudinal increments along x, y and z averaged, separations as given (use
+ *   wt_sp_separations for the reference's list); pmin must be 1.
+ * mode 1 generalises getStructureFunc_x (subroutine.f90:131-162): x-increments of u_x only,
+ *   S[(p-pmin)*nsep + m] = sum (u_x(i+sep_m) - u_x(i))^p / N^3, p = pmin..pmax. */
+int wt_structure_functions(wt_ctx *ctx, int32_t mode, int32_t pmin, int32_t pmax,
+                           int32_t nsep, const int32_t *seps, double *S);
+/* Separations of Sp_calculate (subroutine.f90:78, 97-98). Returns the count. */
+int32_t wt_sp_separations(int32_t n, int32_t *r, int32_t cap);
+
+/* Copy a resident field component to/from host memory, Fortran order, FP64 or FP32 host
+ * buffer (output_v/output_puredata copy-out, subroutine.f90:1342-1399). wt_set_field replaces
+ * the resident omega or u (used to drive wt_biot_savart / statistics on a given field). */
+int wt_get_field(wt_ctx *ctx, int32_t which, void *host, int32_t host_is_fp64);
+int wt_set_field(wt_ctx *ctx, int32_t which, const void *host, int32_t host_is_fp64);
+
+/* Inspection of intermediate results, for parity checks against the reference's arrays:
+ * per section (tube.f90 ntf_list, dtf_list; sections include one ghost per tube),
+ * per sub-point (ci_list2 after the seam fix, s_list2), per sub-point used as a sub-segment
+ * start (bounding box lo[3], hi[3] and Rtube; entries that start no sub-segment get R = -1). */
+int wt_get_sections(wt_ctx *ctx, int32_t *ntf, double *dtf);
+int wt_get_subpoints(wt_ctx *ctx, double *ci_xyz, double *s);
+int wt_get_bins(wt_ctx *ctx, int32_t *lohi, double *R);
+
+/* Timings in milliseconds, measured with HIP events on the context stream. [0..3]: stages of
+ * the last wt_generate (segments, bin, deposit, biot_savart). [4..]: the last launch of each
+ * main kernel: turning angle, sub-points, bin count, deposit, fixed-point conversion, FFT x,
+ * FFT y, FFT z + Biot-Savart, FFT y inverse, FFT x inverse, spectral reduction (-1 if it has
+ * not run). Returns the number of entries available (15). */
+int wt_stage_times(wt_ctx *ctx, double *ms, int32_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
