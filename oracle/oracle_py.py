@@ -172,7 +172,7 @@ class Geometry:
         """Brute-force predicate of tube.f90:800-818 for one (sub-segment, extended cell)."""
         rec = C.pointer(self._bin_recs[seg])
         return bool(lib().or_pair_hits(C.byref(self.params), C.byref(self._g), rec,
-                                       C.c_int64(seg), ip, jp, kp))
+                                       C.c_int64(int(seg)), int(ip), int(jp), int(kp)))
 
     def deposit(self, sigma_line, Gamma_line, n_T_line):
         """or_deposit: FP64 vorticity (3, N, N, N) [c, z, y, x] and the number of hits."""

@@ -1,0 +1,63 @@
+"""Writes the committed golden fixtures of tests/golden/ (run from the repository root):
+
+  fft_known.npz      numpy.fft (pocketfft) answers for random inputs of lengths 8, 12, 16, 24
+  woven_small16.npz  inputs and FP64 oracle outputs of a 2-level, 3-tube woven field on a 16^3
+                     grid with sigma_ratio = 4 and eta = 0.5 (geometry integers, sub-points,
+                     bounding boxes, raw vorticity, normalised velocity, stats, spectra, Sp)
+
+The reference ships no outputs of its own, so these vectors pin the oracle against future
+edits and give the GPU path a second, oracle-independent comparison target.
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from fastweavturb_amd import fbb  # noqa: E402
+from oracle import oracle_py as O  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def small16():
+    p, t = fbb.config_small(N=16, seed=21, ntubes=(1, 2), npts=(30, 24), radii=(0.9, 0.5),
+                            sigma_ratio=4.0, ntf_pre=200)
+    p.eta_t = 0.5
+    return p, t
+
+
+def main():
+    rng = np.random.default_rng(12345)
+    fft = {}
+    for n in (8, 12, 16, 24):
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        fft[f"x{n}"] = x
+        fft[f"X{n}"] = np.fft.fft(x)
+    np.savez_compressed(os.path.join(HERE, "fft_known.npz"), **fft)
+
+    p, t = small16()
+    P = O.Params(p.N, p.n_d, p.line_s, p.line_e, p.sigma_ratio, p.eta_t, p.ntf_times, p.ntf_pre,
+                 p.Rtube_ratio, p.dzeta_sup)
+    geo = O.Geometry(P, t.npoint, t.cx, t.cy, t.cz)
+    b = geo.bins(t.sigma, t.n_T)
+    om_raw, hits = geo.deposit(t.sigma, t.gamma, t.n_T)
+    om, u, st = O.run(P, t.npoint, t.cx, t.cy, t.cz, t.sigma, t.gamma, t.n_T)
+    Ew = O.spectrum(om_raw)
+    Eu = O.spectrum(u)
+    Sp, r = O.sp_calculate(u, 6)
+    np.savez_compressed(
+        os.path.join(HERE, "woven_small16.npz"),
+        params=np.array([p.N, p.n_d, p.line_s, p.line_e, p.sigma_ratio, p.eta_t, p.ntf_times, p.ntf_pre,
+                         p.Rtube_ratio, p.dzeta_sup]),
+        npoint=t.npoint, cx=t.cx, cy=t.cy, cz=t.cz, sigma=t.sigma, gamma=t.gamma, n_T=t.n_T,
+        ntf=geo.ntf, dtf=geo.dtf, ci=geo.ci, lo=b["lo"], hi=b["hi"], R=b["R"],
+        seg_q=geo.sp_base[geo.seg_sec] + geo.seg_j, hits=np.array([hits]),
+        omega_raw=om_raw, omega=om, u=u, stats=st[:4], E_w=Ew, E_u=Eu, Sp=Sp, r=r)
+    print("wrote fft_known.npz and woven_small16.npz")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,304 @@
+"""Pins of the FP64 oracle against what the mathematics fixes (closed forms, invariants,
+brute force, a library DFT), so that the oracle can be trusted as the checker of the GPU path.
+"""
+import numpy as np
+import pytest
+
+from fastweavturb_amd import fbb
+from fastweavturb_amd.woven_io import RunParams, Tubes
+
+
+def params(O, N, nline, **kw):
+    d = dict(line_s=1, line_e=nline, sigma_ratio=1.0, eta_t=0.0, ntf_times=50, ntf_pre=1000,
+             Rtube_ratio=3.0, dzeta_sup=1e-4)
+    d.update(kw)
+    return O.Params(N, N, d["line_s"], d["line_e"], d["sigma_ratio"], d["eta_t"], d["ntf_times"],
+                    d["ntf_pre"], d["Rtube_ratio"], d["dzeta_sup"])
+
+
+# ---- spectral part --------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 16, 17, 24, 30, 31, 48, 64, 96])
+def test_fft1d_matches_pocketfft(oracle, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    np.testing.assert_allclose(oracle.fft1d(x, True), np.fft.fft(x), rtol=0, atol=1e-12 * max(1, n))
+    np.testing.assert_allclose(oracle.fft1d(x, False), np.fft.ifft(x) * n, rtol=0, atol=1e-12 * max(1, n))
+
+
+def test_fft3d_reference_scaling(oracle):
+    """fourier_forward scales by 1/N per axis (subroutine.f90:1002,1010,1018); backward is unscaled."""
+    N = 8
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((N, N, N)) + 0j
+    F = oracle.fft3d(a, True)
+    np.testing.assert_allclose(F, np.fft.fftn(a) / N ** 3, atol=1e-14)
+    np.testing.assert_allclose(oracle.fft3d(F, False), a, atol=1e-13)
+
+
+def grid(N):
+    x = -np.pi + 2 * np.pi * np.arange(N) / N
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    return X, Y, Z
+
+
+def test_biot_savart_single_mode(oracle):
+    """omega = (0, 0, cos y)  =>  u = (-sin y, 0, 0)."""
+    N = 16
+    X, Y, Z = grid(N)
+    om = np.zeros((3, N, N, N))
+    om[2] = np.cos(Y)
+    u = oracle.biot_savart(om)
+    np.testing.assert_allclose(u[0], -np.sin(Y), atol=1e-13)
+    np.testing.assert_allclose(u[1], 0, atol=1e-13)
+    np.testing.assert_allclose(u[2], 0, atol=1e-13)
+
+
+def test_biot_savart_invariants_and_nyquist_rule(oracle):
+    """For random (non-solenoidal) omega the reference keeps Re(IFFT(i k x w_hat / k^2)).
+    Check against an independent numpy evaluation, divergence-free output, and the closed form
+    of the real part: Nyquist wavenumber components zeroed in the numerator, kept in k^2."""
+    N = 8
+    rng = np.random.default_rng(7)
+    om = rng.standard_normal((3, N, N, N))
+    u = oracle.biot_savart(om)
+    k1 = np.fft.fftfreq(N, 1.0 / N)
+    k1[N // 2] = -N // 2
+    KZ, KY, KX = np.meshgrid(k1, k1, k1, indexing="ij")
+    k2 = KX ** 2 + KY ** 2 + KZ ** 2
+    k2[0, 0, 0] = 1.0
+    wh = [np.fft.fftn(om[c]) / N ** 3 for c in range(3)]
+    # full c2c + real part
+    cx = KY * wh[2] - KZ * wh[1]
+    cy = KZ * wh[0] - KX * wh[2]
+    cz = KX * wh[1] - KY * wh[0]
+    uh = [1j * c / k2 for c in (cx, cy, cz)]
+    for c in range(3):
+        uh[c][0, 0, 0] = 0
+    u_np = np.stack([np.real(np.fft.ifftn(h) * N ** 3) for h in uh])
+    np.testing.assert_allclose(u, u_np, atol=1e-12)
+    # Nyquist rule
+    ny = lambda K: np.where(np.abs(K) == N // 2, 0.0, K)
+    qx, qy, qz = ny(KX), ny(KY), ny(KZ)
+    uh2 = [1j * (qy * wh[2] - qz * wh[1]) / k2, 1j * (qz * wh[0] - qx * wh[2]) / k2,
+           1j * (qx * wh[1] - qy * wh[0]) / k2]
+    for c in range(3):
+        uh2[c][0, 0, 0] = 0
+    u_rule = np.stack([np.fft.ifftn(h) * N ** 3 for h in uh2])
+    assert np.max(np.abs(u_rule.imag)) < 1e-12
+    np.testing.assert_allclose(u, u_rule.real, atol=1e-12)
+    # divergence free (spectral derivative with the same Nyquist rule)
+    div = sum(1j * q * np.fft.fftn(u[c]) for c, q in enumerate((qx, qy, qz)))
+    assert np.max(np.abs(div)) < 1e-10
+
+
+def test_normalisation_closed_form(oracle):
+    N = 8
+    rng = np.random.default_rng(2)
+    om = rng.standard_normal((3, N, N, N))
+    u = rng.standard_normal((3, N, N, N))
+    E = 0.5 * np.mean(np.sum(u ** 2, axis=0))
+    ens = 0.5 * np.mean(np.sum(om ** 2, axis=0))
+    st = oracle.normalize(om, u)
+    assert st[0] == pytest.approx(E, rel=1e-13)
+    up = np.sqrt(2 * E / 3)
+    assert st[1] == pytest.approx(up, rel=1e-13)
+    assert st[2] == pytest.approx(ens / up ** 2, rel=1e-12)
+    assert st[3] == pytest.approx(1.5, rel=1e-12)        # u' = 1 after normalisation
+    assert 0.5 * np.mean(np.sum(u ** 2, axis=0)) == pytest.approx(1.5, rel=1e-12)
+
+
+def test_spectrum_single_mode_and_shell_rule(oracle):
+    N = 16
+    X, Y, Z = grid(N)
+    u = np.zeros((3, N, N, N))
+    u[0] = np.cos(3 * X)
+    u[1] = 2 * np.sin(2 * Y + Z)            # |k|^2 = 5 -> shell round(2.236) = 2
+    E = oracle.spectrum(u)
+    assert len(E) == oracle.nek(N) == 8
+    expect = np.zeros(8)
+    expect[3 - 1] = 0.25
+    expect[2 - 1] = 0.5 * 4 * 0.5
+    np.testing.assert_allclose(E, expect, atol=1e-13)
+
+
+def test_nek_rule(oracle):
+    # nek = int(sqrt((nx^2 + ny^2 + nz^2) / 12.0)) = N/2 for the supported cubic grids
+    for N in (8, 12, 16, 24, 32, 64, 96, 128, 192, 256, 512, 1024):
+        assert oracle.nek(N) == N // 2
+
+
+def test_sp_separations_literal(oracle):
+    """subroutine.f90:78, 97-98 evaluated in FP32."""
+    expect = [1, 1, 2, 3, 4, 6, 8, 11, 16, 23, 32, 45, 64, 91, 128, 181, 256, 362, 512, 724, 1024]
+    assert list(oracle.sp_separations(1024)) == expect
+    assert list(oracle.sp_separations(512)) == expect[:19]
+    assert list(oracle.sp_separations(128)) == expect[:15]
+    assert list(oracle.sp_separations(96)) == expect[:13]      # lg2L = 6 for N = 96
+
+
+def test_structure_functions_closed_form(oracle):
+    """u = (sin x, sin y, sin z): S2(r) = 2 sin^2(a/2), S4(r) = 6 sin^4(a/2), odd orders 0."""
+    N = 32
+    X, Y, Z = grid(N)
+    u = np.stack([np.sin(X), np.sin(Y), np.sin(Z)])
+    Sp, r = oracle.sp_calculate(u, 6)
+    a = r * 2 * np.pi / N
+    np.testing.assert_allclose(Sp[1], 2 * np.sin(a / 2) ** 2, atol=1e-13)
+    np.testing.assert_allclose(Sp[3], 6 * np.sin(a / 2) ** 4, atol=1e-13)
+    np.testing.assert_allclose(Sp[0], 0, atol=1e-13)
+    np.testing.assert_allclose(Sp[2], 0, atol=1e-13)
+    seps = np.arange(1, N // 2 + 1)
+    Sx = oracle.sf_x(u[0], 2, 4, seps)
+    a = seps * 2 * np.pi / N
+    np.testing.assert_allclose(Sx[0], 2 * np.sin(a / 2) ** 2, atol=1e-13)
+    np.testing.assert_allclose(Sx[2], 6 * np.sin(a / 2) ** 4, atol=1e-13)
+
+
+def test_structure_functions_brute_force(oracle):
+    N = 8
+    rng = np.random.default_rng(4)
+    u = rng.standard_normal((3, N, N, N))
+    Sp, r = oracle.sp_calculate(u, 5)
+    for ir, rr in enumerate(r):
+        dx = np.roll(u[0], -rr, axis=2) - u[0]
+        dy = np.roll(u[1], -rr, axis=1) - u[1]
+        dz = np.roll(u[2], -rr, axis=0) - u[2]
+        for p in range(1, 6):
+            ref = np.mean(dx ** p + dy ** p + dz ** p) / 3
+            assert Sp[p - 1, ir] == pytest.approx(ref, rel=1e-12, abs=1e-14)
+
+
+# ---- geometry and deposition ------------------------------------------------------------------
+def straight_tube(N, npts, y0, z0, sigma, gamma=1.0):
+    x = -np.pi + (np.arange(npts) + 0.5) * 2 * np.pi / npts
+    t = Tubes(np.array([npts], np.int32), x, np.full(npts, y0), np.full(npts, z0), np.array([sigma]),
+              np.array([gamma]), np.array([1], np.int32), np.array([0], np.int32))
+    return t
+
+
+def test_straight_tube_closed_form_profile(oracle):
+    """A straight periodic tube along x: omega_x = Gamma [e^{-rho^2/2s^2} - e^{-R^2/2s^2}]/(2 pi s^2)
+    for rho < R, 0 elsewhere; omega_y = omega_z = 0; every cell near the axis is hit once."""
+    N, sig, G = 32, 0.15, 1.3
+    y0, z0 = 0.1234, -0.4321
+    t = straight_tube(N, 40, y0, z0, sig, G)
+    P = params(oracle, N, 1, ntf_pre=200)
+    geo = oracle.Geometry(P, t.npoint, t.cx, t.cy, t.cz)
+    om, hits = geo.deposit(t.sigma, t.gamma, t.n_T)
+    X, Y, Z = grid(N)
+    rho = np.sqrt((Y - y0) ** 2 + (Z - z0) ** 2)
+    R = 3 * sig
+    expect = np.where(rho < R, G * (np.exp(-rho ** 2 / (2 * sig ** 2)) - np.exp(-R ** 2 / (2 * sig ** 2)))
+                      / (2 * np.pi * sig ** 2), 0.0)
+    np.testing.assert_allclose(om[0], expect, atol=1e-10)
+    np.testing.assert_allclose(om[1:], 0, atol=1e-10)
+    assert hits == N * int(np.count_nonzero(rho[0, :, 0] * 0 + rho[:, :, 0] < R))
+
+
+def test_circulation_of_straight_tube(oracle):
+    """Flux through a cross-section equals Gamma [1 - e^{-a}(1 + a)], a = R^2/2s^2, to O(dx^2)."""
+    N, sig, G = 64, 0.3, 1.0
+    t = straight_tube(N, 64, 0.05, -0.07, sig, G)
+    geo = oracle.Geometry(params(oracle, N, 1, ntf_pre=100), t.npoint, t.cx, t.cy, t.cz)
+    om, _ = geo.deposit(t.sigma, t.gamma, t.n_T)
+    dx = 2 * np.pi / N
+    flux = om[0][:, :, 5].sum() * dx * dx
+    a = 4.5
+    assert flux == pytest.approx(G * (1 - np.exp(-a) * (1 + a)), rel=2e-3)
+
+
+def test_circle_turning_angle_and_ntf(oracle):
+    """A circle of radius a with np uniform points turns by 2*pi/np per section, so
+    ntf = ceil(ntf_times/np) + 1 (tube.f90:641)."""
+    for npts in (16, 37, 60):
+        a = 1.1
+        phi = 2 * np.pi * np.arange(npts) / npts
+        t = Tubes(np.array([npts], np.int32), a * np.cos(phi), a * np.sin(phi), np.full(npts, 0.3),
+                  np.array([0.1]), np.array([1.0]), np.array([1], np.int32), np.array([0], np.int32))
+        geo = oracle.Geometry(params(oracle, 32, 1), t.npoint, t.cx, t.cy, t.cz)
+        np.testing.assert_allclose(geo.theta[:-1], 2 * np.pi / npts, rtol=2e-3)
+        expect = int(np.ceil(50 / npts)) + 1
+        assert np.all(geo.ntf == expect), (npts, geo.ntf)
+        # sub-points lie on the circle
+        r = np.hypot(geo.ci[:, 0], geo.ci[:, 1])
+        np.testing.assert_allclose(r, a, rtol=1e-3)
+
+
+def test_translation_by_whole_cells_shifts_the_field(oracle):
+    """Shifting every centerline point by k cells (with the periodic wrap, across the x = +-pi
+    seam) shifts the vorticity by k cells."""
+    N = 24
+    p, t = fbb.config_small(N=N, seed=3, ntubes=(1, 1), npts=(40, 30), radii=(0.6, 0.35), ntf_pre=60)
+    P = params(oracle, N, t.nline, ntf_pre=60)
+    geo = oracle.Geometry(P, t.npoint, t.cx, t.cy, t.cz)
+    om, hits = geo.deposit(t.sigma, t.gamma, t.n_T)
+    k = 5
+    dx = 2 * np.pi / N
+    cx2 = np.mod(t.cx + k * dx + np.pi, 2 * np.pi) - np.pi
+    geo2 = oracle.Geometry(P, t.npoint, cx2, t.cy, t.cz)
+    om2, hits2 = geo2.deposit(t.sigma, t.gamma, t.n_T)
+    np.testing.assert_allclose(om2, np.roll(om, k, axis=3), atol=1e-9 * np.abs(om).max())
+
+
+def test_bounding_boxes_bracket_the_cutoff(oracle):
+    """lo/hi are the floor/ceiling of the cut-off interval in 1-based mesh index units
+    (tube.f90:793-798): mesh(lo) <= min - R < mesh(lo) + dx and mesh(hi) - dx < max + R <= mesh(hi)."""
+    N = 32
+    p, t = fbb.config_small(N=N, seed=9, sigma_ratio=4.0)
+    geo = oracle.Geometry(params(oracle, N, t.nline, sigma_ratio=4.0, ntf_pre=100), t.npoint, t.cx, t.cy, t.cz)
+    b = geo.bins(t.sigma, t.n_T)
+    dx = 2 * np.pi / N
+    mn = np.minimum(b["ci0"], b["ci1"]) - b["R"][:, None]
+    mx = np.maximum(b["ci0"], b["ci1"]) + b["R"][:, None]
+    mlo = (b["lo"] - 1) * dx - np.pi
+    mhi = (b["hi"] - 1) * dx - np.pi
+    eps = 1e-12
+    assert np.all(mlo <= mn + eps) and np.all(mn < mlo + dx + eps)
+    assert np.all(mhi >= mx - eps) and np.all(mx > mhi - dx - eps)
+    # unwrap (tube.f90:779-785): the two ends are within pi of each other per axis
+    assert np.all(np.abs(b["ci1"] - b["ci0"]) <= np.pi)
+
+
+def test_brute_force_predicate_matches_deposit_hits(oracle):
+    """Summing the brute-force predicate over every cell of every box equals the number of
+    deposits, and no cell outside a box ever passes."""
+    N = 16
+    p, t = fbb.config_small(N=N, seed=4, ntubes=(1, 1), npts=(24, 20), radii=(0.7, 0.5), ntf_pre=40)
+    P = params(oracle, N, t.nline, ntf_pre=40)
+    geo = oracle.Geometry(P, t.npoint, t.cx, t.cy, t.cz)
+    b = geo.bins(t.sigma, t.n_T)
+    _, hits = geo.deposit(t.sigma, t.gamma, t.n_T)
+    total = 0
+    for s in range(0, geo.nseg, 7):
+        lo, hi = b["lo"][s], b["hi"][s]
+        for kp in range(lo[2], hi[2] + 1):
+            for jp in range(lo[1], hi[1] + 1):
+                for ip in range(lo[0], hi[0] + 1):
+                    total += geo.pair_hits(s, ip, jp, kp)
+        # just outside the box the predicate is false by construction of the box test
+        assert not geo.pair_hits(s, lo[0] - 1, lo[1], lo[2])
+    # full count over all sub-segments
+    full = 0
+    for s in range(geo.nseg):
+        lo, hi = b["lo"][s], b["hi"][s]
+        I, J, K = np.meshgrid(np.arange(lo[0], hi[0] + 1), np.arange(lo[1], hi[1] + 1),
+                              np.arange(lo[2], hi[2] + 1), indexing="ij")
+        full += sum(geo.pair_hits(s, int(i), int(j), int(k)) for i, j, k in zip(I.ravel(), J.ravel(), K.ravel()))
+    assert full == hits
+    assert total > 0
+
+
+def test_sigma_func_closed_form(oracle):
+    """sigma(zeta) = smin + (smax - smin)(sin(w zeta) + 1)/2, w = floor(L/pi + 1/2) n_T."""
+    L, smin, smax = 7.3, 0.1, 0.4
+    for z in (0.0, 0.3, 2.0):
+        w = np.floor(L / np.pi + 0.5) * 2
+        assert oracle.sigma(z, L, smax, smin, 2) == pytest.approx(smin + (smax - smin) * (np.sin(w * z) + 1) / 2, rel=1e-15)
+
+
+def test_pipeline_normalises_to_unit_rms(oracle):
+    p, t = fbb.config_small(N=16, ntf_pre=50, npts=(30, 24))
+    om, u, st = oracle.run(params(oracle, 16, t.nline, ntf_pre=50), t.npoint, t.cx, t.cy, t.cz, t.sigma,
+                           t.gamma, t.n_T)
+    assert 0.5 * np.mean(np.sum(u ** 2, axis=0)) == pytest.approx(1.5, rel=1e-12)
+    assert st[4] > 0
