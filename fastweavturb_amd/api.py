@@ -39,7 +39,7 @@ class _Params(C.Structure):
 class _SegStats(C.Structure):
     _fields_ = [("npoint_all", C.c_int64), ("nsec_all", C.c_int64), ("nsub_all", C.c_int64),
                 ("nseg_all", C.c_int64), ("ntf_max", C.c_int32), ("reserved", C.c_int32),
-                ("dtf_min", C.c_double), ("ntf_sum", C.c_int64)]
+                ("dtf_min", C.c_double), ("ntf_sum", C.c_int64), ("theta_exact_sections", C.c_int64)]
 
 
 class _DepStats(C.Structure):
@@ -194,9 +194,10 @@ class WovenGPU:
         return e.value, u.value, en.value
 
     def stage_times(self) -> np.ndarray:
-        ms = np.zeros(4)
-        self._L.wt_stage_times(self._h, _dp(ms), 4)
-        return ms
+        """[segments, bin, deposit, biot_savart] of the last generate, then per-kernel ms."""
+        ms = np.full(32, -1.0)
+        n = self._check(self._L.wt_stage_times(self._h, _dp(ms), len(ms)))
+        return ms[:n]
 
     # --- statistics ----------------------------------------------------------------------
     def spectrum(self, field: int = 1) -> np.ndarray:

@@ -43,6 +43,29 @@ def _compile(src: str, extra: list[str]) -> str:
     return obj
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN = os.path.join(HERE, "fortran")
+HOST_EXE = os.path.join(FORTRAN, "weave_host")
+
+
+def build_fortran_host(verbose: bool = False) -> str | None:
+    """Compile the Fortran drop-in host (ISO_C_BINDING) against libwovengpu.so with flang."""
+    if not os.path.exists(FLANG):
+        if verbose:
+            print("flang not found; Fortran host not built")
+        return None
+    cmd = [FLANG, "-O2", "-J", BUILD, "-o", HOST_EXE + ".tmp",
+           os.path.join(FORTRAN, "wovengpu_mod.f90"), os.path.join(FORTRAN, "weave.f90"),
+           f"-L{HERE}", "-lwovengpu", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"flang failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(HOST_EXE + ".tmp", HOST_EXE)
+    if verbose:
+        print("built", HOST_EXE)
+    return HOST_EXE
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
@@ -55,6 +78,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran_host(verbose)
     return LIB
 
 

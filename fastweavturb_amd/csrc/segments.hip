@@ -148,11 +148,22 @@ __global__ void k_spline(Geo g) {
     g.czr[s] = e.cze1;
 }
 
-// tube.f90:625-642: turning angle from ntf_pre+1 samples (sequential sum, reference order),
-// ntf = ceiling(ntf_times*theta/pi2) + 1, dtf = (cze_pl(t_i+1)-cze_pl(t_i))/ntf.
-__global__ void k_theta(Geo g, Run r, int32_t *__restrict__ err) {
-    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s >= g.nsec_all) return;
+// Integrand of the turning angle: |c' x c''| / |c'|^2 (tube.f90:629-639).
+__device__ __forceinline__ double turn_rate(double zz, const double *coe) {
+    double dc[3], ddc[3], cr[3];
+    spline_d1(zz, coe, dc);
+    double ndc = norm3(dc);
+    spline_d2(zz, coe, ddc);
+    cr[0] = dc[1] * ddc[2] - dc[2] * ddc[1];
+    cr[1] = dc[2] * ddc[0] - dc[0] * ddc[2];
+    cr[2] = dc[0] * ddc[1] - dc[1] * ddc[0];
+    double ncr = norm3(cr);
+    return ncr / p2(ndc);
+}
+
+// tube.f90:625-642 exactly: ntf_pre+1 samples summed in order, ntf = ceiling(ntf_times*theta/pi2)
+// + 1, dtf = (cze_pl(t_i+1)-cze_pl(t_i))/ntf. Returns false if theta is not finite.
+__device__ __forceinline__ bool theta_exact(const Geo &g, const Run &r, int64_t s) {
     int l = g.sec_tube[s];
     double LD = g.Len_Disc[l];
     double coe[18];
@@ -175,11 +186,81 @@ __global__ void k_theta(Geo g, Run r, int32_t *__restrict__ err) {
         theta = theta + ncr / p2(ndc) * dtfp * LD;
     }
     double q = ceil((double)r.ntf_times * theta / kPi2);
-    int32_t ntf;
-    if (!(q >= 0.0 && q < 1.0e7)) { atomicOr(err, 1); ntf = 1; }
-    else ntf = (int32_t)q + 1;
+    bool ok = q >= 0.0 && q < 1.0e7;
+    int32_t ntf = ok ? (int32_t)q + 1 : 1;
     g.ntf[s] = ntf;
     g.dtf[s] = (czr - czl) / (1.0 * ntf);
+    return ok;
+}
+
+__global__ void k_theta(Geo g, Run r, int32_t *__restrict__ err) {
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= g.nsec_all) return;
+    if (!theta_exact(g, r, s)) atomicOr(err, 1);
+}
+
+// Gauss-Legendre nodes (positive half) and weights on [-1, 1], 8 points.
+__constant__ double kGL8x[4] = {0.1834346424956498049394761, 0.5255324099163289858177390,
+                                0.7966664774136267395915539, 0.9602898564975362316835609};
+__constant__ double kGL8w[4] = {0.3626837833783619829651504, 0.3137066458778872873379622,
+                                0.2223810344533744705443560, 0.1012285362903762591525314};
+
+__device__ __forceinline__ double gl8(const double *coe, double a, double b) {
+    double m = 0.5 * (a + b), hw = 0.5 * (b - a), acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        double d = hw * kGL8x[i];
+        acc += kGL8w[i] * (turn_rate(m - d, coe) + turn_rate(m + d, coe));
+    }
+    return acc * hw;
+}
+
+// Fast turning angle with an exactness guard. The reference's sum of ntf_pre+1 samples equals
+// the trapezoid rule plus half of each end sample, and for a smooth integrand the trapezoid rule
+// equals the integral up to O(h^2/ntf_pre^2) (Euler-Maclaurin). The integral is estimated with
+// Gauss-Legendre rules of 8 and 2x8 points; the integer ntf = ceil(ntf_times*theta/2pi) + 1 is
+// accepted only when the two rules agree to 1e-9 and ntf_times*theta/2pi lies farther than
+// max(1e-6, 2/ntf_pre^2) (relative) from an integer. Every other section is re-evaluated with
+// the reference's exact sum (k_theta_listed), so ntf and dtf match the reference bit for bit.
+__global__ void k_theta_fast(Geo g, Run r, int32_t *__restrict__ redo, unsigned int *__restrict__ nredo) {
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= g.nsec_all) return;
+    int l = g.sec_tube[s];
+    double LD = g.Len_Disc[l];
+    double coe[18];
+#pragma unroll
+    for (int k = 0; k < 18; k++) coe[k] = g.coe[18 * s + k];
+    double czl = g.czl[s], czr = g.czr[s];
+    const int M = r.ntf_pre;
+    double dtfp = (czr - czl) / (1.0 * M);
+    double z0 = 0.0;
+    double zM = ((czl + dtfp * (double)M) - czl) * LD;
+    double dz = dtfp * LD;
+    double I8 = gl8(coe, z0, zM);
+    double zh = 0.5 * zM;
+    double I16 = gl8(coe, z0, zh) + gl8(coe, zh, zM);
+    double f0 = turn_rate(z0, coe), fM = turn_rate(zM, coe);
+    double theta = I16 + 0.5 * dz * (f0 + fM);
+    double x = (double)r.ntf_times * theta / kPi2;
+    double tol = fmax(1e-6, 2.0 / ((double)M * M)) * fmax(1.0, x);
+    double fl = floor(x);
+    bool ok = isfinite(x) && x > tol && (x - fl) > tol && (fl + 1.0 - x) > tol &&
+              fabs(I8 - I16) <= 1e-9 * fabs(I16) && zM > 0.0;
+    if (ok) {
+        int32_t ntf = (int32_t)fl + 2;  // ceil(x) + 1 for non-integer x
+        g.ntf[s] = ntf;
+        g.dtf[s] = (czr - czl) / (1.0 * ntf);
+    } else {
+        unsigned int k = atomicAdd(nredo, 1u);
+        redo[k] = (int32_t)s;
+    }
+}
+
+__global__ void k_theta_listed(Geo g, Run r, const int32_t *__restrict__ redo,
+                               const unsigned int *__restrict__ nredo, int32_t *__restrict__ err) {
+    unsigned int n = *nredo;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (!theta_exact(g, r, redo[i])) atomicOr(err, 1);
 }
 
 __global__ void k_fill_sub_sec(Geo g) {
@@ -292,7 +373,19 @@ hipError_t launch_segments(wt_ctx *c) {
     WT_HIP(dalloc(c, c->seg_allocs, &err, 1));
     WT_HIP(hipMemsetAsync(err, 0, sizeof(int32_t), st));
     kt_begin(c, kSlotTheta);
-    hipLaunchKernelGGL(k_theta, dim3(grid_for(g.nsec_all, 128)), dim3(128), 0, st, g, c->run, err);
+    if (c->theta_exact_all) {
+        hipLaunchKernelGGL(k_theta, dim3(grid_for(g.nsec_all, 128)), dim3(128), 0, st, g, c->run, err);
+    } else {
+        int32_t *redo = nullptr;
+        unsigned int *nredo = nullptr;
+        WT_HIP(dalloc(c, c->seg_allocs, &redo, g.nsec_all));
+        WT_HIP(dalloc(c, c->seg_allocs, &nredo, 1));
+        WT_HIP(hipMemsetAsync(nredo, 0, sizeof(unsigned int), st));
+        hipLaunchKernelGGL(k_theta_fast, dim3(grid_for(g.nsec_all, 128)), dim3(128), 0, st, g, c->run, redo, nredo);
+        hipLaunchKernelGGL(k_theta_listed, dim3(1024), dim3(64), 0, st, g, c->run, (const int32_t *)redo,
+                           (const unsigned int *)nredo, err);
+        WT_HIP(hipMemcpyAsync(&c->theta_redo, nredo, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    }
     kt_end(c, kSlotTheta);
     // sub-point layout
     void *tmp = nullptr;

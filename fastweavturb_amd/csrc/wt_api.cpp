@@ -160,6 +160,10 @@ int wt_create(wt_ctx **out, int32_t n, int32_t precision, int32_t device) {
     if ((e = hipMalloc((void **)&c->E_u, sizeof(double) * (c->nek + 1))) != hipSuccess) return fail(e, "spectra");
     if ((e = hipMemset(c->omega, 0, 3 * N3 * es)) != hipSuccess) return fail(e, "omega");
     if ((e = hipMemset(c->u, 0, 3 * N3 * es)) != hipSuccess) return fail(e, "u");
+    {
+        const char *te = getenv("WT_THETA_EXACT");
+        c->theta_exact_all = te && te[0] == '1';
+    }
     c->run.N = n;
     c->run.dx = wt::kPi2 / n;  // lx/nx (tube.f90:274)
     c->run.start = -wt::kPi;
@@ -321,6 +325,7 @@ int wt_build_segments(wt_ctx *c, wt_seg_stats *out) {
     c->seg_stats.npoint_all = c->geo.npoint_all;
     c->seg_stats.nsec_all = c->geo.nsec_all;
     c->seg_stats.nsub_all = c->geo.nsub_all;
+    c->seg_stats.theta_exact_sections = c->theta_exact_all ? c->geo.nsec_all : (int64_t)c->theta_redo;
     c->have_segments = true;
     if (out) *out = c->seg_stats;
     return WT_OK;

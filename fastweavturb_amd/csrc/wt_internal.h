@@ -105,6 +105,8 @@ struct wt_ctx {
     wt_seg_stats seg_stats{};
     wt_deposit_stats dep_stats{};
     int fixed_shift = 0;
+    bool theta_exact_all = false;         // WT_THETA_EXACT=1: reference sum for every section
+    unsigned int theta_redo = 0;          // sections re-evaluated exactly by the fast path
     double stage_ms[4] = {0, 0, 0, 0};
     hipEvent_t ev[5] = {};
     // per-kernel timing slots (see wt::kSlot*), events recorded on `stream`

@@ -60,6 +60,8 @@ typedef struct {
     int32_t reserved;
     double dtf_min;       /* min over all sections (tube.f90:737)              */
     int64_t ntf_sum;      /* sum of ntf over all sections                      */
+    int64_t theta_exact_sections; /* sections whose turning angle used the full
+                                     reference sum (all of them if WT_THETA_EXACT=1) */
 } wt_seg_stats;
 
 /* Counters of the deposition (wt_deposit). */

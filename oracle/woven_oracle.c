@@ -440,6 +440,9 @@ int or_deposit_range(const or_params *p, const double *sigma_line, const double 
         for (int64_t seg = sa; seg < sb; seg++) {
             or_bin_rec b;
             bin_one(p, smin, n_T, g, l, seg, &b);
+            for (int c = 0; c < 3; c++) /* Fortran MOD of a negative index would address outside the field */
+                if (b.lo[c] < 1 - N || b.hi[c] > 2 * N)
+                    return fail("a tube's cut-off radius spans more than the periodic box");
             int64_t sec = g->seg_sec[seg];
             int64_t q = g->sp_base[sec] + g->seg_j[seg];
             const double *coe = g->coe + 18 * sec;

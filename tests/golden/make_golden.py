@@ -23,7 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def small16():
-    p, t = fbb.config_small(N=16, seed=21, ntubes=(1, 2), npts=(30, 24), radii=(0.9, 0.5),
+    p, t = fbb.config_small(N=16, seed=21, ntubes=(1, 2), npts=(30, 24), radii=(0.3, 0.15),
                             sigma_ratio=4.0, ntf_pre=200)
     p.eta_t = 0.5
     return p, t
