@@ -166,8 +166,8 @@ def main():
     value = p.N ** 3 / (ms_per_step * 1e-3) / 1e6
     times = g.stage_times()
     kernels = {SLOT_NAMES[i]: float(times[4 + i]) for i in range(len(SLOT_NAMES)) if times[4 + i] > 0}
-    seg = g.seg_stats
-    dep = g.deposit()  # counters of one more deposition (same inputs), outside the timed region
+    seg = g.segment_stats()
+    dep = g.deposit_stats()
     log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step; stages {np.round(stage_ms / args.steps, 2).tolist()}")
     log(f"[bench] kernels ms {json.dumps({k: round(v, 3) for k, v in kernels.items()})}")
     log(f"[bench] geometry {json.dumps(seg)}; deposit {json.dumps(dep)}")

@@ -92,6 +92,11 @@ def load_library():
     L.wt_get_subpoints.argtypes = [vp, dp, dp]
     L.wt_get_bins.argtypes = [vp, ip, dp]
     L.wt_stage_times.argtypes = [vp, dp, i32]
+    L.wt_create_batch.argtypes = [C.POINTER(vp), i32, ip, i32, i32]
+    L.wt_select_instance.argtypes = [vp, i32]
+    L.wt_instance_scalars.argtypes = [vp, dp, dp, dp]
+    L.wt_get_seg_stats.argtypes = [vp, C.POINTER(_SegStats)]
+    L.wt_get_deposit_stats.argtypes = [vp, C.POINTER(_DepStats)]
     _lib = L
     return L
 
@@ -117,19 +122,49 @@ def sp_separations(N: int) -> np.ndarray:
 
 
 class WovenGPU:
-    """One device context: an N^3 grid in FP32 or FP64 on one GPU."""
+    """One device context: an N^3 grid in FP32 or FP64 on one GPU, or, with `sizes`, an
+    ensemble of independent fields of grid sizes `sizes` processed together."""
 
-    def __init__(self, N: int, precision: int = 32, device: int = 0):
+    def __init__(self, N: int = 0, precision: int = 32, device: int = 0, sizes=None):
         self._L = load_library()
         h = C.c_void_p()
-        rc = self._L.wt_create(C.byref(h), N, precision, device)
+        if sizes is None:
+            rc = self._L.wt_create(C.byref(h), N, precision, device)
+            sizes = [N]
+        else:
+            arr = np.ascontiguousarray(sizes, dtype=np.int32)
+            rc = self._L.wt_create_batch(C.byref(h), len(arr), _ip(arr), precision, device)
         if rc != 0:
             raise WovenError(rc, self._L.wt_last_error(None).decode())
         self._h = h
-        self.N = N
+        self.sizes = [int(n) for n in sizes]
+        self.N = self.sizes[0]
         self.precision = precision
         self.dtype = np.float64 if precision == 64 else np.float32
         self.nline = 0
+        self.instance = 0
+
+    def select(self, inst: int):
+        """Address instance `inst` with the per-instance calls."""
+        self._check(self._L.wt_select_instance(self._h, inst))
+        self.instance = inst
+        self.N = self.sizes[inst]
+        return self
+
+    def scalars(self):
+        e, u, en = C.c_double(), C.c_double(), C.c_double()
+        self._check(self._L.wt_instance_scalars(self._h, C.byref(e), C.byref(u), C.byref(en)))
+        return e.value, u.value, en.value
+
+    def segment_stats(self) -> dict:
+        s = _SegStats()
+        self._check(self._L.wt_get_seg_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _SegStats._fields_ if k != "reserved"}
+
+    def deposit_stats(self) -> dict:
+        s = _DepStats()
+        self._check(self._L.wt_get_deposit_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _DepStats._fields_}
 
     def close(self):
         if getattr(self, "_h", None):

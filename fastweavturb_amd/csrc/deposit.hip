@@ -22,7 +22,6 @@ namespace wt {
 
 constexpr int kWaves = 4;             // waves per block
 constexpr int kQCap = 128;            // LDS hit queue entries per wave
-constexpr int kColsPerItemDep = 256;  // must match the binning layout
 
 struct Hit {
     uint32_t q;
@@ -58,7 +57,7 @@ __device__ __forceinline__ int wave_max_i(int v) {
 // ---- hit evaluation, FP64: tube.f90:819-892 in the reference order ----------------------
 __device__ void hit_math_f64(const DepArgs &A, const Hit &h, const SubSeg &sg, double *v) {
     const Geo &g = A.g;
-    const Run &r = A.r;
+    const Run &r = g.runs[g.tube_inst[sg.tube]];
     const int l = sg.tube;
     const double LD = g.Len_Disc[l], len = g.length[l], Gamma = g.gamma[l];
     const double smin = g.sigma[l], smax = r.sigma_ratio * smin;
@@ -66,7 +65,7 @@ __device__ void hit_math_f64(const DepArgs &A, const Hit &h, const SubSeg &sg, d
     const double *coe = g.coe + 18 * sg.sec;
     const double sfirst = g.czl[sg.sec];
     const double dzeta = r.dzeta_sup < g.dtf[sg.sec] ? r.dzeta_sup : g.dtf[sg.sec];
-    double X[3] = {mesh(h.ip, r), mesh(h.jp, r), mesh(h.kp, r)};
+    double X[3] = {mesh(h.ip, sg), mesh(h.jp, sg), mesh(h.kp, sg)};
     double a0 = X[0] - sg.ci0[0], a1 = X[1] - sg.ci0[1], a2 = X[2] - sg.ci0[2];
     double t0 = (a0 * sg.tci0[0] + a1 * sg.tci0[1]) + a2 * sg.tci0[2];
     double t1 = ((X[0] - sg.ci1[0]) * sg.tci0[0] + (X[1] - sg.ci1[1]) * sg.tci0[1]) +
@@ -137,19 +136,20 @@ __device__ void eval_hit_f64(const DepArgs &A, const Hit &h) {
     subseg_load(A.g, A.r, h.q, sg);
     double v[3];
     hit_math_f64(A, h, sg, v);
-    const int N = A.r.N;
+    const int N = sg.N;
     int64_t idx = cell_index(h, N);
     int64_t N3 = (int64_t)N * N * N;
-    double *om = (double *)A.omega;
+    double *om = (double *)A.omega + sg.field_off;
     unsafeAtomicAdd(om + idx, v[0]);
     unsafeAtomicAdd(om + N3 + idx, v[1]);
     unsafeAtomicAdd(om + 2 * N3 + idx, v[2]);
 }
 
 // Exact, order-independent accumulation of one contribution in 64-bit fixed point.
-__device__ __forceinline__ void accumulate_fixed(const DepArgs &A, const Hit &h, const double *v) {
-    const int N = A.r.N;
-    int64_t idx = cell_index(h, N);
+__device__ __forceinline__ void accumulate_fixed(const DepArgs &A, const SubSeg &sg, const Hit &h,
+                                                 const double *v) {
+    const int N = sg.N;
+    int64_t idx = cell_index(h, N) + sg.field_off;
     int64_t N3 = (int64_t)N * N * N;
     bool ovf = false;
 #pragma unroll
@@ -173,17 +173,18 @@ __device__ void eval_hit_f32(const DepArgs &A, const Hit &h) {
     const Geo &g = A.g;
     const Run &r = A.r;
     SubSeg sg;
-    subseg_load(g, r, h.q, sg);
+    subseg_load(g, A.r, h.q, sg);
+    const Run &ri = g.runs[g.tube_inst[sg.tube]];
     double v[3];
-    if (r.eta != 0.0 || r.sigma_ratio != 1.0) {
+    if (ri.eta != 0.0 || ri.sigma_ratio != 1.0) {
         hit_math_f64(A, h, sg, v);
-        accumulate_fixed(A, h, v);
+        accumulate_fixed(A, sg, h, v);
         return;
     }
     const int l = sg.tube;
     const double LD = g.Len_Disc[l];
     const double *coe = g.coe + 18 * sg.sec;
-    double X[3] = {mesh(h.ip, r), mesh(h.jp, r), mesh(h.kp, r)};
+    double X[3] = {mesh(h.ip, sg), mesh(h.jp, sg), mesh(h.kp, sg)};
     double a[3] = {X[0] - sg.ci0[0], X[1] - sg.ci0[1], X[2] - sg.ci0[2]};
     double t0 = (a[0] * sg.tci0[0] + a[1] * sg.tci0[1]) + a[2] * sg.tci0[2];
     double t1 = ((X[0] - sg.ci1[0]) * sg.tci0[0] + (X[1] - sg.ci1[1]) * sg.tci0[1]) +
@@ -219,7 +220,7 @@ __device__ void eval_hit_f32(const DepArgs &A, const Hit &h) {
 #pragma unroll
         for (int c = 0; c < 3; c++) v[c] = (double)(tg * dcf[c]);
     }
-    accumulate_fixed(A, h, v);
+    accumulate_fixed(A, sg, h, v);
 }
 
 template <bool F64>
@@ -257,8 +258,9 @@ __global__ __launch_bounds__(kWaves * 64) void k_deposit(DepArgs A) {
             int64_t blk = it - A.b.item_off[q];
             int64_t ncol = A.b.col_count[q];
             subseg_load(g, r, q, sg);
-            c_begin = blk * kColsPerItemDep;
-            int64_t c_end = c_begin + kColsPerItemDep < ncol ? c_begin + kColsPerItemDep : ncol;
+            const int64_t cpi = A.b.cols_per_item;
+            c_begin = blk * cpi;
+            int64_t c_end = c_begin + cpi < ncol ? c_begin + cpi : ncol;
             ncl = (int)(c_end - c_begin);
             ax = dominant_axis(sg.tci0);
             bx = (ax + 1) % 3;
@@ -266,14 +268,14 @@ __global__ __launch_bounds__(kWaves * 64) void k_deposit(DepArgs A) {
             nb = sg.hi[bx] - sg.lo[bx] + 1;
             // plane index functions: ia*(b, c) = ia0 - (A0b*(Xb - ci0_b) + A0c*(Xc - ci0_c))
             double t0a = sg.tci0[ax], t1a = sg.tci1[ax];
-            ia0 = (sg.ci0[ax] - r.start) / r.dx + 1.0;
-            A0b = sg.tci0[bx] / (t0a * r.dx);
-            A0c = sg.tci0[cxs] / (t0a * r.dx);
+            ia0 = (sg.ci0[ax] - sg.start) / sg.dx + 1.0;
+            A0b = sg.tci0[bx] / (t0a * sg.dx);
+            A0c = sg.tci0[cxs] / (t0a * sg.dx);
             s0sgn = t0a > 0.0 ? 1.0 : -1.0;
             if (t1a != 0.0) {
-                ia1 = (sg.ci1[ax] - r.start) / r.dx + 1.0;
-                A1b = sg.tci1[bx] / (t1a * r.dx);
-                A1c = sg.tci1[cxs] / (t1a * r.dx);
+                ia1 = (sg.ci1[ax] - sg.start) / sg.dx + 1.0;
+                A1b = sg.tci1[bx] / (t1a * sg.dx);
+                A1c = sg.tci1[cxs] / (t1a * sg.dx);
                 s1sgn = t1a > 0.0 ? 1.0 : -1.0;
             } else {
                 s1sgn = 0.0;
@@ -281,14 +283,17 @@ __global__ __launch_bounds__(kWaves * 64) void k_deposit(DepArgs A) {
         }
         ncols_seen += (unsigned long long)ncl;
         const int maxc = wave_max_i(ncl);
-        int64_t kcol = c_begin;
-        for (int cc = 0; cc < maxc; cc++, kcol++) {
+        // column (b, c) of the box, advanced incrementally from the item's first column
+        int32_t bi = sg.lo[bx] + (int32_t)(c_begin % nb);
+        int32_t ci_ = sg.lo[cxs] + (int32_t)(c_begin / nb);
+        for (int cc = 0; cc < maxc; cc++) {
             int alo = 1, ahi = 0;
-            int32_t bi = 0, ci_ = 0;
+            if (cc > 0) {
+                bi++;
+                if (bi > sg.hi[bx]) { bi = sg.lo[bx]; ci_++; }
+            }
             if (cc < ncl) {
-                bi = sg.lo[bx] + (int32_t)(kcol % nb);
-                ci_ = sg.lo[cxs] + (int32_t)(kcol / nb);
-                double Xb = mesh(bi, r), Xc = mesh(ci_, r);
+                double Xb = mesh(bi, sg), Xc = mesh(ci_, sg);
                 const double lo_a = sg.lo[ax], hi_a = sg.hi[ax];
                 double flo = lo_a, fhi = hi_a;
                 double p0 = ia0 - (A0b * (Xb - sg.ci0[bx]) + A0c * (Xc - sg.ci0[cxs]));
@@ -314,7 +319,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_deposit(DepArgs A) {
                     ijk[ax] = alo + t;
                     ijk[bx] = bi;
                     ijk[cxs] = ci_;
-                    hit = pair_hits(sg, r, ijk[0], ijk[1], ijk[2]);
+                    hit = pair_hits(sg, ijk[0], ijk[1], ijk[2]);
                 }
                 unsigned long long m = __ballot(hit);
                 if (m) {
@@ -370,7 +375,7 @@ __global__ void k_fixed_to_float(const unsigned long long *__restrict__ acc, flo
 
 hipError_t launch_deposit(wt_ctx *c) {
     hipStream_t st = c->stream;
-    const int64_t N3 = (int64_t)c->N * c->N * c->N;
+    const int64_t total = c->field_elems;   // all instances' 3*N^3 blocks
     const bool f64 = c->precision == 64;
     unsigned long long *cnt = nullptr;
     WT_HIP(hipMalloc(&cnt, 4 * sizeof(unsigned long long)));
@@ -384,9 +389,9 @@ hipError_t launch_deposit(wt_ctx *c) {
     A.scal = c->scal;
     A.counters = cnt;
     if (f64) {
-        WT_HIP(hipMemsetAsync(c->omega, 0, sizeof(double) * 3 * N3, st));
+        WT_HIP(hipMemsetAsync(c->omega, 0, sizeof(double) * total, st));
     } else {
-        WT_HIP(hipMemsetAsync(c->acc, 0, sizeof(long long) * 3 * N3, st));
+        WT_HIP(hipMemsetAsync(c->acc, 0, sizeof(long long) * total, st));
         A.fx_scale = ldexp(1.0, c->fixed_shift);
         A.fx_limit = ldexp(1.0, 62) / 64.0;
     }
@@ -406,7 +411,7 @@ hipError_t launch_deposit(wt_ctx *c) {
     if (!f64) {
         kt_begin(c, kSlotFixedToFloat);
         hipLaunchKernelGGL(k_fixed_to_float, dim3(4096), dim3(256), 0, st,
-                           (const unsigned long long *)c->acc, (float *)c->omega, 3 * N3,
+                           (const unsigned long long *)c->acc, (float *)c->omega, total,
                            ldexp(1.0, -c->fixed_shift));
         kt_end(c, kSlotFixedToFloat);
         WT_HIP(hipGetLastError());

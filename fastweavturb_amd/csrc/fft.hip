@@ -219,7 +219,8 @@ __global__ void k_fft_x_r2c(const T *__restrict__ f, C2<T> *__restrict__ spec, F
 // ---- x pass, inverse: two half spectra -> two real lines, times inv_up; omega too --------
 template <typename T>
 __global__ void k_fft_x_c2r(const C2<T> *__restrict__ spec, T *__restrict__ u, T *__restrict__ omega, FftPlan pl,
-                            int pairs_per_block, int64_t npairs, const DeviceScalars *__restrict__ sc) {
+                            int pairs_per_block, int64_t npairs, const DeviceScalars *__restrict__ sc,
+                            int64_t pairs_per_inst) {
     extern __shared__ unsigned char smem_raw[];
     C2<T> *buf = (C2<T> *)smem_raw;
     const int N = pl.N, H = N / 2 + 1;
@@ -237,10 +238,11 @@ __global__ void k_fft_x_c2r(const C2<T> *__restrict__ spec, T *__restrict__ u, T
     }
     __syncthreads();
     fft_lds_dif<T, true>(buf, np, N, pl);
-    const T s = (T)sc->inv_up;
+    // a block's pairs may straddle two instances of the group
     for (int i = threadIdx.x; i < np * N; i += blockDim.x) {
         int pp = i / N, x = i - pp * N;
         int64_t line = 2 * (pair0 + pp);
+        const T s = (T)sc[(pair0 + pp) / pairs_per_inst].inv_up;
         C2<T> z = buf[pp * N + pl.rev[x]];
         u[line * N + x] = z.x * s;
         u[(line + 1) * N + x] = z.y * s;
@@ -297,7 +299,8 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl, int W, int nchunk,
     const int N = pl.N, H = N / 2 + 1;
     const int64_t N3h = (int64_t)N * N * H;
     int chunk = blockIdx.x % nchunk;
-    int y = blockIdx.x / nchunk;
+    int y = (blockIdx.x / nchunk) % N;
+    spec += (int64_t)3 * N3h * (blockIdx.x / ((int64_t)nchunk * N));  // instance of the group
     int kx0 = chunk * W;
     int w_valid = H - kx0 < W ? H - kx0 : W;
     for (int i = threadIdx.x; i < 2 * nek + 2; i += blockDim.x) shell[i] = 0.0;
@@ -378,14 +381,20 @@ __global__ void k_reduce_columns(const double *__restrict__ partials, int64_t ro
                                  double *__restrict__ out) {
     int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= cols) return;
+    partials += (int64_t)blockIdx.y * rows * cols;   // rows of instance blockIdx.y
     double s = 0.0;
     for (int64_t r = 0; r < rows; r++) s += partials[r * cols + col];
-    out[col] = s;
+    out[(int64_t)blockIdx.y * cols + col] = s;
 }
 
 __global__ void k_finalize(const double *__restrict__ sums, int nek, int mode, DeviceScalars *__restrict__ sc,
                            double *__restrict__ E_w, double *__restrict__ E_u) {
-    // sums: [0,nek) w shells, [nek,2nek) u shells, [2nek] sum|w|^2, [2nek+1] sum|u|^2
+    // sums: [0,nek) w shells, [nek,2nek) u shells, [2nek] sum|w|^2, [2nek+1] sum|u|^2; one
+    // block per instance of the group
+    sums += (int64_t)blockIdx.x * (2 * nek + 2);
+    sc += blockIdx.x;
+    E_w += (int64_t)blockIdx.x * nek;
+    E_u += (int64_t)blockIdx.x * nek;
     if (mode == 0) {
         double etot = 0.5 * sums[2 * nek + 1];
         double up = sqrt(2 * etot / 3);
@@ -440,31 +449,42 @@ void fft_reversal(int N, std::vector<int32_t> &rev) {
     }
 }
 
+static const void *twiddles_for(wt_ctx *c, int N) {
+    for (size_t i = 0; i < c->tw_N.size(); i++)
+        if (c->tw_N[i] == N) return c->tw_tab[i];
+    return nullptr;
+}
+
+// Biot-Savart (mode 0) or the velocity spectrum (mode 1) of K instances of equal N whose fields,
+// spectra, scalars and spectra arrays are contiguous from arena position `pos`.
 template <typename T>
-static hipError_t biot_savart_t(wt_ctx *c, int mode) {
+static hipError_t spectral_group(wt_ctx *c, int mode, int pos, int K, bool timed) {
     hipStream_t st = c->stream;
-    const int N = c->N, H = N / 2 + 1;
+    const wt::Instance &I0 = c->inst[c->order[pos]];
+    const int N = I0.N, H = N / 2 + 1;
     FftPlan pl;
     make_plan(N, pl);
-    pl.rev = (const int32_t *)((char *)c->twiddle + sizeof(C2<T>) * N);
-    pl.tw = c->twiddle;
+    pl.tw = twiddles_for(c, N);
+    if (!pl.tw) return hipErrorInvalidValue;
+    pl.rev = (const int32_t *)((const char *)pl.tw + sizeof(C2<T>) * N);
     int dev_lds = 65536;
     (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
     const size_t cs = sizeof(C2<T>);
-    T *field = (T *)(mode == 0 ? c->omega : c->u);
-    C2<T> *spec = (C2<T> *)c->spec;
+    T *field = (T *)(mode == 0 ? c->omega : c->u) + I0.field_off;
+    C2<T> *spec = (C2<T> *)c->spec + I0.spec_off;
     // x forward
     int ppb = (int)(8192 / N) > 1 ? (int)(8192 / N) : 1;  // pairs per block
     while ((size_t)ppb * N * cs > (size_t)dev_lds) ppb /= 2;
     if (ppb < 1) ppb = 1;
-    int64_t npairs = (int64_t)3 * N * N / 2;
+    const int64_t pairs_per_inst = (int64_t)3 * N * N / 2;
+    const int64_t npairs = pairs_per_inst * K;
     unsigned gx = (unsigned)((npairs + ppb - 1) / ppb);
     T scale = (T)(1.0 / ((double)N * N * N));
     size_t shx = (size_t)ppb * N * cs;
     WT_HIP(hipFuncSetAttribute((const void *)k_fft_x_r2c<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shx));
-    kt_begin(c, kSlotFftX);
+    if (timed) kt_begin(c, kSlotFftX);
     hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(gx), dim3(256), shx, st, (const T *)field, spec, pl, ppb, npairs, scale);
-    kt_end(c, kSlotFftX);
+    if (timed) kt_end(c, kSlotFftX);
     WT_HIP(hipGetLastError());
     // y forward
     int Wy = 1;
@@ -473,64 +493,87 @@ static hipError_t biot_savart_t(wt_ctx *c, int mode) {
     size_t shy = (size_t)Wy * N * cs;
     WT_HIP(hipFuncSetAttribute((const void *)k_fft_y<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shy));
     WT_HIP(hipFuncSetAttribute((const void *)k_fft_y<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shy));
-    kt_begin(c, kSlotFftY);
-    hipLaunchKernelGGL((k_fft_y<T, false>), dim3((unsigned)(3 * N * ncy)), dim3(256), shy, st, spec, pl, Wy, ncy);
-    kt_end(c, kSlotFftY);
+    if (timed) kt_begin(c, kSlotFftY);
+    hipLaunchKernelGGL((k_fft_y<T, false>), dim3((unsigned)(3 * K * N * ncy)), dim3(256), shy, st, spec, pl, Wy, ncy);
+    if (timed) kt_end(c, kSlotFftY);
     WT_HIP(hipGetLastError());
     // z fused
-    const int nek = c->nek;
+    const int nek = I0.nek;
     size_t shell_bytes = sizeof(double) * (2 * nek + 2);
     int Wz = 1;
     while (Wz < 16 && 3 * (size_t)(2 * Wz) * N * cs + shell_bytes <= (size_t)dev_lds) Wz *= 2;
     int ncz = (H + Wz - 1) / Wz;
     size_t shz = 3 * (size_t)Wz * N * cs + shell_bytes;
-    int64_t zblocks = (int64_t)N * ncz;
-    size_t need = sizeof(double) * (size_t)zblocks * (2 * nek + 2);
+    const int64_t zrows = (int64_t)N * ncz;    // blocks (partial rows) per instance
+    const int64_t zblocks = zrows * K;
+    const int cols = 2 * nek + 2;
+    size_t need = sizeof(double) * ((size_t)zblocks * cols + (size_t)K * cols);
     if (need > c->partials_cap) {
+        WT_HIP(hipStreamSynchronize(st));
         if (c->partials) (void)hipFree(c->partials);
         c->partials = nullptr;
         c->partials_cap = 0;
         WT_HIP(hipMalloc(&c->partials, need));
         c->partials_cap = need;
     }
+    double *colsum = c->partials + (size_t)zblocks * cols;
     WT_HIP(hipFuncSetAttribute((const void *)k_fft_z<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shz));
-    kt_begin(c, kSlotFftZ);
+    if (timed) kt_begin(c, kSlotFftZ);
     hipLaunchKernelGGL(k_fft_z<T>, dim3((unsigned)zblocks), dim3(256), shz, st, spec, pl, Wz, ncz, mode, nek,
                        c->partials);
-    kt_end(c, kSlotFftZ);
+    if (timed) kt_end(c, kSlotFftZ);
     WT_HIP(hipGetLastError());
-    int cols = 2 * nek + 2;
-    double *colsum = nullptr;
-    WT_HIP(hipMallocAsync((void **)&colsum, sizeof(double) * cols, st));
-    kt_begin(c, kSlotReduce);
-    hipLaunchKernelGGL(k_reduce_columns, dim3((unsigned)((cols + 63) / 64)), dim3(64), 0, st,
-                       (const double *)c->partials, zblocks, cols, colsum);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (const double *)colsum, nek, mode, c->scal, c->E_w,
-                       c->E_u);
-    kt_end(c, kSlotReduce);
+    if (timed) kt_begin(c, kSlotReduce);
+    hipLaunchKernelGGL(k_reduce_columns, dim3((unsigned)((cols + 63) / 64), (unsigned)K), dim3(64), 0, st,
+                       (const double *)c->partials, zrows, cols, colsum);
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)K), dim3(256), 0, st, (const double *)colsum, nek, mode,
+                       c->scal + pos, c->E_w + I0.E_off, c->E_u + I0.E_off);
+    if (timed) kt_end(c, kSlotReduce);
     WT_HIP(hipGetLastError());
-    WT_HIP(hipFreeAsync(colsum, st));
     if (mode != 0) return hipSuccess;
     // y inverse
-    kt_begin(c, kSlotFftYInv);
-    hipLaunchKernelGGL((k_fft_y<T, true>), dim3((unsigned)(3 * N * ncy)), dim3(256), shy, st, spec, pl, Wy, ncy);
-    kt_end(c, kSlotFftYInv);
+    if (timed) kt_begin(c, kSlotFftYInv);
+    hipLaunchKernelGGL((k_fft_y<T, true>), dim3((unsigned)(3 * K * N * ncy)), dim3(256), shy, st, spec, pl, Wy, ncy);
+    if (timed) kt_end(c, kSlotFftYInv);
     WT_HIP(hipGetLastError());
     // x inverse (C2R), scaled by 1/up, omega scaled too
     WT_HIP(hipFuncSetAttribute((const void *)k_fft_x_c2r<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shx));
-    kt_begin(c, kSlotFftXInv);
-    hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(gx), dim3(256), shx, st, (const C2<T> *)spec, (T *)c->u, (T *)c->omega, pl,
-                       ppb, npairs, (const DeviceScalars *)c->scal);
-    kt_end(c, kSlotFftXInv);
+    if (timed) kt_begin(c, kSlotFftXInv);
+    hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(gx), dim3(256), shx, st, (const C2<T> *)spec, (T *)c->u + I0.field_off,
+                       (T *)c->omega + I0.field_off, pl, ppb, npairs, (const DeviceScalars *)(c->scal + pos),
+                       pairs_per_inst);
+    if (timed) kt_end(c, kSlotFftXInv);
     return hipGetLastError();
 }
 
-hipError_t launch_biot_savart(wt_ctx *c) {
-    return c->precision == 64 ? biot_savart_t<double>(c, 0) : biot_savart_t<float>(c, 0);
+template <typename T>
+static hipError_t spectral_t(wt_ctx *c, int mode, int only) {
+    const int n = c->ninst();
+    if (only >= 0) {  // a single instance: find its arena position
+        for (int pos = 0; pos < n; pos++)
+            if (c->order[pos] == only) return spectral_group<T>(c, mode, pos, 1, false);
+        return hipErrorInvalidValue;
+    }
+    // groups of equal N are contiguous in arena order; time the largest group
+    int best = 0, bestw = -1;
+    for (int pos = 0; pos < n;) {
+        int end = pos;
+        while (end < n && c->inst[c->order[end]].N == c->inst[c->order[pos]].N) end++;
+        int64_t w = (int64_t)(end - pos) * c->inst[c->order[pos]].N * c->inst[c->order[pos]].N;
+        if (w > bestw) { bestw = (int)w; best = pos; }
+        pos = end;
+    }
+    for (int pos = 0; pos < n;) {
+        int end = pos;
+        while (end < n && c->inst[c->order[end]].N == c->inst[c->order[pos]].N) end++;
+        WT_HIP(spectral_group<T>(c, mode, pos, end - pos, pos == best));
+        pos = end;
+    }
+    return hipSuccess;
 }
 
-hipError_t launch_spectrum_of_u(wt_ctx *c) {
-    return c->precision == 64 ? biot_savart_t<double>(c, 1) : biot_savart_t<float>(c, 1);
+hipError_t launch_spectral(wt_ctx *c, int mode, int only) {
+    return c->precision == 64 ? spectral_t<double>(c, mode, only) : spectral_t<float>(c, mode, only);
 }
 
 }  // namespace wt

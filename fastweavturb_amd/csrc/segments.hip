@@ -10,8 +10,6 @@
 
 namespace wt {
 
-constexpr int kColsPerItem = 256;
-
 // Fill out[i] = l for every i in [base[l], base[l+1]); one block per tube.
 __global__ void k_fill_owner(const int64_t *__restrict__ base, int32_t n, int32_t *__restrict__ out) {
     int l = blockIdx.x;
@@ -163,8 +161,9 @@ __device__ __forceinline__ double turn_rate(double zz, const double *coe) {
 
 // tube.f90:625-642 exactly: ntf_pre+1 samples summed in order, ntf = ceiling(ntf_times*theta/pi2)
 // + 1, dtf = (cze_pl(t_i+1)-cze_pl(t_i))/ntf. Returns false if theta is not finite.
-__device__ __forceinline__ bool theta_exact(const Geo &g, const Run &r, int64_t s) {
+__device__ __forceinline__ bool theta_exact(const Geo &g, const Run &, int64_t s) {
     int l = g.sec_tube[s];
+    const Run &r = g.runs[g.tube_inst[l]];
     double LD = g.Len_Disc[l];
     double coe[18];
 #pragma unroll
@@ -222,10 +221,11 @@ __device__ __forceinline__ double gl8(const double *coe, double a, double b) {
 // accepted only when the two rules agree to 1e-9 and ntf_times*theta/2pi lies farther than
 // max(1e-6, 2/ntf_pre^2) (relative) from an integer. Every other section is re-evaluated with
 // the reference's exact sum (k_theta_listed), so ntf and dtf match the reference bit for bit.
-__global__ void k_theta_fast(Geo g, Run r, int32_t *__restrict__ redo, unsigned int *__restrict__ nredo) {
+__global__ void k_theta_fast(Geo g, Run, int32_t *__restrict__ redo, unsigned int *__restrict__ nredo) {
     int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= g.nsec_all) return;
     int l = g.sec_tube[s];
+    const Run &r = g.runs[g.tube_inst[l]];
     double LD = g.Len_Disc[l];
     double coe[18];
 #pragma unroll
@@ -307,10 +307,10 @@ __global__ void k_bin_count(Geo g, Run r, Bins b, int32_t *__restrict__ nitems,
         ncol = (int64_t)(sg.hi[bx] - sg.lo[bx] + 1) * (int64_t)(sg.hi[cxs] - sg.lo[cxs] + 1);
 #pragma unroll
         for (int c = 0; c < 3; c++)  // wrap_idx needs ip + N - 1 >= 0 for every visited index
-            if (sg.lo[c] < 1 - r.N || sg.hi[c] > 2 * r.N) atomicOr(err, 2);
+            if (sg.lo[c] < 1 - sg.N || sg.hi[c] > 2 * sg.N) atomicOr(err, 2);
     }
     b.col_count[q] = ncol;
-    nitems[q] = (int32_t)((ncol + kColsPerItem - 1) / kColsPerItem);
+    nitems[q] = (int32_t)((ncol + b.cols_per_item - 1) / b.cols_per_item);
 }
 
 __global__ void k_fill_items(Geo g, Bins b) {

@@ -134,14 +134,15 @@ __global__ void k_reduce_rows(const double *__restrict__ partials, int64_t rows,
 template <typename T>
 static hipError_t sf_t(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const int32_t *d_seps, double *d_out) {
     hipStream_t st = c->stream;
-    const int N = c->N;
+    const wt::Instance &I = c->inst[c->cur];
+    const int N = I.N;
     const int np = pmax - pmin + 1;
     int cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) cus = prop.multiProcessorCount;
     int dev_lds = 65536;
     (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
-    const T *u = (const T *)c->u;
+    const T *u = (const T *)c->u + I.field_off;
     size_t acc_bytes = sizeof(double) * ((size_t)nsep * np + 4 * kMaxOrder);
     int64_t blocks;
     double *partials = nullptr;

@@ -1,22 +1,22 @@
 // C ABI of the woven-turbulence generator (declarations and reference citations in
-// include/wovengpu.h). Owns device memory, the HIP stream and the call order checks.
+// include/wovengpu.h). Owns device memory, the HIP stream, the instances of a batch and the
+// call-order checks.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <numeric>
 #include <vector>
 
 #include "wt_internal.h"
 
 namespace wt {
-hipError_t launch_seg_stats(wt_ctx *c, wt_seg_stats *out);
-hipError_t launch_spectrum_of_u(wt_ctx *c);
-bool fft_size_supported(int N);
-void fft_reversal(int N, std::vector<int32_t> &rev);
 int64_t nek_of(int N) {
     float v = (float)(3 * N * N) / 12.0f;  // get_spectrum: int(sqrt((nx^2+ny^2+nz^2)/12.0))
     return (int64_t)std::sqrt(v);
@@ -37,7 +37,8 @@ static int set_err(wt_ctx *c, int code, const char *fmt, ...) {
 }
 
 static int hip_err(wt_ctx *c, hipError_t e, const char *where) {
-    return set_err(c, WT_EDEVICE, "%s: HIP error %d (%s)", where, (int)e, hipGetErrorString(e));
+    return set_err(c, e == hipErrorOutOfMemory ? WT_ENOMEM : WT_EDEVICE, "%s: HIP error %d (%s)", where, (int)e,
+                   hipGetErrorString(e));
 }
 
 #define CHECK_CTX(c)                                   \
@@ -60,17 +61,27 @@ static size_t mem_cap_bytes() {
     return (size_t)strtoull(s, nullptr, 10);
 }
 
-static size_t field_bytes(const wt_ctx *c) {
-    size_t N = (size_t)c->N, N3 = N * N * N;
-    size_t es = c->precision == 64 ? 8 : 4;
-    size_t b = 2 * 3 * N3 * es + 3 * N * N * (N / 2 + 1) * 2 * es;
-    if (c->precision == 32) b += 3 * N3 * 8;
+static size_t field_bytes(int64_t field_elems, int64_t spec_elems, int precision) {
+    size_t es = precision == 64 ? 8 : 4;
+    size_t b = 2 * (size_t)field_elems * es + (size_t)spec_elems * 2 * es;
+    if (precision == 32) b += (size_t)field_elems * 8;
     return b;
 }
 
 template <typename T>
 static hipError_t alloc(T **p, size_t n) {
     return hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
+}
+
+static void free_list(std::vector<void *> &l) {
+    for (void *p : l) (void)hipFree(p);
+    l.clear();
+}
+
+static int pos_of(const wt_ctx *c, int inst) {
+    for (int p = 0; p < c->ninst(); p++)
+        if (c->order[p] == inst) return p;
+    return -1;
 }
 
 extern "C" {
@@ -96,86 +107,6 @@ int32_t wt_sp_separations(int32_t n, int32_t *r, int32_t cap) {
     return nr;
 }
 
-int wt_create(wt_ctx **out, int32_t n, int32_t precision, int32_t device) {
-    if (!out) return set_err(nullptr, WT_EINVAL, "wt_create: ctx pointer is NULL");
-    *out = nullptr;
-    if (precision != 32 && precision != 64) return set_err(nullptr, WT_EINVAL, "precision must be 32 or 64");
-    if (!wt::fft_size_supported(n))
-        return set_err(nullptr, WT_ERANGE, "N=%d unsupported: need even 4 <= N <= 1024 with factors 2 and 3 only", n);
-    int ndev = 0;
-    hipError_t e = hipGetDeviceCount(&ndev);
-    if (e != hipSuccess || ndev <= 0) return set_err(nullptr, WT_EDEVICE, "no HIP device available");
-    if (device < 0 || device >= ndev) return set_err(nullptr, WT_EINVAL, "device %d out of range (%d)", device, ndev);
-    wt_ctx *c = new (std::nothrow) wt_ctx();
-    if (!c) return set_err(nullptr, WT_ENOMEM, "host allocation failed");
-    c->device = device;
-    c->precision = precision;
-    c->N = n;
-    size_t cap = mem_cap_bytes();
-    if (cap && field_bytes(c) > cap) {
-        delete c;
-        return set_err(nullptr, WT_ENOMEM, "fields need %zu bytes, above WT_DEVICE_MEM_CAP=%zu", field_bytes(c), cap);
-    }
-    auto fail = [&](hipError_t er, const char *w) {
-        set_err(nullptr, er == hipErrorOutOfMemory ? WT_ENOMEM : WT_EDEVICE, "%s: %s", w, hipGetErrorString(er));
-        wt_destroy(c);
-        return er == hipErrorOutOfMemory ? WT_ENOMEM : WT_EDEVICE;
-    };
-    if ((e = hipSetDevice(device)) != hipSuccess) return fail(e, "hipSetDevice");
-    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
-    for (auto &ev : c->ev)
-        if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e, "event");
-    for (int s = 0; s < wt::kNumSlots; s++)
-        for (int k = 0; k < 2; k++)
-            if ((e = hipEventCreate(&c->kev[s][k])) != hipSuccess) return fail(e, "event");
-    const size_t N = n, N3 = N * N * N, es = precision == 64 ? 8 : 4;
-    if ((e = hipMalloc(&c->omega, 3 * N3 * es)) != hipSuccess) return fail(e, "omega");
-    if ((e = hipMalloc(&c->u, 3 * N3 * es)) != hipSuccess) return fail(e, "u");
-    if ((e = hipMalloc(&c->spec, 3 * N * N * (N / 2 + 1) * 2 * es)) != hipSuccess) return fail(e, "spectrum buffer");
-    if (precision == 32 && (e = hipMalloc((void **)&c->acc, 3 * N3 * sizeof(long long))) != hipSuccess)
-        return fail(e, "accumulator");
-    // twiddles exp(-2 pi i k / N) in the compute precision, then the digit-reversal table
-    std::vector<int32_t> rev;
-    wt::fft_reversal(n, rev);
-    size_t twb = 2 * N * es + N * sizeof(int32_t);
-    std::vector<unsigned char> host(twb);
-    for (size_t k = 0; k < N; k++) {
-        long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)N;
-        double cr = (double)cosl(ang), ci = (double)sinl(ang);
-        if (precision == 64) {
-            ((double *)host.data())[2 * k] = cr;
-            ((double *)host.data())[2 * k + 1] = ci;
-        } else {
-            ((float *)host.data())[2 * k] = (float)cr;
-            ((float *)host.data())[2 * k + 1] = (float)ci;
-        }
-    }
-    memcpy(host.data() + 2 * N * es, rev.data(), N * sizeof(int32_t));
-    if ((e = hipMalloc(&c->twiddle, twb)) != hipSuccess) return fail(e, "twiddles");
-    if ((e = hipMemcpy(c->twiddle, host.data(), twb, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "twiddles");
-    if ((e = hipMalloc((void **)&c->scal, sizeof(wt::DeviceScalars))) != hipSuccess) return fail(e, "scalars");
-    if ((e = hipMemset(c->scal, 0, sizeof(wt::DeviceScalars))) != hipSuccess) return fail(e, "scalars");
-    c->nek = (int)wt::nek_of(n);
-    if ((e = hipMalloc((void **)&c->E_w, sizeof(double) * (c->nek + 1))) != hipSuccess) return fail(e, "spectra");
-    if ((e = hipMalloc((void **)&c->E_u, sizeof(double) * (c->nek + 1))) != hipSuccess) return fail(e, "spectra");
-    if ((e = hipMemset(c->omega, 0, 3 * N3 * es)) != hipSuccess) return fail(e, "omega");
-    if ((e = hipMemset(c->u, 0, 3 * N3 * es)) != hipSuccess) return fail(e, "u");
-    {
-        const char *te = getenv("WT_THETA_EXACT");
-        c->theta_exact_all = te && te[0] == '1';
-    }
-    c->run.N = n;
-    c->run.dx = wt::kPi2 / n;  // lx/nx (tube.f90:274)
-    c->run.start = -wt::kPi;
-    *out = c;
-    return WT_OK;
-}
-
-static void free_list(std::vector<void *> &l) {
-    for (void *p : l) (void)hipFree(p);
-    l.clear();
-}
-
 void wt_destroy(wt_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
@@ -183,8 +114,10 @@ void wt_destroy(wt_ctx *c) {
     free_list(c->bin_allocs);
     free_list(c->seg_allocs);
     free_list(c->geo_allocs);
-    void *bufs[] = {c->omega, c->u, c->spec, c->acc, c->partials, c->twiddle, c->scal, c->E_w, c->E_u};
+    void *bufs[] = {c->omega, c->u, c->spec, c->acc, c->partials, c->scal, c->E_w, c->E_u, c->runs_dev};
     for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    for (void *p : c->tw_tab)
         if (p) (void)hipFree(p);
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -195,24 +128,138 @@ void wt_destroy(wt_ctx *c) {
     delete c;
 }
 
+int wt_create_batch(wt_ctx **out, int32_t ninst, const int32_t *n, int32_t precision, int32_t device) {
+    if (!out) return set_err(nullptr, WT_EINVAL, "wt_create: ctx pointer is NULL");
+    *out = nullptr;
+    if (ninst <= 0 || !n) return set_err(nullptr, WT_EINVAL, "need at least one instance");
+    if (precision != 32 && precision != 64) return set_err(nullptr, WT_EINVAL, "precision must be 32 or 64");
+    for (int i = 0; i < ninst; i++)
+        if (!wt::fft_size_supported(n[i]))
+            return set_err(nullptr, WT_ERANGE,
+                           "N=%d unsupported: need even 4 <= N <= 1024 with factors 2 and 3 only", n[i]);
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) return set_err(nullptr, WT_EDEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) return set_err(nullptr, WT_EINVAL, "device %d out of range (%d)", device, ndev);
+    wt_ctx *c = new (std::nothrow) wt_ctx();
+    if (!c) return set_err(nullptr, WT_ENOMEM, "host allocation failed");
+    c->device = device;
+    c->precision = precision;
+    c->inst.resize(ninst);
+    c->order.resize(ninst);
+    std::iota(c->order.begin(), c->order.end(), 0);
+    std::stable_sort(c->order.begin(), c->order.end(), [&](int a, int b) { return n[a] < n[b]; });
+    // arena placement in order of N, so that equal-N instances are contiguous
+    int64_t fo = 0, so = 0, eo = 0;
+    for (int pos = 0; pos < ninst; pos++) {
+        wt::Instance &I = c->inst[c->order[pos]];
+        int64_t N = n[c->order[pos]];
+        I.N = (int)N;
+        I.field_off = fo;
+        I.spec_off = so;
+        I.nek = (int)wt::nek_of((int)N);
+        I.E_off = eo;
+        fo += 3 * N * N * N;
+        so += 3 * N * N * (N / 2 + 1);
+        eo += I.nek;
+    }
+    c->field_elems = fo;
+    c->spec_elems = so;
+    c->E_elems = eo;
+    c->N = c->inst[0].N;
+    size_t cap = mem_cap_bytes();
+    if (cap && field_bytes(fo, so, precision) > cap) {
+        size_t need = field_bytes(fo, so, precision);
+        delete c;
+        return set_err(nullptr, WT_ENOMEM, "fields need %zu bytes, above WT_DEVICE_MEM_CAP=%zu", need, cap);
+    }
+    auto fail = [&](hipError_t er, const char *w) {
+        int code = er == hipErrorOutOfMemory ? WT_ENOMEM : WT_EDEVICE;
+        set_err(nullptr, code, "%s: %s", w, hipGetErrorString(er));
+        wt_destroy(c);
+        return code;
+    };
+    if ((e = hipSetDevice(device)) != hipSuccess) return fail(e, "hipSetDevice");
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
+    for (auto &ev : c->ev)
+        if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e, "event");
+    for (int s = 0; s < wt::kNumSlots; s++)
+        for (int k = 0; k < 2; k++)
+            if ((e = hipEventCreate(&c->kev[s][k])) != hipSuccess) return fail(e, "event");
+    const size_t es = precision == 64 ? 8 : 4;
+    if ((e = hipMalloc(&c->omega, fo * es)) != hipSuccess) return fail(e, "omega");
+    if ((e = hipMalloc(&c->u, fo * es)) != hipSuccess) return fail(e, "u");
+    if ((e = hipMalloc(&c->spec, so * 2 * es)) != hipSuccess) return fail(e, "spectrum buffer");
+    if (precision == 32 && (e = hipMalloc((void **)&c->acc, fo * sizeof(long long))) != hipSuccess)
+        return fail(e, "accumulator");
+    // per grid size: twiddles exp(-2 pi i k / N) in the compute precision, then the
+    // digit-reversal table
+    for (int pos = 0; pos < ninst; pos++) {
+        int N = c->inst[c->order[pos]].N;
+        if (std::find(c->tw_N.begin(), c->tw_N.end(), N) != c->tw_N.end()) continue;
+        std::vector<int32_t> rev;
+        wt::fft_reversal(N, rev);
+        size_t twb = 2 * (size_t)N * es + (size_t)N * sizeof(int32_t);
+        std::vector<unsigned char> host(twb);
+        for (int k = 0; k < N; k++) {
+            long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)N;
+            double cr = (double)cosl(ang), ci = (double)sinl(ang);
+            if (precision == 64) {
+                ((double *)host.data())[2 * k] = cr;
+                ((double *)host.data())[2 * k + 1] = ci;
+            } else {
+                ((float *)host.data())[2 * k] = (float)cr;
+                ((float *)host.data())[2 * k + 1] = (float)ci;
+            }
+        }
+        memcpy(host.data() + 2 * (size_t)N * es, rev.data(), (size_t)N * sizeof(int32_t));
+        void *d = nullptr;
+        if ((e = hipMalloc(&d, twb)) != hipSuccess) return fail(e, "twiddles");
+        c->tw_N.push_back(N);
+        c->tw_tab.push_back(d);
+        if ((e = hipMemcpy(d, host.data(), twb, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "twiddles");
+    }
+    if ((e = hipMalloc((void **)&c->scal, sizeof(wt::DeviceScalars) * ninst)) != hipSuccess) return fail(e, "scalars");
+    if ((e = hipMemset(c->scal, 0, sizeof(wt::DeviceScalars) * ninst)) != hipSuccess) return fail(e, "scalars");
+    if ((e = hipMalloc((void **)&c->E_w, sizeof(double) * (eo + 1))) != hipSuccess) return fail(e, "spectra");
+    if ((e = hipMalloc((void **)&c->E_u, sizeof(double) * (eo + 1))) != hipSuccess) return fail(e, "spectra");
+    if ((e = hipMemset(c->omega, 0, fo * es)) != hipSuccess) return fail(e, "omega");
+    if ((e = hipMemset(c->u, 0, fo * es)) != hipSuccess) return fail(e, "u");
+    if ((e = hipMalloc((void **)&c->runs_dev, sizeof(wt::Run) * ninst)) != hipSuccess) return fail(e, "runs");
+    {
+        const char *te = getenv("WT_THETA_EXACT");
+        c->theta_exact_all = te && te[0] == '1';
+        const char *cp = getenv("WT_COLS_PER_ITEM");
+        if (cp && atoi(cp) > 0) c->cols_per_item = atoi(cp);
+    }
+    *out = c;
+    return WT_OK;
+}
+
+int wt_create(wt_ctx **out, int32_t n, int32_t precision, int32_t device) {
+    return wt_create_batch(out, 1, &n, precision, device);
+}
+
+int wt_select_instance(wt_ctx *c, int32_t inst) {
+    CHECK_CTX(c);
+    if (inst < 0 || inst >= c->ninst()) return set_err(c, WT_EINVAL, "instance %d out of range (%d)", inst, c->ninst());
+    c->cur = inst;
+    c->N = c->inst[inst].N;
+    return WT_OK;
+}
+
 int wt_set_params(wt_ctx *c, const wt_params *p) {
     CHECK_CTX(c);
     if (!p) return set_err(c, WT_EINVAL, "params is NULL");
-    if (p->N != c->N) return set_err(c, WT_EINVAL, "params N=%d differs from the context N=%d", p->N, c->N);
+    wt::Instance &I = c->inst[c->cur];
+    if (p->N != I.N) return set_err(c, WT_EINVAL, "params N=%d differs from the instance's N=%d", p->N, I.N);
     if (p->ntf_times < 1 || p->ntf_pre < 1) return set_err(c, WT_EINVAL, "ntf_times and ntf_pre must be >= 1");
     if (!(p->Rtube_ratio > 0) || !(p->dzeta_sup > 0) || !(p->sigma_ratio > 0) || !std::isfinite(p->eta_t))
         return set_err(c, WT_EINVAL, "Rtube_ratio, dzeta_sup and sigma_ratio must be positive, eta_t finite");
     if (p->line_s < 1 || p->line_e < p->line_s) return set_err(c, WT_EINVAL, "need 1 <= line_s <= line_e");
-    c->params = *p;
-    c->run.line_s = p->line_s - 1;
-    c->run.line_e = p->line_e - 1;
-    c->run.sigma_ratio = p->sigma_ratio;
-    c->run.eta = p->eta_t;
-    c->run.Rtube_ratio = p->Rtube_ratio;
-    c->run.dzeta_sup = p->dzeta_sup;
-    c->run.ntf_times = p->ntf_times;
-    c->run.ntf_pre = p->ntf_pre;
-    c->have_params = true;
+    I.params = *p;
+    I.have_params = true;
+    c->tubes_dirty = true;
     c->have_segments = c->have_bins = false;
     return WT_OK;
 }
@@ -222,53 +269,129 @@ int wt_set_tubes(wt_ctx *c, int32_t nline, const int32_t *npoint, const double *
     CHECK_CTX(c);
     if (nline <= 0 || !npoint || !sigma || !gamma || !n_T || !cx || !cy || !cz)
         return set_err(c, WT_EINVAL, "wt_set_tubes: need nline > 0 and non-NULL arrays");
-    std::vector<int64_t> pt(nline + 1), sec(nline + 1);
-    pt[0] = sec[0] = 0;
-    double bound = 0.0;
+    int64_t P = 0;
     for (int l = 0; l < nline; l++) {
         if (npoint[l] < 3) return set_err(c, WT_EINVAL, "tube %d has %d points; need at least 3", l + 1, npoint[l]);
         if (!(sigma[l] > 0) || !std::isfinite(sigma[l]) || !std::isfinite(gamma[l]))
             return set_err(c, WT_EINVAL, "tube %d: sigma must be positive and finite, Gamma finite", l + 1);
         if (if_filter && if_filter[l] != 0)
             return set_err(c, WT_EINVAL, "tube %d: if_filter=1 (filter_spec) is not supported", l + 1);
-        pt[l + 1] = pt[l] + npoint[l];
-        sec[l + 1] = sec[l] + npoint[l] + 1;
-        bound += std::fabs(gamma[l]) / (wt::kPi2 * sigma[l] * sigma[l]);
+        P += npoint[l];
     }
-    const int64_t P = pt[nline], S = sec[nline];
     for (int64_t i = 0; i < P; i++)
         if (!std::isfinite(cx[i]) || !std::isfinite(cy[i]) || !std::isfinite(cz[i]))
             return set_err(c, WT_EINVAL, "non-finite centerline coordinate at point %lld", (long long)i + 1);
+    wt::Instance &I = c->inst[c->cur];
+    I.npoint.assign(npoint, npoint + nline);
+    I.n_T.assign(n_T, n_T + nline);
+    I.sigma.assign(sigma, sigma + nline);
+    I.gamma.assign(gamma, gamma + nline);
+    I.cx.assign(cx, cx + P);
+    I.cy.assign(cy, cy + P);
+    I.cz.assign(cz, cz + P);
+    I.have_tubes = true;
+    c->tubes_dirty = true;
+    c->have_segments = c->have_bins = false;
+    return WT_OK;
+}
+
+// Concatenates every instance's tubes into the device tube table.
+static int upload_tubes(wt_ctx *c) {
+    const int ni = c->ninst();
+    for (int i = 0; i < ni; i++)
+        if (!c->inst[i].have_params || !c->inst[i].have_tubes)
+            return set_err(c, WT_ESTATE, "instance %d needs wt_set_params and wt_set_tubes", i);
+    int64_t nline = 0, P = 0;
+    for (auto &I : c->inst) {
+        I.tube0 = (int32_t)nline;
+        I.ntube = (int32_t)I.npoint.size();
+        nline += I.ntube;
+        P += (int64_t)I.cx.size();
+        if (I.params.line_e > I.ntube)
+            return set_err(c, WT_EINVAL, "line_e=%d exceeds the %d tubes of its instance", I.params.line_e, I.ntube);
+    }
+    if (nline > INT32_MAX / 2) return set_err(c, WT_ERANGE, "too many tubes");
+    std::vector<int64_t> pt(nline + 1), sec(nline + 1);
+    std::vector<int32_t> npt, nT, tinst;
+    std::vector<double> sig, gam, xs, ys, zs;
+    npt.reserve(nline);
+    xs.reserve(P);
+    ys.reserve(P);
+    zs.reserve(P);
+    double bound = 0.0;
+    for (int i = 0; i < ni; i++) {
+        const wt::Instance &I = c->inst[i];
+        for (int l = 0; l < I.ntube; l++) {
+            npt.push_back(I.npoint[l]);
+            nT.push_back(I.n_T[l]);
+            sig.push_back(I.sigma[l]);
+            gam.push_back(I.gamma[l]);
+            tinst.push_back(i);
+            bound += std::fabs(I.gamma[l]) / (wt::kPi2 * I.sigma[l] * I.sigma[l]);
+        }
+        xs.insert(xs.end(), I.cx.begin(), I.cx.end());
+        ys.insert(ys.end(), I.cy.begin(), I.cy.end());
+        zs.insert(zs.end(), I.cz.begin(), I.cz.end());
+    }
+    pt[0] = sec[0] = 0;
+    for (int64_t l = 0; l < nline; l++) {
+        pt[l + 1] = pt[l] + npt[l];
+        sec[l + 1] = sec[l] + npt[l] + 1;
+    }
+    const int64_t S = sec[nline];
     size_t cap = mem_cap_bytes();
     size_t need = (size_t)P * 8 * 13 + (size_t)S * 8 * 24;
-    if (cap && field_bytes(c) + need > cap)
+    if (cap && field_bytes(c->field_elems, c->spec_elems, c->precision) + need > cap)
         return set_err(c, WT_ENOMEM, "tube data needs %zu bytes beyond the fields, above WT_DEVICE_MEM_CAP", need);
+    // per-instance run table
+    c->runs_host.resize(ni);
+    for (int i = 0; i < ni; i++) {
+        const wt::Instance &I = c->inst[i];
+        const wt_params &p = I.params;
+        wt::Run &r = c->runs_host[i];
+        r.N = I.N;
+        r.line_s = p.line_s - 1;
+        r.line_e = p.line_e - 1;
+        r.tube0 = I.tube0;
+        r.sigma_ratio = p.sigma_ratio;
+        r.eta = p.eta_t;
+        r.Rtube_ratio = p.Rtube_ratio;
+        r.dzeta_sup = p.dzeta_sup;
+        r.ntf_times = p.ntf_times;
+        r.ntf_pre = p.ntf_pre;
+        r.dx = wt::kPi2 / I.N;  // lx/nx (tube.f90:274)
+        r.start = -wt::kPi;
+        r.field_off = I.field_off;
+    }
+    c->run = c->runs_host[0];
     free_list(c->bin_allocs);
     free_list(c->seg_allocs);
     free_list(c->geo_allocs);
-    c->have_tubes = c->have_segments = c->have_bins = false;
+    c->have_segments = c->have_bins = false;
     wt::Geo g{};
-    g.nline = nline;
+    g.nline = (int32_t)nline;
     g.npoint_all = P;
     g.nsec_all = S;
+    g.runs = c->runs_dev;
     auto A = [&](auto **p, size_t n) -> hipError_t {
         hipError_t e = alloc(p, n);
         if (e == hipSuccess) c->geo_allocs.push_back((void *)*p);
         return e;
     };
     int64_t *pt_base, *sec_base, *sp_base;
-    double *sig, *gam, *xs, *ys, *zs;
-    int32_t *nT;
+    double *sigd, *gamd, *xd, *yd, *zd;
+    int32_t *nTd, *tid;
     HIPCALL(c, A(&pt_base, nline + 1), "alloc");
     HIPCALL(c, A(&sec_base, nline + 1), "alloc");
-    HIPCALL(c, A(&sig, nline), "alloc");
-    HIPCALL(c, A(&gam, nline), "alloc");
-    HIPCALL(c, A(&nT, nline), "alloc");
+    HIPCALL(c, A(&sigd, nline), "alloc");
+    HIPCALL(c, A(&gamd, nline), "alloc");
+    HIPCALL(c, A(&nTd, nline), "alloc");
+    HIPCALL(c, A(&tid, nline), "alloc");
     HIPCALL(c, A(&g.Len_Disc, nline), "alloc");
     HIPCALL(c, A(&g.length, nline), "alloc");
-    HIPCALL(c, A(&xs, P), "alloc");
-    HIPCALL(c, A(&ys, P), "alloc");
-    HIPCALL(c, A(&zs, P), "alloc");
+    HIPCALL(c, A(&xd, P), "alloc");
+    HIPCALL(c, A(&yd, P), "alloc");
+    HIPCALL(c, A(&zd, P), "alloc");
     HIPCALL(c, A(&g.pt_tube, P), "alloc");
     HIPCALL(c, A(&g.chord, P), "alloc");
     HIPCALL(c, A(&g.cze, P), "alloc");
@@ -283,25 +406,27 @@ int wt_set_tubes(wt_ctx *c, int32_t nline, const int32_t *npoint, const double *
     HIPCALL(c, A(&sp_base, S + 1), "alloc");
     g.sp_base = sp_base;
     hipStream_t st = c->stream;
+    HIPCALL(c, hipMemcpyAsync(c->runs_dev, c->runs_host.data(), sizeof(wt::Run) * ni, hipMemcpyHostToDevice, st), "copy");
     HIPCALL(c, hipMemcpyAsync(pt_base, pt.data(), sizeof(int64_t) * (nline + 1), hipMemcpyHostToDevice, st), "copy");
     HIPCALL(c, hipMemcpyAsync(sec_base, sec.data(), sizeof(int64_t) * (nline + 1), hipMemcpyHostToDevice, st), "copy");
-    HIPCALL(c, hipMemcpyAsync(sig, sigma, sizeof(double) * nline, hipMemcpyHostToDevice, st), "copy");
-    HIPCALL(c, hipMemcpyAsync(gam, gamma, sizeof(double) * nline, hipMemcpyHostToDevice, st), "copy");
-    HIPCALL(c, hipMemcpyAsync(nT, n_T, sizeof(int32_t) * nline, hipMemcpyHostToDevice, st), "copy");
-    HIPCALL(c, hipMemcpyAsync(xs, cx, sizeof(double) * P, hipMemcpyHostToDevice, st), "copy");
-    HIPCALL(c, hipMemcpyAsync(ys, cy, sizeof(double) * P, hipMemcpyHostToDevice, st), "copy");
-    HIPCALL(c, hipMemcpyAsync(zs, cz, sizeof(double) * P, hipMemcpyHostToDevice, st), "copy");
+    HIPCALL(c, hipMemcpyAsync(sigd, sig.data(), sizeof(double) * nline, hipMemcpyHostToDevice, st), "copy");
+    HIPCALL(c, hipMemcpyAsync(gamd, gam.data(), sizeof(double) * nline, hipMemcpyHostToDevice, st), "copy");
+    HIPCALL(c, hipMemcpyAsync(nTd, nT.data(), sizeof(int32_t) * nline, hipMemcpyHostToDevice, st), "copy");
+    HIPCALL(c, hipMemcpyAsync(tid, tinst.data(), sizeof(int32_t) * nline, hipMemcpyHostToDevice, st), "copy");
+    HIPCALL(c, hipMemcpyAsync(xd, xs.data(), sizeof(double) * P, hipMemcpyHostToDevice, st), "copy");
+    HIPCALL(c, hipMemcpyAsync(yd, ys.data(), sizeof(double) * P, hipMemcpyHostToDevice, st), "copy");
+    HIPCALL(c, hipMemcpyAsync(zd, zs.data(), sizeof(double) * P, hipMemcpyHostToDevice, st), "copy");
     HIPCALL(c, hipStreamSynchronize(st), "copy");
     g.pt_base = pt_base;
     g.sec_base = sec_base;
-    g.sigma = sig;
-    g.gamma = gam;
-    g.n_T = nT;
-    g.cx = xs;
-    g.cy = ys;
-    g.cz = zs;
+    g.sigma = sigd;
+    g.gamma = gamd;
+    g.n_T = nTd;
+    g.tube_inst = tid;
+    g.cx = xd;
+    g.cy = yd;
+    g.cz = zd;
     c->geo = g;
-    c->npoint.assign(npoint, npoint + nline);
     c->pt_base = pt;
     c->sec_base = sec;
     // fixed-point scale for the FP32 accumulation: |omega| <= sum Gamma/(2 pi sigma^2) times a
@@ -309,13 +434,16 @@ int wt_set_tubes(wt_ctx *c, int32_t nline, const int32_t *npoint, const double *
     double B = bound * 64.0 + 1.0;
     int shift = (int)std::floor(60.0 - std::log2(B));
     c->fixed_shift = shift < 8 ? 8 : (shift > 44 ? 44 : shift);
-    c->have_tubes = true;
+    c->tubes_dirty = false;
     return WT_OK;
 }
 
 int wt_build_segments(wt_ctx *c, wt_seg_stats *out) {
     CHECK_CTX(c);
-    if (!c->have_params || !c->have_tubes) return set_err(c, WT_ESTATE, "wt_build_segments needs params and tubes");
+    if (c->tubes_dirty) {
+        int rc = upload_tubes(c);
+        if (rc) return rc;
+    }
     c->have_segments = c->have_bins = false;
     hipError_t e = wt::launch_segments(c);
     if (e == hipErrorInvalidValue)
@@ -331,11 +459,18 @@ int wt_build_segments(wt_ctx *c, wt_seg_stats *out) {
     return WT_OK;
 }
 
+int wt_get_seg_stats(wt_ctx *c, wt_seg_stats *out) {
+    CHECK_CTX(c);
+    if (!c->have_segments) return set_err(c, WT_ESTATE, "no segments built");
+    if (!out) return set_err(c, WT_EINVAL, "NULL output");
+    *out = c->seg_stats;
+    return WT_OK;
+}
+
 int wt_bin(wt_ctx *c) {
     CHECK_CTX(c);
     if (!c->have_segments) return set_err(c, WT_ESTATE, "wt_bin needs wt_build_segments");
-    if (c->run.line_e >= c->geo.nline)
-        return set_err(c, WT_EINVAL, "line_e=%d exceeds the %d tubes", c->run.line_e + 1, c->geo.nline);
+    c->bins.cols_per_item = c->cols_per_item;
     hipError_t e = wt::launch_bins(c);
     if (e == hipErrorInvalidValue)
         return set_err(c, WT_ERANGE, "a tube's cut-off radius spans more than the periodic box");
@@ -358,15 +493,34 @@ int wt_deposit(wt_ctx *c, wt_deposit_stats *out) {
     return WT_OK;
 }
 
+int wt_get_deposit_stats(wt_ctx *c, wt_deposit_stats *out) {
+    CHECK_CTX(c);
+    if (!c->have_omega || !out) return set_err(c, WT_ESTATE, "no deposition has run");
+    *out = c->dep_stats;
+    return WT_OK;
+}
+
+static int read_scalars(wt_ctx *c, int inst, wt::DeviceScalars *s) {
+    int pos = pos_of(c, inst);
+    HIPCALL(c, hipMemcpyAsync(s, c->scal + pos, sizeof *s, hipMemcpyDeviceToHost, c->stream), "scalars");
+    HIPCALL(c, hipStreamSynchronize(c->stream), "scalars");
+    return WT_OK;
+}
+
 int wt_biot_savart(wt_ctx *c, double *etot, double *up, double *enstrophy) {
     CHECK_CTX(c);
     if (!c->have_omega) return set_err(c, WT_ESTATE, "wt_biot_savart needs a deposited or set vorticity");
-    hipError_t e = wt::launch_biot_savart(c);
+    hipError_t e = wt::launch_spectral(c, 0, -1);
     if (e != hipSuccess) return hip_err(c, e, "biot_savart");
-    wt::DeviceScalars s;
-    HIPCALL(c, hipMemcpyAsync(&s, c->scal, sizeof s, hipMemcpyDeviceToHost, c->stream), "scalars");
+    std::vector<wt::DeviceScalars> all(c->ninst());
+    HIPCALL(c, hipMemcpyAsync(all.data(), c->scal, sizeof(wt::DeviceScalars) * c->ninst(), hipMemcpyDeviceToHost,
+                              c->stream), "scalars");
     HIPCALL(c, hipStreamSynchronize(c->stream), "biot_savart");
-    if (!(s.up > 0) || !std::isfinite(s.up)) return set_err(c, WT_ERANGE, "velocity is zero: cannot normalise (up=%g)", s.up);
+    for (int p = 0; p < c->ninst(); p++)
+        if (!(all[p].up > 0) || !std::isfinite(all[p].up))
+            return set_err(c, WT_ERANGE, "instance %d: velocity is zero, cannot normalise (up=%g)", c->order[p],
+                           all[p].up);
+    const wt::DeviceScalars &s = all[pos_of(c, c->cur)];
     if (etot) *etot = s.etot_raw;
     if (up) *up = s.up;
     if (enstrophy) *enstrophy = s.enstrophy;
@@ -375,8 +529,24 @@ int wt_biot_savart(wt_ctx *c, double *etot, double *up, double *enstrophy) {
     return WT_OK;
 }
 
+int wt_instance_scalars(wt_ctx *c, double *etot, double *up, double *enstrophy) {
+    CHECK_CTX(c);
+    if (!c->have_u) return set_err(c, WT_ESTATE, "no velocity resident");
+    wt::DeviceScalars s;
+    int rc = read_scalars(c, c->cur, &s);
+    if (rc) return rc;
+    if (etot) *etot = s.etot_raw;
+    if (up) *up = s.up;
+    if (enstrophy) *enstrophy = s.enstrophy;
+    return WT_OK;
+}
+
 int wt_generate(wt_ctx *c, double *etot, double *up, double *enstrophy) {
     CHECK_CTX(c);
+    if (c->tubes_dirty) {  // the upload is not part of the timed stages
+        int rc = upload_tubes(c);
+        if (rc) return rc;
+    }
     HIPCALL(c, hipEventRecord(c->ev[0], c->stream), "event");
     int rc = wt_build_segments(c, nullptr);
     if (rc) return rc;
@@ -413,23 +583,26 @@ int wt_stage_times(wt_ctx *c, double *ms, int32_t cap) {
 
 int wt_spectrum(wt_ctx *c, int32_t field, double *E, int32_t cap) {
     CHECK_CTX(c);
+    const wt::Instance &I = c->inst[c->cur];
     if (!E) return set_err(c, WT_EINVAL, "E is NULL");
-    if (cap < c->nek) return set_err(c, WT_ERANGE, "need room for %d shells", c->nek);
+    if (cap < I.nek) return set_err(c, WT_ERANGE, "need room for %d shells", I.nek);
     if (field == 0) {
         if (!c->have_spectra) return set_err(c, WT_ESTATE, "the vorticity spectrum comes from wt_biot_savart");
-        HIPCALL(c, hipMemcpyAsync(E, c->E_w, sizeof(double) * c->nek, hipMemcpyDeviceToHost, c->stream), "spectrum");
+        HIPCALL(c, hipMemcpyAsync(E, c->E_w + I.E_off, sizeof(double) * I.nek, hipMemcpyDeviceToHost, c->stream),
+                "spectrum");
     } else if (field == 1) {
         if (!c->have_u) return set_err(c, WT_ESTATE, "no velocity resident");
         if (!c->have_spectra) {
-            hipError_t e = wt::launch_spectrum_of_u(c);
+            hipError_t e = wt::launch_spectral(c, 1, c->cur);
             if (e != hipSuccess) return hip_err(c, e, "spectrum");
         }
-        HIPCALL(c, hipMemcpyAsync(E, c->E_u, sizeof(double) * c->nek, hipMemcpyDeviceToHost, c->stream), "spectrum");
+        HIPCALL(c, hipMemcpyAsync(E, c->E_u + I.E_off, sizeof(double) * I.nek, hipMemcpyDeviceToHost, c->stream),
+                "spectrum");
     } else {
         return set_err(c, WT_EINVAL, "field must be 0 (vorticity) or 1 (velocity)");
     }
     HIPCALL(c, hipStreamSynchronize(c->stream), "spectrum");
-    return c->nek;
+    return I.nek;
 }
 
 int wt_structure_functions(wt_ctx *c, int32_t mode, int32_t pmin, int32_t pmax, int32_t nsep, const int32_t *seps,
@@ -437,8 +610,10 @@ int wt_structure_functions(wt_ctx *c, int32_t mode, int32_t pmin, int32_t pmax, 
     CHECK_CTX(c);
     if (!c->have_u) return set_err(c, WT_ESTATE, "no velocity resident");
     if (mode != 0 && mode != 1) return set_err(c, WT_EINVAL, "mode must be 0 or 1");
-    if (mode == 0 && pmin != 1) return set_err(c, WT_EINVAL, "mode 0 (Sp_calculate) computes orders 1..pmax; pmin must be 1");
-    if (pmin < 1 || pmax < pmin || pmax - pmin + 1 > 12) return set_err(c, WT_EINVAL, "need 1 <= pmin <= pmax, at most 12 orders");
+    if (mode == 0 && pmin != 1)
+        return set_err(c, WT_EINVAL, "mode 0 (Sp_calculate) computes orders 1..pmax; pmin must be 1");
+    if (pmin < 1 || pmax < pmin || pmax - pmin + 1 > 12)
+        return set_err(c, WT_EINVAL, "need 1 <= pmin <= pmax, at most 12 orders");
     if (nsep <= 0 || !seps || !S) return set_err(c, WT_EINVAL, "need nsep > 0 and non-NULL arrays");
     int32_t *d_seps = nullptr;
     double *d_out = nullptr;
@@ -457,18 +632,21 @@ int wt_structure_functions(wt_ctx *c, int32_t mode, int32_t pmin, int32_t pmax, 
 }
 
 static void *field_ptr(wt_ctx *c, int which) {
-    size_t N3 = (size_t)c->N * c->N * c->N, es = c->precision == 64 ? 8 : 4;
+    const wt::Instance &I = c->inst[c->cur];
+    size_t N3 = (size_t)I.N * I.N * I.N, es = c->precision == 64 ? 8 : 4;
     char *base = (char *)(which < 3 ? c->omega : c->u);
-    return base + (size_t)(which % 3) * N3 * es;
+    return base + ((size_t)I.field_off + (size_t)(which % 3) * N3) * es;
 }
 
 int wt_get_field(wt_ctx *c, int32_t which, void *host, int32_t host_is_fp64) {
     CHECK_CTX(c);
     if (which < 0 || which > 5 || !host) return set_err(c, WT_EINVAL, "bad field selector or NULL buffer");
-    size_t N3 = (size_t)c->N * c->N * c->N;
+    const wt::Instance &I = c->inst[c->cur];
+    size_t N3 = (size_t)I.N * I.N * I.N;
     bool dev64 = c->precision == 64;
     if ((bool)host_is_fp64 == dev64) {
-        HIPCALL(c, hipMemcpyAsync(host, field_ptr(c, which), N3 * (dev64 ? 8 : 4), hipMemcpyDeviceToHost, c->stream), "copy");
+        HIPCALL(c, hipMemcpyAsync(host, field_ptr(c, which), N3 * (dev64 ? 8 : 4), hipMemcpyDeviceToHost, c->stream),
+                "copy");
         HIPCALL(c, hipStreamSynchronize(c->stream), "copy");
         return WT_OK;
     }
@@ -483,10 +661,12 @@ int wt_get_field(wt_ctx *c, int32_t which, void *host, int32_t host_is_fp64) {
 int wt_set_field(wt_ctx *c, int32_t which, const void *host, int32_t host_is_fp64) {
     CHECK_CTX(c);
     if (which < 0 || which > 5 || !host) return set_err(c, WT_EINVAL, "bad field selector or NULL buffer");
-    size_t N3 = (size_t)c->N * c->N * c->N;
+    const wt::Instance &I = c->inst[c->cur];
+    size_t N3 = (size_t)I.N * I.N * I.N;
     bool dev64 = c->precision == 64;
     if ((bool)host_is_fp64 == dev64) {
-        HIPCALL(c, hipMemcpyAsync(field_ptr(c, which), host, N3 * (dev64 ? 8 : 4), hipMemcpyHostToDevice, c->stream), "copy");
+        HIPCALL(c, hipMemcpyAsync(field_ptr(c, which), host, N3 * (dev64 ? 8 : 4), hipMemcpyHostToDevice, c->stream),
+                "copy");
     } else {
         std::vector<unsigned char> tmp(N3 * (dev64 ? 8 : 4));
         if (dev64) for (size_t i = 0; i < N3; i++) ((double *)tmp.data())[i] = (double)((const float *)host)[i];
@@ -503,8 +683,12 @@ int wt_set_field(wt_ctx *c, int32_t which, const void *host, int32_t host_is_fp6
 int wt_get_sections(wt_ctx *c, int32_t *ntf, double *dtf) {
     CHECK_CTX(c);
     if (!c->have_segments) return set_err(c, WT_ESTATE, "no segments built");
-    if (ntf) HIPCALL(c, hipMemcpyAsync(ntf, c->geo.ntf, sizeof(int32_t) * c->geo.nsec_all, hipMemcpyDeviceToHost, c->stream), "copy");
-    if (dtf) HIPCALL(c, hipMemcpyAsync(dtf, c->geo.dtf, sizeof(double) * c->geo.nsec_all, hipMemcpyDeviceToHost, c->stream), "copy");
+    if (ntf)
+        HIPCALL(c, hipMemcpyAsync(ntf, c->geo.ntf, sizeof(int32_t) * c->geo.nsec_all, hipMemcpyDeviceToHost, c->stream),
+                "copy");
+    if (dtf)
+        HIPCALL(c, hipMemcpyAsync(dtf, c->geo.dtf, sizeof(double) * c->geo.nsec_all, hipMemcpyDeviceToHost, c->stream),
+                "copy");
     HIPCALL(c, hipStreamSynchronize(c->stream), "copy");
     return WT_OK;
 }
@@ -520,7 +704,8 @@ int wt_get_subpoints(wt_ctx *c, double *ci_xyz, double *s) {
         std::vector<int64_t> spb(S + 1);
         HIPCALL(c, hipMemcpyAsync(czl.data(), c->geo.czl, sizeof(double) * S, hipMemcpyDeviceToHost, c->stream), "copy");
         HIPCALL(c, hipMemcpyAsync(dtf.data(), c->geo.dtf, sizeof(double) * S, hipMemcpyDeviceToHost, c->stream), "copy");
-        HIPCALL(c, hipMemcpyAsync(spb.data(), c->geo.sp_base, sizeof(int64_t) * (S + 1), hipMemcpyDeviceToHost, c->stream), "copy");
+        HIPCALL(c, hipMemcpyAsync(spb.data(), c->geo.sp_base, sizeof(int64_t) * (S + 1), hipMemcpyDeviceToHost,
+                                  c->stream), "copy");
         HIPCALL(c, hipStreamSynchronize(c->stream), "copy");
         for (int64_t sec = 0; sec < S; sec++)  // s_list2 = cze_pl(t_i) + dtf(t_i)*(t_j-1) (tube.f90:708)
             for (int64_t q = spb[sec]; q < spb[sec + 1]; q++) s[q] = czl[sec] + dtf[sec] * (double)(q - spb[sec]);

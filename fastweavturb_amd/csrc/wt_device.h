@@ -82,6 +82,10 @@ struct SubSeg {
     double tci0[3], tci1[3];
     double ndci0, s0, s1, R;
     int32_t lo[3], hi[3];
+    // grid of the tube's instance
+    int32_t N;
+    double dx, start;
+    int64_t field_off;
 };
 
 // Chord unit vector and length between sub-points a and b (tube.f90:740-750).
@@ -96,13 +100,18 @@ __device__ __forceinline__ void chord(const double *ci, int64_t b, int64_t a, do
 
 // Returns false when q starts no deposited sub-segment (ghost section, last sub-point of a
 // section, or a tube outside line_s..line_e).
-__device__ __forceinline__ bool subseg_load(const Geo &g, const Run &r, int64_t q, SubSeg &o) {
+__device__ __forceinline__ bool subseg_load(const Geo &g, const Run &, int64_t q, SubSeg &o) {
     int32_t s = g.sub_sec[q];
     int32_t l = g.sec_tube[s];
     int32_t j = (int32_t)(q - g.sp_base[s]);
     if ((int64_t)s == g.sec_base[l + 1] - 1) return false;
     if (j >= g.ntf[s]) return false;
-    if (l < r.line_s || l > r.line_e) return false;
+    const Run &r = g.runs[g.tube_inst[l]];
+    if (l - r.tube0 < r.line_s || l - r.tube0 > r.line_e) return false;
+    o.N = r.N;
+    o.dx = r.dx;
+    o.start = r.start;
+    o.field_off = r.field_off;
     o.sec = s;
     o.tube = l;
     o.j = j;
@@ -145,19 +154,18 @@ __device__ __forceinline__ bool subseg_load(const Geo &g, const Run &r, int64_t 
 }
 
 // Mesh coordinate of an extended 1-based index (tube.f90:331-339).
-__device__ __forceinline__ double mesh(int32_t ip, const Run &r) {
-    return ((double)ip - 1.0) * r.dx + r.start;
+__device__ __forceinline__ double mesh(int32_t ip, const SubSeg &sg) {
+    return ((double)ip - 1.0) * sg.dx + sg.start;
 }
 
 // Fortran MOD wrap of an extended index (tube.f90:803, 870-871); <= 0 means "outside".
 __device__ __forceinline__ int32_t wrap_idx(int32_t ip, int32_t N) { return (ip + N - 1) % N + 1; }
 
 // The reference's candidate test for one extended cell (tube.f90:803-818).
-__device__ __forceinline__ bool pair_hits(const SubSeg &sg, const Run &r, int32_t ip, int32_t jp,
-                                          int32_t kp) {
-    int32_t k = wrap_idx(kp, r.N);
-    if (!(k > 0 && k <= r.N)) return false;
-    double X0 = mesh(ip, r), X1 = mesh(jp, r), X2 = mesh(kp, r);
+__device__ __forceinline__ bool pair_hits(const SubSeg &sg, int32_t ip, int32_t jp, int32_t kp) {
+    int32_t k = wrap_idx(kp, sg.N);
+    if (!(k > 0 && k <= sg.N)) return false;
+    double X0 = mesh(ip, sg), X1 = mesh(jp, sg), X2 = mesh(kp, sg);
     double a0 = X0 - sg.ci0[0], a1 = X1 - sg.ci0[1], a2 = X2 - sg.ci0[2];
     double t0 = (a0 * sg.tci0[0] + a1 * sg.tci0[1]) + a2 * sg.tci0[2];
     if (!(t0 >= 0.0)) return false;

@@ -1,4 +1,9 @@
 // Internal declarations shared by the wovengpu host code and the HIP kernel files.
+//
+// A context holds one or more instances (independent woven fields, possibly of different grid
+// sizes). Their tubes are concatenated into one tube table, so geometry, binning and
+// deposition run as single launches over all instances; the FFT runs once per group of
+// instances with equal N. A plain wt_create context is a batch of one.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,10 +19,24 @@ namespace wt {
 constexpr double kPi = 3.141592653589793115997963468544185161590576171875;   // 4*atan(1d0)
 constexpr double kPi2 = 6.28318530717958623199592693708837032318115234375;   // 8*atan(1d0)
 
+// Per-instance run description (device copy in Geo::runs, indexed by Geo::tube_inst).
+struct Run {
+    int32_t N;
+    int32_t line_s, line_e;     // 0-based inclusive tube range (relative to tube0) that deposits
+    int32_t tube0;              // first global tube of the instance
+    double sigma_ratio, eta, Rtube_ratio, dzeta_sup;
+    int32_t ntf_times, ntf_pre;
+    double dx;                  // 2*pi/N
+    double start;               // -pi
+    int64_t field_off;          // element offset of the instance's 3*N^3 field block
+};
+
 // Device-side view of the tube table and geometry (all pointers are device memory).
 struct Geo {
     int32_t nline;
     int64_t npoint_all, nsec_all, nsub_all;
+    const Run *runs;            // [ninst]
+    const int32_t *tube_inst;   // [nline] instance of each tube
     // per tube
     const int64_t *pt_base;     // [nline+1]
     const int64_t *sec_base;    // [nline+1]
@@ -45,28 +64,33 @@ struct Geo {
     double *ci;                 // [3*NSP] xyz interleaved
 };
 
-// Run parameters copied to kernels by value.
-struct Run {
-    int32_t N;
-    int32_t line_s, line_e;     // 0-based inclusive tube range that deposits
-    double sigma_ratio, eta, Rtube_ratio, dzeta_sup;
-    int32_t ntf_times, ntf_pre;
-    double dx;                  // 2*pi/N
-    double start;               // -pi
-};
-
-// Work layout of the deposition: sub-point q starts a sub-segment iff ncol[q] > 0.
+// Work layout of the deposition: sub-point q starts a sub-segment iff col_count[q] > 0.
 struct Bins {
     int64_t *col_count;         // [NSP] columns of the sub-segment's box (0 = not a sub-segment)
     int64_t *item_off;          // [NSP+1] prefix sum of work items (column blocks)
     uint32_t *item_q;           // [items] sub-point of each work item
     int64_t items;
     int64_t columns;
+    int32_t cols_per_item;      // columns per work item
 };
 
-struct DeviceScalars {          // small device-resident results
+struct DeviceScalars {          // small device-resident results, one per instance
     double etot_raw, ens_raw, up, inv_up, enstrophy, etot_norm;
     unsigned long long overflow, hits, candidates;
+};
+
+// Host copy of one instance's inputs and its places in the arenas.
+struct Instance {
+    int N = 0;
+    wt_params params{};
+    bool have_params = false, have_tubes = false;
+    std::vector<int32_t> npoint, n_T;
+    std::vector<double> sigma, gamma, cx, cy, cz;
+    int64_t field_off = 0;      // elements, in the omega/u arenas (3*N^3 per instance)
+    int64_t spec_off = 0;       // complex elements, in the spectrum arena
+    int nek = 0;
+    int64_t E_off = 0;          // doubles, in the spectra arenas
+    int32_t tube0 = 0, ntube = 0;
 };
 
 }  // namespace wt
@@ -74,15 +98,19 @@ struct DeviceScalars {          // small device-resident results
 struct wt_ctx {
     int device = 0;
     int precision = 32;
-    int N = 0;
+    int N = 0;                            // grid of the selected instance
     hipStream_t stream = nullptr;
     std::string err;
-    bool have_params = false, have_tubes = false, have_segments = false, have_bins = false,
-         have_omega = false, have_u = false;
-    wt_params params{};
-    wt::Run run{};
-    // host copy of the tube table
-    std::vector<int32_t> npoint;
+    bool have_segments = false, have_bins = false, have_omega = false, have_u = false;
+    bool tubes_dirty = true;
+    // instances
+    std::vector<wt::Instance> inst;
+    int cur = 0;                          // instance addressed by the per-instance calls
+    std::vector<int> order;               // instances sorted by N (arena order)
+    wt::Run *runs_dev = nullptr;
+    std::vector<wt::Run> runs_host;
+    wt::Run run{};                        // parameters shared by all instances (from instance 0)
+    // host copy of the concatenated tube table
     std::vector<int64_t> pt_base, sec_base;
     // device buffers
     wt::Geo geo{};
@@ -90,21 +118,24 @@ struct wt_ctx {
     std::vector<void *> seg_allocs;       // sub-point level buffers
     wt::Bins bins{};
     std::vector<void *> bin_allocs;
-    void *omega = nullptr;                // 3*N^3 of float or double
-    void *u = nullptr;                    // 3*N^3
-    void *spec = nullptr;                 // 3 * N * N * (N/2+1) complex
-    long long *acc = nullptr;             // 3*N^3 fixed-point accumulators (FP32 mode)
+    void *omega = nullptr;                // arena: sum of 3*N_i^3 values (float or double)
+    void *u = nullptr;
+    void *spec = nullptr;                 // arena: sum of 3*N_i*N_i*(N_i/2+1) complex
+    long long *acc = nullptr;             // arena: fixed-point accumulators (FP32 mode)
+    int64_t field_elems = 0, spec_elems = 0;
     double *partials = nullptr;           // per-block partial sums of the spectral pass
     size_t partials_cap = 0;
-    void *twiddle = nullptr;              // N complex twiddles exp(-2 pi i k / N)
-    wt::DeviceScalars *scal = nullptr;
-    double *E_w = nullptr, *E_u = nullptr;   // device spectra [nek]
-    int nek = 0;
+    std::vector<int> tw_N;                // grid sizes with a twiddle table
+    std::vector<void *> tw_tab;           // twiddles exp(-2 pi i k / N) + digit-reversal table
+    wt::DeviceScalars *scal = nullptr;    // [ninst]
+    double *E_w = nullptr, *E_u = nullptr;   // spectra arenas
+    int64_t E_elems = 0;
     bool have_spectra = false;
     // stats
     wt_seg_stats seg_stats{};
     wt_deposit_stats dep_stats{};
     int fixed_shift = 0;
+    int cols_per_item = 256;
     bool theta_exact_all = false;         // WT_THETA_EXACT=1: reference sum for every section
     unsigned int theta_redo = 0;          // sections re-evaluated exactly by the fast path
     double stage_ms[4] = {0, 0, 0, 0};
@@ -112,6 +143,7 @@ struct wt_ctx {
     // per-kernel timing slots (see wt::kSlot*), events recorded on `stream`
     hipEvent_t kev[16][2] = {};
     bool kvalid[16] = {};
+    int ninst() const { return (int)inst.size(); }
 };
 
 namespace wt {
@@ -127,14 +159,18 @@ inline void kt_end(wt_ctx *c, int slot) {
     if (c->kev[slot][1] && hipEventRecord(c->kev[slot][1], c->stream) == hipSuccess) c->kvalid[slot] = true;
 }
 // segments.hip
-hipError_t launch_segments(wt_ctx *c);       // points -> sections -> sub-points (needs sp_base alloc inside)
+hipError_t launch_segments(wt_ctx *c);       // points -> sections -> sub-points
 hipError_t launch_bins(wt_ctx *c);           // per sub-point boxes -> work items
 hipError_t launch_get_bins(wt_ctx *c, int32_t *d_lohi, double *d_R);
+hipError_t launch_seg_stats(wt_ctx *c, wt_seg_stats *out);
 // deposit.hip
 hipError_t launch_deposit(wt_ctx *c);
-// fft.hip
-hipError_t launch_biot_savart(wt_ctx *c);
-// stats.hip
+// fft.hip: mode 0 = Biot-Savart + normalisation of every instance; mode 1 = spectrum of the
+// resident velocity of instance `only`
+hipError_t launch_spectral(wt_ctx *c, int mode, int only);
+bool fft_size_supported(int N);
+void fft_reversal(int N, std::vector<int32_t> &rev);
+// stats.hip: structure functions of instance c->cur
 hipError_t launch_sf(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const int32_t *d_seps,
                      double *d_out);
 // scan.hip

@@ -87,6 +87,18 @@ typedef struct {
  * subroutine.f90:860-919). */
 int wt_create(wt_ctx **ctx, int32_t n, int32_t precision, int32_t device);
 void wt_destroy(wt_ctx *ctx);
+
+/* Ensemble of ninst independent woven fields with grid sizes n[i] in one context (no
+ * counterpart in the reference, which makes one field per run). Their tubes are concatenated,
+ * so geometry, binning and deposition run as single launches over all instances, and the FFTs
+ * run once per group of equal N. Each instance gets its own wt_set_params/wt_set_tubes after
+ * wt_select_instance; wt_generate and the stage calls then process every instance. */
+int wt_create_batch(wt_ctx **ctx, int32_t ninst, const int32_t *n, int32_t precision, int32_t device);
+/* Instance addressed by wt_set_params, wt_set_tubes, wt_spectrum, wt_structure_functions,
+ * wt_get_field, wt_set_field and wt_instance_scalars (0 after creation). */
+int wt_select_instance(wt_ctx *ctx, int32_t inst);
+/* Etot before normalisation, up and Enstrophy of the selected instance (after Biot-Savart). */
+int wt_instance_scalars(wt_ctx *ctx, double *etot, double *up, double *enstrophy);
 const char *wt_last_error(const wt_ctx *ctx);
 
 /* Replaces the header read of parameters_input.dat (tube.f90:191-216). p->N must equal the
@@ -104,6 +116,8 @@ int wt_set_tubes(wt_ctx *ctx, int32_t nline, const int32_t *npoint, const double
 /* Replaces tube.f90:392-751: cze/Len_Disc/ck/ckk, quintic splines, turning angle -> ntf/dtf,
  * sub-points with the seam fix. FP64, bit-identical to the reference's operation order. */
 int wt_build_segments(wt_ctx *ctx, wt_seg_stats *out);
+/* Counters of the last wt_build_segments (also run inside wt_generate). */
+int wt_get_seg_stats(wt_ctx *ctx, wt_seg_stats *out);
 
 /* Replaces tube.f90:766-798 (unwrap, Rtube, integer bounding box) for every sub-segment of
  * tubes line_s..line_e, and lays out the column work of the deposition. Integer-exact. */
@@ -112,6 +126,8 @@ int wt_bin(wt_ctx *ctx);
 /* Replaces tube.f90:800-892: vorticity of every (cell, sub-segment) pair that passes the
  * reference's tests, summed over sub-segments. Leaves omega resident on the device. */
 int wt_deposit(wt_ctx *ctx, wt_deposit_stats *out);
+/* Counters of the last deposition (also run inside wt_generate). */
+int wt_get_deposit_stats(wt_ctx *ctx, wt_deposit_stats *out);
 
 /* Replaces tube.f90:985-1043 (+ subroutine.f90:983-1084, 1550-1569): forward FFT of omega,
  * u_hat = i k x w_hat / k^2 with the DC mode zeroed, inverse FFT, then normalisation of omega
