@@ -130,6 +130,32 @@ def cpu_baseline(p, t, budget_s: float):
             "seconds_parts": {"geometry": t_geo, "deposition": t_dep, "fft": t_fft}}
 
 
+def copy_probe(nbytes: int = 1 << 30, reps: int = 3):
+    """A plain elementwise copy of known size: each launch reads and writes exactly nbytes."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe": {"bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                                     "launches": reps, "kernel": "elementwise mul (y = x * 1)"}}), flush=True)
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            tab = json.load(f)
+        e = tab["workloads"][workload][kernel]
+        return float(e["fetch_kib"] * 1024 * tab["calibration"]["fetch_factor"] +
+                     e["write_kib"] * 1024 * tab["calibration"]["write_factor"])
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,11 +164,17 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c2", "c1", "small"])
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain 1 GiB -> 1 GiB device copy kernel (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU decomposition is described in DESIGN.md and not built.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
 
     from fastweavturb_amd.api import WovenGPU
     t0 = time.time()
@@ -179,7 +211,7 @@ def main():
         sec = kernels[dom] * 1e-3
         achieved = amount / sec / (1e9 if unit == "GB/s" else 1e12)
         roof = {"kernel": dom, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-                "frac": achieved / peak, "traffic": None, "ms": kernels[dom]}
+                "frac": achieved / peak, "traffic": measured_traffic(wname, dom), "ms": kernels[dom]}
     cpu = None
     if not args.no_cpu_baseline:
         t0 = time.time()

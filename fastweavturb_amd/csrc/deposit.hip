@@ -11,10 +11,10 @@
 //
 // Precision modes:
 //   double: the reference formulas in the reference order; accumulation by FP64 atomics.
-//   float : geometry, tests and the spline parameter zeta in FP64; the Gaussian core, the
-//           spline evaluation and the Frenet terms in FP32 in coordinates local to the
-//           sub-segment start; accumulation in 64-bit fixed point (exact and
-//           order-independent, so results are bitwise reproducible), converted once at the end.
+//   float : geometry, tests, zeta, the spline point and the offset to it in FP64; for constant
+//           cores without twist the Gaussian core in FP32, otherwise the whole reference
+//           formula in FP64; accumulation in 64-bit fixed point (exact and order-independent,
+//           so results are bitwise reproducible), converted once at the end.
 // This file is compiled with -ffp-contract=off; the FP32 evaluation re-enables contraction.
 #include "wt_device.h"
 
@@ -165,13 +165,14 @@ __device__ __forceinline__ void accumulate_fixed(const DepArgs &A, const SubSeg 
 // ---- hit evaluation, FP32 mode ----------------------------------------------------------
 // Constant core radius and no twist (sigma_ratio = 1, eta = 0; every benchmark workload): the
 // reference's contribution reduces exactly to tp*Gamma*T (t5..t8 and the straight-branch extras
-// vanish). The spline point and tangent are evaluated in FP64 in coordinates local to the
-// sub-segment start (FP64 FMAs run at the FP32 rate on CDNA4, and the quintic's high-order terms
-// cancel), the Gaussian core in FP32. Otherwise the curvature terms carry 1/(1 - kappa rho cos),
-// which amplifies rounding near its zero, so the whole reference formula runs in FP64.
+// vanish). The spline point, its tangent and the offset xc are the reference's FP64 values
+// (spline wrapped to [-pi, pi), then cminus's single +-2pi correction, which is not a full
+// minimum image: cells of an extended box more than 3pi from the wrapped spline point keep a
+// far offset and receive the negative cut-off term, as in the reference). The Gaussian core runs
+// in FP32. Otherwise the curvature terms carry 1/(1 - kappa rho cos), which amplifies rounding
+// near its zero, so the whole reference formula runs in FP64.
 __device__ void eval_hit_f32(const DepArgs &A, const Hit &h) {
     const Geo &g = A.g;
-    const Run &r = A.r;
     SubSeg sg;
     subseg_load(g, A.r, h.q, sg);
     const Run &ri = g.runs[g.tube_inst[sg.tube]];
@@ -191,19 +192,15 @@ __device__ void eval_hit_f32(const DepArgs &A, const Hit &h) {
                 (X[2] - sg.ci1[2]) * sg.tci0[2];
     double zeta = (sg.s1 * t0 - sg.s0 * t1) / sg.ndci0;
     double z = (zeta - g.czl[sg.sec]) * LD;
+    double cz3[3], dc[3], xcd[3];
+    spline_pos(z, coe, cz3);
+    spline_d1(z, coe, dc);
+    cminus(X, cz3, xcd);
     float xc[3], dcf[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const double *k = coe + 6 * c;
-        double d0 = k[0] - sg.ci0[c];           // spline constant term, nearest periodic image
-        d0 = d0 - kPi2 * rint(d0 / kPi2);
-        double cl = d0 + z * (k[1] + z * (k[2] + z * (k[3] + z * (k[4] + z * k[5]))));
-        double dc = k[1] + z * (2.0 * k[2] + z * (3.0 * k[3] + z * (4.0 * k[4] + z * (5.0 * k[5]))));
-        double d = a[c] - cl;                   // minimum image of X - c (cminus)
-        if (d > kPi) d -= kPi2;
-        if (d < -kPi) d += kPi2;
-        xc[c] = (float)d;
-        dcf[c] = (float)dc;
+        xc[c] = (float)xcd[c];
+        dcf[c] = (float)dc[c];
     }
     const float Gamma = (float)g.gamma[l];
     const double smin = g.sigma[l];

@@ -167,7 +167,10 @@ def test_spectrum_and_sf_of_given_velocity(oracle):
     np.testing.assert_allclose(S, Sref, rtol=1e-10, atol=1e-12 * np.abs(Sref).max())
     xs = np.arange(1, N // 2 + 1)
     Sx = g.structure_functions(1, 2, 6, xs)
-    np.testing.assert_allclose(Sx, oracle.sf_x(u[0], 2, 6, xs), rtol=1e-11)
+    Sx_ref = oracle.sf_x(u[0], 2, 6, xs)
+    # odd orders at r = N/2 vanish identically (the increments pair up as +d, -d): compare those
+    # against the scale of the whole table
+    np.testing.assert_allclose(Sx, Sx_ref, rtol=1e-11, atol=1e-13 * np.abs(Sx_ref).max())
 
 
 @pytest.mark.parametrize("precision", [64, 32])
