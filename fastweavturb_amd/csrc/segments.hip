@@ -198,56 +198,113 @@ __global__ void k_theta(Geo g, Run r, int32_t *__restrict__ err) {
     if (!theta_exact(g, r, s)) atomicOr(err, 1);
 }
 
-// Gauss-Legendre nodes (positive half) and weights on [-1, 1], 8 points.
-__constant__ double kGL8x[4] = {0.1834346424956498049394761, 0.5255324099163289858177390,
-                                0.7966664774136267395915539, 0.9602898564975362316835609};
-__constant__ double kGL8w[4] = {0.3626837833783619829651504, 0.3137066458778872873379622,
-                                0.2223810344533744705443560, 0.1012285362903762591525314};
-
-__device__ __forceinline__ double gl8(const double *coe, double a, double b) {
-    double m = 0.5 * (a + b), hw = 0.5 * (b - a), acc = 0.0;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        double d = hw * kGL8x[i];
-        acc += kGL8w[i] * (turn_rate(m - d, coe) + turn_rate(m + d, coe));
-    }
-    return acc * hw;
+// Fast turning angle with a certified error bound. Only the integer
+// ntf = ceiling(ntf_times*theta/pi2) + 1 leaves this stage (dtf is (czr-czl)/ntf), so theta itself
+// need not be reproduced bit for bit: the reference's sum of ntf_pre+1 samples of
+// |c' x c''| / |c'|^2 is re-evaluated in FP32 at the same sample points, from polynomial
+// coefficients in t = z/zM on [0, 1] (c' x c'' has degree 7, |c'|^2 degree 8; coefficients
+// formed in FP64 and rounded once), with a running bound on the FP32 error of every sample
+// (Horner with FMA: |err| <= gamma * sum (1+n)|a_n| for t in [0, 1], plus the rounding of the
+// coefficients, of t and of the sqrt/reciprocal). A section is decided here when the interval
+// [x - dx, x + dx] around x = ntf_times*theta/pi2 contains no integer; every other section is
+// listed and re-evaluated with the reference's exact FP64 sum (k_theta_listed), so ntf and dtf
+// always equal the reference's.
+__device__ __forceinline__ float horner8(const float *a, int n, float t) {
+    float v = a[n];
+    for (int k = n - 1; k >= 0; k--) v = __builtin_fmaf(v, t, a[k]);
+    return v;
 }
 
-// Fast turning angle with an exactness guard. The reference's sum of ntf_pre+1 samples equals
-// the trapezoid rule plus half of each end sample, and for a smooth integrand the trapezoid rule
-// equals the integral up to O(h^2/ntf_pre^2) (Euler-Maclaurin). The integral is estimated with
-// Gauss-Legendre rules of 8 and 2x8 points; the integer ntf = ceil(ntf_times*theta/2pi) + 1 is
-// accepted only when the two rules agree to 1e-9 and ntf_times*theta/2pi lies farther than
-// max(1e-6, 2/ntf_pre^2) (relative) from an integer. Every other section is re-evaluated with
-// the reference's exact sum (k_theta_listed), so ntf and dtf match the reference bit for bit.
-__global__ void k_theta_fast(Geo g, Run, int32_t *__restrict__ redo, unsigned int *__restrict__ nredo) {
+__global__ __launch_bounds__(128) void k_theta_fast(Geo g, Run, int32_t *__restrict__ redo,
+                                                    unsigned int *__restrict__ nredo) {
     int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= g.nsec_all) return;
     int l = g.sec_tube[s];
     const Run &r = g.runs[g.tube_inst[l]];
-    double LD = g.Len_Disc[l];
+    const double LD = g.Len_Disc[l];
     double coe[18];
 #pragma unroll
     for (int k = 0; k < 18; k++) coe[k] = g.coe[18 * s + k];
-    double czl = g.czl[s], czr = g.czr[s];
+    const double czl = g.czl[s], czr = g.czr[s];
     const int M = r.ntf_pre;
-    double dtfp = (czr - czl) / (1.0 * M);
-    double z0 = 0.0;
-    double zM = ((czl + dtfp * (double)M) - czl) * LD;
-    double dz = dtfp * LD;
-    double I8 = gl8(coe, z0, zM);
-    double zh = 0.5 * zM;
-    double I16 = gl8(coe, z0, zh) + gl8(coe, zh, zM);
-    double f0 = turn_rate(z0, coe), fM = turn_rate(zM, coe);
-    double theta = I16 + 0.5 * dz * (f0 + fM);
-    double x = (double)r.ntf_times * theta / kPi2;
-    double tol = fmax(1e-6, 2.0 / ((double)M * M)) * fmax(1.0, x);
-    double fl = floor(x);
-    bool ok = isfinite(x) && x > tol && (x - fl) > tol && (fl + 1.0 - x) > tol &&
-              fabs(I8 - I16) <= 1e-9 * fabs(I16) && zM > 0.0;
+    const double dtfp = (czr - czl) / (1.0 * M);
+    const double dz = dtfp * LD;
+    const double zM = dz * (double)M;
+    // c'(z) = sum_m b[c][m] t^m, c''(z) = sum_m e[c][m] t^m with z = zM t
+    double b[3][5], e[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        double zp = 1.0;
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            b[c][m] = (double)(m + 1) * coe[6 * c + m + 1] * zp;
+            if (m < 4) e[c][m] = (double)((m + 2) * (m + 1)) * coe[6 * c + m + 2] * zp;
+            zp *= zM;
+        }
+    }
+    float wf[3][8], gf[9];
+    double Wb2 = 0.0, Gb = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        double wb = 0.0;
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            double v = 0.0;
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                int mm = n - m;
+                if (mm >= 0 && mm < 4) v += b[c1][m] * e[c2][mm] - b[c2][m] * e[c1][mm];
+            }
+            wf[c][n] = (float)v;
+            wb += (double)(1 + n) * fabs((double)wf[c][n]);
+        }
+        Wb2 += wb * wb;
+    }
+#pragma unroll
+    for (int n = 0; n < 9; n++) {
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                int mm = n - m;
+                if (mm >= 0 && mm < 5) v += b[c][m] * b[c][mm];
+            }
+        gf[n] = (float)v;
+        Gb += (double)(1 + n) * fabs((double)gf[n]);
+    }
+    const float invM = 1.0f / (float)M;
+    double acc = 0.0;           // sum of f
+    float sg = 0.0f, sfg = 0.0f, gmin = 3.0e38f;
+    for (int j = 0; j <= M; j++) {
+        float t = (float)j * invM;
+        float wx = horner8(wf[0], 7, t), wy = horner8(wf[1], 7, t), wz = horner8(wf[2], 7, t);
+        float gg = horner8(gf, 8, t);
+        float n2 = __builtin_fmaf(wx, wx, __builtin_fmaf(wy, wy, wz * wz));
+        float rg = __builtin_amdgcn_rcpf(gg);
+        float f = __builtin_amdgcn_sqrtf(n2) * rg;
+        acc += (double)f;
+        sg += rg;
+        sfg = __builtin_fmaf(f, rg, sfg);
+        gmin = fminf(gmin, gg);
+    }
+    // error bound of the FP32 sum against the reference's FP64 sum (both times dz)
+    const double u = 5.9604644775390625e-08;             // 2^-24
+    const double zrel = 4.0e-15 * LD / fmax(zM, 1e-300);  // the reference's sample points
+    const double kw = 16.0 * u + zrel, kg = 20.0 * u + zrel;
+    const double Wb = sqrt(Wb2);
+    const double sum_b = 1.1 * (kw * Wb * (double)sg + 6.0 * u * acc + kg * Gb * (double)sfg) * 1.01;
+    const double theta = acc * dz;
+    const double err = sum_b * dz + 1.0e-9 * theta;
+    const double sc = (double)r.ntf_times / kPi2;
+    const double x = sc * theta;
+    const double dx = sc * err + 1.0e-12 * x + 1.0e-12;
+    const double lo = floor(x - dx), hi = floor(x + dx);
+    bool ok = isfinite(x) && isfinite(dx) && M > 0 && zM > 0.0 && (double)gmin > 1000.0 * u * Gb &&
+              x - dx > 0.0 && lo == hi && x + dx < 1.0e7;
     if (ok) {
-        int32_t ntf = (int32_t)fl + 2;  // ceil(x) + 1 for non-integer x
+        int32_t ntf = (int32_t)lo + 2;  // ceiling(x_ref) + 1 for x_ref in (lo, lo + 1)
         g.ntf[s] = ntf;
         g.dtf[s] = (czr - czl) / (1.0 * ntf);
     } else {
@@ -457,50 +514,71 @@ hipError_t launch_get_bins(wt_ctx *c, int32_t *d_lohi, double *d_R) {
 namespace wt {
 
 // Counters of wt_seg_stats: max ntf, min dtf (all sections), sum of ntf (all sections) and
-// number of sub-segments (real sections only).
-__global__ void k_seg_stats(Geo g, unsigned long long *__restrict__ out) {
-    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    int mx = 0;
-    unsigned long long sum = 0, nseg = 0, dmin = ~0ull;
-    if (s < g.nsec_all) {
-        int l = g.sec_tube[s];
-        mx = g.ntf[s];
-        sum = (unsigned long long)g.ntf[s];
-        if ((int64_t)s < g.sec_base[l + 1] - 1) nseg = (unsigned long long)g.ntf[s];
-        double d = g.dtf[s];
-        dmin = (unsigned long long)__double_as_longlong(d);  // dtf > 0: bit order = value order
-    }
+// number of sub-segments (real sections only). Each block writes one partial row; one block
+// combines the rows (no contended atomics).
+struct SegAcc {
+    unsigned long long mx, sum, nseg, dmin;
+};
+
+__device__ __forceinline__ void seg_acc_combine(SegAcc &a, const SegAcc &b) {
+    a.mx = b.mx > a.mx ? b.mx : a.mx;
+    a.sum += b.sum;
+    a.nseg += b.nseg;
+    a.dmin = b.dmin < a.dmin ? b.dmin : a.dmin;
+}
+
+__device__ __forceinline__ SegAcc seg_acc_block(SegAcc v) {
+    __shared__ SegAcc part[4];
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
-        int m2 = __shfl_xor(mx, o, 64);
-        mx = m2 > mx ? m2 : mx;
-        sum += __shfl_xor(sum, o, 64);
-        nseg += __shfl_xor(nseg, o, 64);
-        unsigned long long d2 = __shfl_xor(dmin, o, 64);
-        dmin = d2 < dmin ? d2 : dmin;
+        SegAcc w{__shfl_xor(v.mx, o, 64), __shfl_xor(v.sum, o, 64), __shfl_xor(v.nseg, o, 64),
+                 __shfl_xor(v.dmin, o, 64)};
+        seg_acc_combine(v, w);
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax(out + 0, (unsigned long long)mx);
-        atomicAdd(out + 1, sum);
-        atomicAdd(out + 2, nseg);
-        atomicMin(out + 3, dmin);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    SegAcc r = part[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) seg_acc_combine(r, part[w]);
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_seg_stats(Geo g, SegAcc *__restrict__ rows) {
+    SegAcc v{0, 0, 0, ~0ull};
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < g.nsec_all;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        int l = g.sec_tube[s];
+        unsigned long long n = (unsigned long long)g.ntf[s];
+        SegAcc w{n, n, (int64_t)s < g.sec_base[l + 1] - 1 ? n : 0ull,
+                 (unsigned long long)__double_as_longlong(g.dtf[s])};  // dtf > 0: bit order = value order
+        seg_acc_combine(v, w);
     }
+    SegAcc r = seg_acc_block(v);
+    if (threadIdx.x == 0) rows[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(256) void k_seg_stats_rows(const SegAcc *__restrict__ rows, int nrows,
+                                                        SegAcc *__restrict__ out) {
+    SegAcc v{0, 0, 0, ~0ull};
+    for (int i = threadIdx.x; i < nrows; i += blockDim.x) seg_acc_combine(v, rows[i]);
+    SegAcc r = seg_acc_block(v);
+    if (threadIdx.x == 0) *out = r;
 }
 
 hipError_t launch_seg_stats(wt_ctx *c, wt_seg_stats *o) {
-    unsigned long long *d = nullptr, h[4] = {0, 0, 0, ~0ull};
-    WT_HIP(hipMalloc(&d, sizeof h));
-    WT_HIP(hipMemcpyAsync(d, h, sizeof h, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_seg_stats, dim3(grid_for(c->geo.nsec_all, 256)), dim3(256), 0, c->stream, c->geo, d);
+    const int blocks = 1024;
+    SegAcc *d = nullptr, h{};
+    WT_HIP(hipMallocAsync((void **)&d, sizeof(SegAcc) * (blocks + 1), c->stream));
+    hipLaunchKernelGGL(k_seg_stats, dim3(blocks), dim3(256), 0, c->stream, c->geo, d + 1);
+    hipLaunchKernelGGL(k_seg_stats_rows, dim3(1), dim3(256), 0, c->stream, (const SegAcc *)(d + 1), blocks, d);
     WT_HIP(hipGetLastError());
-    WT_HIP(hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    WT_HIP(hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    WT_HIP(hipFreeAsync(d, c->stream));
     WT_HIP(hipStreamSynchronize(c->stream));
-    (void)hipFree(d);
-    o->ntf_max = (int32_t)h[0];
-    o->ntf_sum = (int64_t)h[1];
-    o->nseg_all = (int64_t)h[2];
+    o->ntf_max = (int32_t)h.mx;
+    o->ntf_sum = (int64_t)h.sum;
+    o->nseg_all = (int64_t)h.nseg;
     union { unsigned long long u; double d; } bits;
-    bits.u = h[3];
+    bits.u = h.dmin;
     o->dtf_min = bits.d;
     return hipSuccess;
 }
