@@ -25,7 +25,7 @@ METRIC = "Mpts/s woven field at 512^3 FP32; time-to-target-spectrum (±5%) vs sw
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
-SLOT_NAMES = ["theta", "subpoints", "bin_count", "deposit", "fixed_to_float", "fft_x", "fft_y",
+SLOT_NAMES = ["theta", "subpoints", "bin_count", "deposit", "bin_fill", "fft_x", "fft_y",
               "fft_z_biot_savart", "fft_y_inv", "fft_x_inv", "spectral_reduce"]
 
 

@@ -44,7 +44,7 @@ class _SegStats(C.Structure):
 
 class _DepStats(C.Structure):
     _fields_ = [("columns", C.c_int64), ("candidates", C.c_int64), ("hits", C.c_int64),
-                ("fixed_point_shift", C.c_int32), ("overflow", C.c_int32)]
+                ("fixed_point_shift", C.c_int32), ("overflow", C.c_int32), ("pairs", C.c_int64)]
 
 
 _lib = None

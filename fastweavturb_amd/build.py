@@ -21,6 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = {
     "segments.hip": ["-ffp-contract=off"],
     "deposit.hip": ["-ffp-contract=off"],
+    "bricks.hip": ["-ffp-contract=off"],
     "scan.hip": [],
     "fft.hip": [],
     "stats.hip": [],

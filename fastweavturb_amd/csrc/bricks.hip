@@ -1,0 +1,219 @@
+// Spatial binning of the deposition work (tube.f90:793-798 boxes).
+//
+// The grid of each instance is cut into bricks of kBrickX x kBrickY x kBrickZ cells (partial
+// bricks at the upper edge when a brick edge does not divide N). A sub-segment is paired with every brick that its
+// integer bounding box (in extended indices, wrapped periodically) overlaps, once per periodic
+// image of the brick that the box meets. The deposition
+// kernel then owns one brick at a time and accumulates all of its pairs in LDS, so no global
+// atomics touch the field.
+//
+// Pairs are built in two passes over the sub-segments: count per brick, exclusive scan, fill.
+// Both passes combine the per-brick atomics of a wavefront (consecutive sub-segments of a tube
+// mostly share their bricks), so a brick counter sees about one atomic per wave.
+// Compiled with -ffp-contract=off: the boxes are the reference's FP64 expressions.
+#include <algorithm>
+
+#include "wt_device.h"
+
+namespace wt {
+
+__device__ __forceinline__ uint64_t lanemask_lt64() {
+    unsigned lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Adds 1 to cnt[key] for every lane with `active`; one atomic per distinct key in the wave.
+// Returns the counter value before this lane's increment (its slot).
+__device__ __forceinline__ uint32_t wave_count_add(uint32_t *cnt, int64_t key, bool active) {
+    uint64_t pending = __ballot(active);
+    uint32_t slot = 0;
+    const int lane = (int)__lane_id();
+    while (pending) {
+        int leader = __ffsll((unsigned long long)pending) - 1;
+        int64_t lk = __shfl(key, leader, 64);
+        uint64_t m = __ballot(active && key == lk) & pending;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(cnt + lk, (uint32_t)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if ((m >> lane) & 1ull) slot = base + (uint32_t)__popcll(m & lanemask_lt64());
+        pending &= ~m;
+    }
+    return slot;
+}
+
+// Bricks of one axis met by the extended index range [lo, hi] (lo >= 1 - N, hi <= 2N): for each
+// periodic image k in {-1, 0, 1}, the part of [lo, hi] in [1 + kN, N + kN] covers the wrapped
+// bricks f[k] .. l[k] (empty when f > l). A brick met through two images gives two pairs.
+struct AxisImages {
+    int32_t f[3], l[3];
+    __device__ int32_t count() const {
+        int32_t n = 0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) n += l[k] >= f[k] ? l[k] - f[k] + 1 : 0;
+        return n;
+    }
+    // e-th (brick, image) of the axis; image returned as 0..2 for k = -1..1
+    __device__ void at(int32_t e, int32_t &brick, int32_t &img) const {
+        img = 0;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            int32_t n = l[k] >= f[k] ? l[k] - f[k] + 1 : 0;
+            if (img == k && e >= n) { e -= n; img = k + 1; }
+        }
+        brick = f[img] + e;
+    }
+};
+
+__device__ __forceinline__ AxisImages axis_images(int32_t lo, int32_t hi, int32_t N, int32_t bd) {
+    AxisImages r;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int32_t s = (k - 1) * N;
+        const int32_t a = lo > 1 + s ? lo : 1 + s, b = hi < N + s ? hi : N + s;
+        r.f[k] = 0;
+        r.l[k] = -1;
+        if (a <= b) {
+            r.f[k] = (a - s - 1) / bd;
+            r.l[k] = (b - s - 1) / bd;
+        }
+    }
+    return r;
+}
+
+// Pass 0 (FILL = false): counts pairs per brick and flags boxes outside [1-N, 2N].
+// Pass 1 (FILL = true): writes every pair (sub-point index, image code kx + 3ky + 9kz) into its
+// brick's list.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restrict__ cnt,
+                                               const int64_t *__restrict__ off, uint32_t *__restrict__ pairs,
+                                               uint8_t *__restrict__ pair_img, int32_t *__restrict__ err) {
+    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    SubSeg sg;
+    bool ok = q < g.nsub_all && subseg_load(g, r0, q, sg);
+    AxisImages ab[3];
+    int32_t n = 0, nx = 1, ny = 1;
+    int64_t brick0 = 0;
+    int32_t nbx = 1, nby = 1;
+    if (ok) {
+        const Run &r = g.runs[g.tube_inst[sg.tube]];
+        brick0 = r.brick0;
+        nbx = r.nb[0];
+        nby = r.nb[1];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            if (sg.lo[c] < 1 - sg.N || sg.hi[c] > 2 * sg.N) ok = false;
+        if (!ok) {
+            if (!FILL) atomicOr(err, 2);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; c++) ab[c] = axis_images(sg.lo[c], sg.hi[c], sg.N, brick_dim(c));
+            nx = ab[0].count();
+            ny = ab[1].count();
+            n = nx * ny * ab[2].count();
+        }
+    }
+    // wave-uniform loop over the lanes' brick lists
+    int32_t nmax = n;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        int32_t w = __shfl_xor(nmax, o, 64);
+        nmax = w > nmax ? w : nmax;
+    }
+    for (int32_t t = 0; t < nmax; t++) {
+        bool act = t < n;
+        int64_t key = 0;
+        int32_t code = 0;
+        if (act) {
+            int32_t bxi, byi, bzi, kx, ky, kz;
+            ab[0].at(t % nx, bxi, kx);
+            ab[1].at((t / nx) % ny, byi, ky);
+            ab[2].at(t / (nx * ny), bzi, kz);
+            key = brick0 + ((int64_t)bzi * nby + byi) * nbx + bxi;
+            code = kx + 3 * ky + 9 * kz;
+        }
+        uint32_t slot = wave_count_add(cnt, key, act);
+        if (FILL && act) {
+            pairs[off[key] + slot] = (uint32_t)q;
+            pair_img[off[key] + slot] = (uint8_t)code;
+        }
+    }
+}
+
+static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename T>
+static hipError_t balloc(wt_ctx *c, T **p, size_t count) {
+    void *v = nullptr;
+    hipError_t e = hipMalloc(&v, sizeof(T) * (count ? count : 1));
+    if (e != hipSuccess) return e;
+    c->bin_allocs.push_back(v);
+    *p = (T *)v;
+    return hipSuccess;
+}
+
+hipError_t launch_bins(wt_ctx *c) {
+    Geo &g = c->geo;
+    Bins &b = c->bins;
+    hipStream_t st = c->stream;
+    for (void *p : c->bin_allocs) (void)hipFree(p);
+    c->bin_allocs.clear();
+    // brick table: (instance, bx, by, bz) of every brick, instances in arena order
+    int64_t nbricks = 0;
+    for (int i = 0; i < c->ninst(); i++) {
+        const Run &r = c->runs_host[i];
+        nbricks = std::max<int64_t>(nbricks, r.brick0 + (int64_t)r.nb[0] * r.nb[1] * r.nb[2]);
+    }
+    b.nbricks = nbricks;
+    std::vector<int4> tab((size_t)nbricks);
+    for (int i = 0; i < c->ninst(); i++) {
+        const Run &r = c->runs_host[i];
+        for (int z = 0; z < r.nb[2]; z++)
+            for (int y = 0; y < r.nb[1]; y++)
+                for (int x = 0; x < r.nb[0]; x++)
+                    tab[(size_t)(r.brick0 + ((int64_t)z * r.nb[1] + y) * r.nb[0] + x)] = make_int4(i, x, y, z);
+    }
+    uint32_t *cnt = nullptr;
+    int32_t *err = nullptr;
+    WT_HIP(balloc(c, &b.brick_tab, nbricks));
+    WT_HIP(balloc(c, &b.brick_off, nbricks + 1));
+    WT_HIP(balloc(c, &cnt, nbricks));
+    WT_HIP(balloc(c, &err, 1));
+    WT_HIP(hipMemcpyAsync(b.brick_tab, tab.data(), sizeof(int4) * nbricks, hipMemcpyHostToDevice, st));
+    WT_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * nbricks, st));
+    WT_HIP(hipMemsetAsync(err, 0, sizeof(int32_t), st));
+    kt_begin(c, kSlotBinCount);
+    hipLaunchKernelGGL(k_pairs<false>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, cnt,
+                       (const int64_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr, err);
+    kt_end(c, kSlotBinCount);
+    WT_HIP(hipGetLastError());
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    WT_HIP(exclusive_scan_i32_to_i64((const int32_t *)cnt, 0, b.brick_off, nbricks, st, &tmp, &tmp_bytes));
+    int64_t npairs = 0;
+    int32_t herr = 0;
+    WT_HIP(hipMemcpyAsync(&npairs, b.brick_off + nbricks, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    WT_HIP(hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    WT_HIP(hipStreamSynchronize(st));
+    if (tmp) (void)hipFree(tmp);
+    if (herr) return hipErrorInvalidValue;
+    b.npairs = npairs;
+    // the pair list is kept between calls and grown when needed
+    if ((size_t)npairs > c->pairs_cap) {
+        if (c->pairs_buf) (void)hipFree(c->pairs_buf);
+        c->pairs_buf = nullptr;
+        c->pairs_cap = 0;
+        size_t cap = (size_t)npairs + (size_t)npairs / 8 + 1024;
+        WT_HIP(hipMalloc((void **)&c->pairs_buf, (sizeof(uint32_t) + sizeof(uint8_t)) * cap));
+        c->pairs_cap = cap;
+    }
+    b.pairs = c->pairs_buf;
+    b.pair_img = (uint8_t *)(c->pairs_buf + c->pairs_cap);
+    WT_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * nbricks, st));
+    kt_begin(c, kSlotBinFill);
+    hipLaunchKernelGGL(k_pairs<true>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, cnt,
+                       (const int64_t *)b.brick_off, b.pairs, b.pair_img, err);
+    kt_end(c, kSlotBinFill);
+    return hipGetLastError();
+}
+
+}  // namespace wt

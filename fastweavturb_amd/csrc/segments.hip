@@ -352,30 +352,6 @@ __global__ void k_subpoints(Geo g) {
 
 // Per sub-point: number of grid columns of its sub-segment's box (0 if it starts none).
 // Columns run along the chord's dominant axis. Also counts the work items (column blocks).
-__global__ void k_bin_count(Geo g, Run r, Bins b, int32_t *__restrict__ nitems,
-                            int32_t *__restrict__ err) {
-    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q >= g.nsub_all) return;
-    SubSeg sg;
-    int64_t ncol = 0;
-    if (subseg_load(g, r, q, sg)) {
-        int ax = dominant_axis(sg.tci0);
-        int bx = (ax + 1) % 3, cxs = (ax + 2) % 3;
-        ncol = (int64_t)(sg.hi[bx] - sg.lo[bx] + 1) * (int64_t)(sg.hi[cxs] - sg.lo[cxs] + 1);
-#pragma unroll
-        for (int c = 0; c < 3; c++)  // wrap_idx needs ip + N - 1 >= 0 for every visited index
-            if (sg.lo[c] < 1 - sg.N || sg.hi[c] > 2 * sg.N) atomicOr(err, 2);
-    }
-    b.col_count[q] = ncol;
-    nitems[q] = (int32_t)((ncol + b.cols_per_item - 1) / b.cols_per_item);
-}
-
-__global__ void k_fill_items(Geo g, Bins b) {
-    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q >= g.nsub_all) return;
-    for (int64_t it = b.item_off[q]; it < b.item_off[q + 1]; it++) b.item_q[it] = (uint32_t)q;
-}
-
 __global__ void k_get_bins(Geo g, Run r, int32_t *__restrict__ lohi, double *__restrict__ R) {
     int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q >= g.nsub_all) return;
@@ -462,44 +438,6 @@ hipError_t launch_segments(wt_ctx *c) {
     kt_begin(c, kSlotSubpoints);
     hipLaunchKernelGGL(k_subpoints, dim3(grid_for(nsub, bs)), dim3(bs), 0, st, g);
     kt_end(c, kSlotSubpoints);
-    return hipGetLastError();
-}
-
-hipError_t launch_bins(wt_ctx *c) {
-    Geo &g = c->geo;
-    Bins &b = c->bins;
-    hipStream_t st = c->stream;
-    dfree(c->bin_allocs);
-    const int bs = 256;
-    int32_t *nitems = nullptr, *err = nullptr;
-    WT_HIP(dalloc(c, c->bin_allocs, &b.col_count, g.nsub_all));
-    WT_HIP(dalloc(c, c->bin_allocs, &b.item_off, g.nsub_all + 1));
-    WT_HIP(dalloc(c, c->bin_allocs, &nitems, g.nsub_all));
-    WT_HIP(dalloc(c, c->bin_allocs, &err, 1));
-    WT_HIP(hipMemsetAsync(err, 0, sizeof(int32_t), st));
-    kt_begin(c, kSlotBinCount);
-    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(g.nsub_all, bs)), dim3(bs), 0, st, g, c->run, b,
-                       nitems, err);
-    kt_end(c, kSlotBinCount);
-    void *tmp = nullptr;
-    size_t tmp_bytes = 0;
-    WT_HIP(exclusive_scan_i32_to_i64(nitems, 0, b.item_off, g.nsub_all, st, &tmp, &tmp_bytes));
-    int64_t *colsum = nullptr;
-    WT_HIP(dalloc(c, c->bin_allocs, &colsum, g.nsub_all + 1));
-    // total columns (for statistics): scan of the column counts
-    WT_HIP(exclusive_scan_i64(b.col_count, colsum, g.nsub_all, st, &tmp, &tmp_bytes));
-    int64_t items = 0, sumcol = 0;
-    int32_t herr = 0;
-    WT_HIP(hipMemcpyAsync(&items, b.item_off + g.nsub_all, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    WT_HIP(hipMemcpyAsync(&sumcol, colsum + g.nsub_all, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    WT_HIP(hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    WT_HIP(hipStreamSynchronize(st));
-    if (tmp) (void)hipFree(tmp);
-    if (herr) return hipErrorInvalidValue;
-    b.items = items;
-    b.columns = sumcol;
-    WT_HIP(dalloc(c, c->bin_allocs, &b.item_q, items));
-    hipLaunchKernelGGL(k_fill_items, dim3(grid_for(g.nsub_all, bs)), dim3(bs), 0, st, g, b);
     return hipGetLastError();
 }
 

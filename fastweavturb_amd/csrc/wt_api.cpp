@@ -63,9 +63,7 @@ static size_t mem_cap_bytes() {
 
 static size_t field_bytes(int64_t field_elems, int64_t spec_elems, int precision) {
     size_t es = precision == 64 ? 8 : 4;
-    size_t b = 2 * (size_t)field_elems * es + (size_t)spec_elems * 2 * es;
-    if (precision == 32) b += (size_t)field_elems * 8;
-    return b;
+    return 2 * (size_t)field_elems * es + (size_t)spec_elems * 2 * es;
 }
 
 template <typename T>
@@ -114,7 +112,7 @@ void wt_destroy(wt_ctx *c) {
     free_list(c->bin_allocs);
     free_list(c->seg_allocs);
     free_list(c->geo_allocs);
-    void *bufs[] = {c->omega, c->u, c->spec, c->acc, c->partials, c->scal, c->E_w, c->E_u, c->runs_dev};
+    void *bufs[] = {c->omega, c->u, c->spec, c->pairs_buf, c->partials, c->scal, c->E_w, c->E_u, c->runs_dev};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (void *p : c->tw_tab)
@@ -190,8 +188,6 @@ int wt_create_batch(wt_ctx **out, int32_t ninst, const int32_t *n, int32_t preci
     if ((e = hipMalloc(&c->omega, fo * es)) != hipSuccess) return fail(e, "omega");
     if ((e = hipMalloc(&c->u, fo * es)) != hipSuccess) return fail(e, "u");
     if ((e = hipMalloc(&c->spec, so * 2 * es)) != hipSuccess) return fail(e, "spectrum buffer");
-    if (precision == 32 && (e = hipMalloc((void **)&c->acc, fo * sizeof(long long))) != hipSuccess)
-        return fail(e, "accumulator");
     // per grid size: twiddles exp(-2 pi i k / N) in the compute precision, then the
     // digit-reversal table
     for (int pos = 0; pos < ninst; pos++) {
@@ -229,8 +225,6 @@ int wt_create_batch(wt_ctx **out, int32_t ninst, const int32_t *n, int32_t preci
     {
         const char *te = getenv("WT_THETA_EXACT");
         c->theta_exact_all = te && te[0] == '1';
-        const char *cp = getenv("WT_COLS_PER_ITEM");
-        if (cp && atoi(cp) > 0) c->cols_per_item = atoi(cp);
     }
     *out = c;
     return WT_OK;
@@ -345,6 +339,7 @@ static int upload_tubes(wt_ctx *c) {
         return set_err(c, WT_ENOMEM, "tube data needs %zu bytes beyond the fields, above WT_DEVICE_MEM_CAP", need);
     // per-instance run table
     c->runs_host.resize(ni);
+    int64_t brick0 = 0;
     for (int i = 0; i < ni; i++) {
         const wt::Instance &I = c->inst[i];
         const wt_params &p = I.params;
@@ -362,6 +357,9 @@ static int upload_tubes(wt_ctx *c) {
         r.dx = wt::kPi2 / I.N;  // lx/nx (tube.f90:274)
         r.start = -wt::kPi;
         r.field_off = I.field_off;
+        for (int a = 0; a < 3; a++) r.nb[a] = (I.N + wt::brick_dim(a) - 1) / wt::brick_dim(a);
+        r.brick0 = brick0;
+        brick0 += (int64_t)r.nb[0] * r.nb[1] * r.nb[2];
     }
     c->run = c->runs_host[0];
     free_list(c->bin_allocs);
@@ -470,7 +468,6 @@ int wt_get_seg_stats(wt_ctx *c, wt_seg_stats *out) {
 int wt_bin(wt_ctx *c) {
     CHECK_CTX(c);
     if (!c->have_segments) return set_err(c, WT_ESTATE, "wt_bin needs wt_build_segments");
-    c->bins.cols_per_item = c->cols_per_item;
     hipError_t e = wt::launch_bins(c);
     if (e == hipErrorInvalidValue)
         return set_err(c, WT_ERANGE, "a tube's cut-off radius spans more than the periodic box");

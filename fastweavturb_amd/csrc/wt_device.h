@@ -27,8 +27,10 @@ __device__ __forceinline__ void cminus(const double *r, const double *l, double 
 }
 
 // sigma_func (subroutine.f90:164-173); omega is the integer floor(length/pi + 0.5) * n_T
+// (for smax == smin the reference's expression is smin + 0 * finite = smin exactly)
 __device__ __forceinline__ double sigma_func(double zeta, double length, double smax, double smin,
                                              int n_T) {
+    if (smax == smin) return smin;
     double om = (double)((long long)floor(length / kPi + 0.5) * (long long)n_T);
     return smin + (smax - smin) * (sin(om * zeta) + 1.0) / 2.0;
 }

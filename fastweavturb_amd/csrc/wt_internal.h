@@ -29,7 +29,13 @@ struct Run {
     double dx;                  // 2*pi/N
     double start;               // -pi
     int64_t field_off;          // element offset of the instance's 3*N^3 field block
+    int64_t brick0;             // first global brick of the instance
+    int32_t nb[3];              // bricks per axis, ceil(N / kBrickDim[axis])
 };
+
+// The deposition accumulates one brick of kBrickX * kBrickY * kBrickZ cells at a time in LDS.
+constexpr int kBrickX = 16, kBrickY = 16, kBrickZ = 8;
+__host__ __device__ constexpr int brick_dim(int axis) { return axis == 0 ? kBrickX : (axis == 1 ? kBrickY : kBrickZ); }
 
 // Device-side view of the tube table and geometry (all pointers are device memory).
 struct Geo {
@@ -64,14 +70,16 @@ struct Geo {
     double *ci;                 // [3*NSP] xyz interleaved
 };
 
-// Work layout of the deposition: sub-point q starts a sub-segment iff col_count[q] > 0.
+// Work layout of the deposition: for every brick, the sub-segments whose bounding box overlaps
+// it (pairs[brick_off[b] .. brick_off[b+1]) hold sub-point indices, in no particular order, and
+// pair_img which periodic image of the brick the box meets).
 struct Bins {
-    int64_t *col_count;         // [NSP] columns of the sub-segment's box (0 = not a sub-segment)
-    int64_t *item_off;          // [NSP+1] prefix sum of work items (column blocks)
-    uint32_t *item_q;           // [items] sub-point of each work item
-    int64_t items;
-    int64_t columns;
-    int32_t cols_per_item;      // columns per work item
+    int64_t nbricks;
+    int4 *brick_tab;            // [nbricks] (instance, bx, by, bz)
+    int64_t *brick_off;         // [nbricks+1]
+    uint32_t *pairs;            // [npairs]
+    uint8_t *pair_img;          // [npairs] periodic image of the brick, kx + 3ky + 9kz (k = 0..2)
+    int64_t npairs;
 };
 
 struct DeviceScalars {          // small device-resident results, one per instance
@@ -118,10 +126,11 @@ struct wt_ctx {
     std::vector<void *> seg_allocs;       // sub-point level buffers
     wt::Bins bins{};
     std::vector<void *> bin_allocs;
+    uint32_t *pairs_buf = nullptr;        // brick pair list, grown on demand
+    size_t pairs_cap = 0;
     void *omega = nullptr;                // arena: sum of 3*N_i^3 values (float or double)
     void *u = nullptr;
     void *spec = nullptr;                 // arena: sum of 3*N_i*N_i*(N_i/2+1) complex
-    long long *acc = nullptr;             // arena: fixed-point accumulators (FP32 mode)
     int64_t field_elems = 0, spec_elems = 0;
     double *partials = nullptr;           // per-block partial sums of the spectral pass
     size_t partials_cap = 0;
@@ -135,7 +144,6 @@ struct wt_ctx {
     wt_seg_stats seg_stats{};
     wt_deposit_stats dep_stats{};
     int fixed_shift = 0;
-    int cols_per_item = 256;
     bool theta_exact_all = false;         // WT_THETA_EXACT=1: reference sum for every section
     unsigned int theta_redo = 0;          // sections re-evaluated exactly by the fast path
     double stage_ms[4] = {0, 0, 0, 0};
@@ -149,7 +157,7 @@ struct wt_ctx {
 namespace wt {
 // kernel timing slots reported by wt_stage_times after the four stage totals
 enum : int {
-    kSlotTheta = 0, kSlotSubpoints, kSlotBinCount, kSlotDeposit, kSlotFixedToFloat,
+    kSlotTheta = 0, kSlotSubpoints, kSlotBinCount, kSlotDeposit, kSlotBinFill,
     kSlotFftX, kSlotFftY, kSlotFftZ, kSlotFftYInv, kSlotFftXInv, kSlotReduce, kNumSlots
 };
 inline void kt_begin(wt_ctx *c, int slot) {
