@@ -319,7 +319,8 @@ __device__ __forceinline__ void fetch_owner(const SegRec &me, int own, const Run
 }
 
 // Per-wave work state of the deposition pipeline: pairs (lanes) -> row segments -> columns ->
-// candidates -> hits, each stage run on up to 64 entries at a time with every lane busy.
+// candidates -> hits; the candidate and hit stages run on 64 entries at a time with every lane
+// busy.
 template <bool F64>
 struct Wave {
     using Acc = typename AccT<F64>::type;
@@ -328,7 +329,6 @@ struct Wave {
     Acc *acc;
     ColWalk *walks;     // [64]
     RowSeg *segs;       // [128]
-    int32_t *segpre;    // [64]
     HitQ *candq;        // [128]
     HitQ *hitq;         // [128]
     int ox, oy, oz;
@@ -394,41 +394,27 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
     }
 }
 
-// Walks the columns of row segments [segn - cnt, segn); candidates go to the candidate queue.
+// Walks the columns of row segments [segn - cnt, segn), one segment per lane and one column of
+// it per step; candidates go to the candidate queue.
 template <bool F64, bool FAST>
 __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, int cnt) {
     const int lane = W.lane;
-    const int base = W.segn - cnt;
-    RowSeg sgm{0, 0, 0, 0, 0};
-    int32_t len = 0;
+    int32_t len = 0, ib = 0, ic = 0, own = 0;
+    ColWalk w;
     if (lane < cnt) {
-        sgm = W.segs[base + lane];
-        len = sgm.len;
+        const RowSeg r = W.segs[W.segn - cnt + lane];
+        len = r.len;
+        ib = r.bl;
+        ic = r.c;
+        own = r.pair;
+        w = W.walks[own];
     }
-    // inclusive scan of the segment lengths
-    int32_t incl = len;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int32_t y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y;
-    }
-    const int32_t total = __shfl(incl, 63, 64);
-    W.segpre[lane] = incl - len;
-    wave_sync_lds();
-    for (int32_t it = 0; it < total; it += 64) {
-        const int32_t t = it + lane;
-        int32_t nc = 0, ia0 = 0, ib = 0, ic = 0, own = 0, ax = 0;
-        if (t < total) {
-            int k = 0;
-#pragma unroll
-            for (int step = 32; step >= 1; step >>= 1)
-                if (k + step < cnt && W.segpre[k + step] <= t) k += step;
-            const RowSeg r = W.segs[base + k];
-            own = r.pair;
-            ic = r.c;
-            ib = r.bl + (t - W.segpre[k]);
-            const ColWalk w = W.walks[own];
-            ax = w.ax;
+    const int ax = lane < cnt ? (int)w.ax : 0;
+    const int bx = ax == 2 ? 0 : ax + 1;
+    W.segn -= cnt;
+    for (int32_t col = 0; __ballot(col < len); col++, ib++) {
+        int32_t nc = 0, ia0 = 0;
+        if (col < len) {
             nc = column_range(w, ib, ic, &ia0);
             W.ncols++;
         }
@@ -437,7 +423,6 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
             const unsigned long long m = __ballot(c);
             if (c) {
                 const int32_t ia = ia0 + u;
-                const int bx = ax == 2 ? 0 : ax + 1;
                 HitQ q;
                 q.ip = (int16_t)(ax == 0 ? ia : (bx == 0 ? ib : ic));
                 q.jp = (int16_t)(ax == 1 ? ia : (bx == 1 ? ib : ic));
@@ -451,8 +436,6 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
             if (W.candn >= 64) stage_candidates<F64, FAST>(W, me, 64);
         }
     }
-    wave_sync_lds();
-    W.segn -= cnt;
 }
 
 template <bool F64, bool FAST>
@@ -461,8 +444,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     Acc *acc = (Acc *)smem;                                                  // 3 * kCells
     unsigned char *wbase = smem + sizeof(Acc) * 3 * kCells;
-    constexpr size_t kWaveBytes = sizeof(ColWalk) * 64 + sizeof(RowSeg) * 128 + sizeof(int32_t) * 64 +
-                                  sizeof(HitQ) * 256;
+    constexpr size_t kWaveBytes = sizeof(ColWalk) * 64 + sizeof(RowSeg) * 128 + sizeof(HitQ) * 256;
     unsigned long long *s_misc = (unsigned long long *)(wbase + kWaveBytes * kDepWaves);  // brick, cursor
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -475,8 +457,6 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
         p += sizeof(ColWalk) * 64;
         W.segs = (RowSeg *)p;
         p += sizeof(RowSeg) * 128;
-        W.segpre = (int32_t *)p;
-        p += sizeof(int32_t) * 64;
         W.candq = (HitQ *)p;
         W.hitq = W.candq + 128;
     }
@@ -643,7 +623,7 @@ template <bool F64, bool FAST>
 static hipError_t deposit_t(wt_ctx *c, DepArgs &A) {
     hipStream_t st = c->stream;
     using Acc = typename AccT<F64>::type;
-    const size_t wave_bytes = sizeof(ColWalk) * 64 + sizeof(RowSeg) * 128 + sizeof(int32_t) * 64 + sizeof(HitQ) * 256;
+    const size_t wave_bytes = sizeof(ColWalk) * 64 + sizeof(RowSeg) * 128 + sizeof(HitQ) * 256;
     const size_t lds = sizeof(Acc) * 3 * kCells + wave_bytes * kDepWaves + 2 * sizeof(unsigned long long);
     const void *fn = (const void *)k_deposit<F64, FAST>;
     WT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -34,7 +34,7 @@ struct Run {
 };
 
 // The deposition accumulates one brick of kBrickX * kBrickY * kBrickZ cells at a time in LDS.
-constexpr int kBrickX = 16, kBrickY = 16, kBrickZ = 8;
+constexpr int kBrickX = 16, kBrickY = 16, kBrickZ = 16;
 __host__ __device__ constexpr int brick_dim(int axis) { return axis == 0 ? kBrickX : (axis == 1 ? kBrickY : kBrickZ); }
 
 // Device-side view of the tube table and geometry (all pointers are device memory).
