@@ -56,15 +56,17 @@ def algorithmic_cost(slot: str, N: int, seg: dict, dep: dict, p):
         return real + half, "GB/s", "hbm", PEAK_HBM_GBS
     if slot == "fft_x_inv":
         return half + real + 2 * real, "GB/s", "hbm", PEAK_HBM_GBS
-    if slot == "fixed_to_float":
-        return 3 * N ** 3 * (8 + 4), "GB/s", "hbm", PEAK_HBM_GBS
     if slot == "theta":
-        # 1001 samples per section; per sample 108 FP64 arithmetic ops + 2 sqrt + 1 divide,
-        # counted as 120 FLOP
-        return seg["nsec_all"] * (p.ntf_pre + 1) * 120, "TFLOP/s", "valu-fp64", PEAK_FP64_TFLOPS
+        # certified FP32 re-evaluation of the reference's ntf_pre+1 samples per section: per sample
+        # 4 Horner polynomials (29 FMA), |w|^2 (1 mul + 2 FMA), sqrt, reciprocal, 1 mul, the FP64
+        # running sum and 3 bound updates: 71 FLOP
+        return seg["nsec_all"] * (p.ntf_pre + 1) * 71, "TFLOP/s", "mfma", PEAK_FP32_TFLOPS
     if slot == "deposit":
-        # 24 FP64 FLOP per column scanned, 30 per candidate test, 250 per hit
-        return dep["columns"] * 24 + dep["candidates"] * 30 + dep["hits"] * 250, "TFLOP/s", "valu", PEAK_FP32_TFLOPS
+        # FP64 on the vector ALU (its peak equals the published FP64 peak): 16 FLOP per grid column
+        # walked (two plane positions and the interval), 33 per candidate cell (the reference's
+        # three tests), 150 per hit (zeta, spline point and tangent, offset, Gaussian core,
+        # fixed-point conversion)
+        return dep["columns"] * 16 + dep["candidates"] * 33 + dep["hits"] * 150, "TFLOP/s", "mfma", PEAK_FP64_TFLOPS
     return None
 
 
@@ -210,8 +212,11 @@ def main():
         amount, unit, bound, peak = cost
         sec = kernels[dom] * 1e-3
         achieved = amount / sec / (1e9 if unit == "GB/s" else 1e12)
+        resource = {"deposit": "FP64 vector ALU (peak = published FP64 peak)",
+                    "theta": "FP32 vector ALU (packed FP32 disabled)"}.get(dom, "HBM")
         roof = {"kernel": dom, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-                "frac": achieved / peak, "traffic": measured_traffic(wname, dom), "ms": kernels[dom]}
+                "frac": achieved / peak, "traffic": measured_traffic(wname, dom), "ms": kernels[dom],
+                "resource": resource}
     cpu = None
     if not args.no_cpu_baseline:
         t0 = time.time()
