@@ -32,7 +32,6 @@ namespace wt {
 
 constexpr int kDepWaves = 8;
 constexpr int kDepThreads = kDepWaves * 64;
-constexpr double kPlaneMargin = 1e-6;                // cells; FP64 plane positions err far less
 
 struct DepArgs {
     Geo g;
@@ -219,12 +218,22 @@ struct SegRec {
 // Column walk of one (sub-segment, brick image) pair, kept in LDS for the column stage. Columns
 // run along the chord's dominant axis a; on column (b, c) the plane t0 = 0 sits at a-index
 // (relative to lo_a) P0 - B0 db - C0 dc and the plane t1 = 0 at P1 - B1 db - C1 dc (db, dc
-// relative to lo_b, lo_c).
+// relative to lo_b, lo_c). Every cell that can pass the reference's tests lies on a column
+// whose crossing with the plane t0 = 0 is within Rp of ci0 (cells): with (ub, uc) the (b, c)
+// position of ci0, (B0 Db + C0 Dc)^2 + Db^2 + Dc^2 < Rp^2 for Db = db - ub, Dc = dc - uc.
+// Single precision: these only select candidates, with margins m0, m1 of at least three times
+// the worst-case rounding of each plane position over the box; the reference's FP64 tests decide.
 struct ColWalk {
-    double P0, B0, C0, P1, B1, C1;
+    float P0, B0, C0, P1, B1, C1;
+    float ub, uc, Rp2, m0, m1;
     int16_t lo_a, lo_b, lo_c, a0, a1;   // a0..a1: the pair's a-range in the brick image
     int8_t ax, up0, up1, use1;
 };
+
+// Margin of a single-precision plane position P - B db - C dc over db, dc in [0, W].
+__device__ __forceinline__ float plane_margin(double P, double B, double C, double Wb, double Wc) {
+    return (float)(1e-5 + 3.0e-7 * (fabs(P) + fabs(B) * Wb + fabs(C) * Wc));
+}
 
 // A row of columns of one pair: columns b = bl .. bl + len - 1 of row c.
 struct RowSeg {
@@ -237,39 +246,50 @@ struct HitQ {
     int16_t ip, jp, kp, owner;                       // extended indices, owning lane
 };
 
-// b-range of row c whose slab interval can reach a-indices a0..a1 (w's fields, plus the
-// reciprocal slopes): the two plane conditions are linear in db, so each row contributes one
-// interval (a superset, with twice the candidate margin), clipped to b0..b1. False if empty.
-__device__ __forceinline__ bool row_band(const ColWalk &w, double iB0, double iB1, int32_t b0, int32_t b1,
-                                         int32_t ic, int32_t &bl, int32_t &bh) {
-    const double dcn = (double)(ic - w.lo_c);
-    const double A0 = (double)(w.a0 - w.lo_a), A1 = (double)(w.a1 - w.lo_a);
-    const double m2 = 2.0 * kPlaneMargin;
-    double lo = -1e9, hi = 1e9;
+// b-range of row ic whose columns can hold candidates: the slab interval reaches a-indices
+// a0..a1 (each plane condition is linear in db) and the column crosses the plane t0 = 0 inside
+// the ellipse of radius Rp (a quadratic in db). A superset, clipped to b0..b1; false if empty.
+__device__ __forceinline__ bool row_band(const ColWalk &w, int32_t b0, int32_t b1, int32_t ic, int32_t &bl,
+                                         int32_t &bh) {
+    const float dcn = (float)(ic - w.lo_c);
+    const float A0 = (float)(w.a0 - w.lo_a), A1 = (float)(w.a1 - w.lo_a);
+    float lo = -1e9f, hi = 1e9f;
     {   // plane 0: up0 needs B0 db >= r, else B0 db <= r
-        const double q0 = __builtin_fma(-w.C0, dcn, w.P0);
-        const double r = w.up0 ? q0 - m2 - A1 : q0 + m2 - A0;
-        if (w.B0 == 0.0) {
-            if (w.up0 ? (0.0 < r) : (0.0 > r)) return false;
+        const float q0 = __builtin_fmaf(-w.C0, dcn, w.P0);
+        const float m2 = 2.0f * w.m0;
+        const float r = w.up0 ? q0 - m2 - A1 : q0 + m2 - A0;
+        if (w.B0 == 0.0f) {
+            if (w.up0 ? (0.0f < r) : (0.0f > r)) return false;
         } else {
-            const double x = r * iB0;
-            if ((w.B0 > 0.0) == (bool)w.up0) lo = fmax(lo, x); else hi = fmin(hi, x);
+            const float x = r / w.B0;
+            if ((w.B0 > 0.0f) == (bool)w.up0) lo = fmaxf(lo, x); else hi = fminf(hi, x);
         }
     }
     if (w.use1) {   // plane 1: up1 needs B1 db <= r, else B1 db >= r
-        const double q1 = __builtin_fma(-w.C1, dcn, w.P1);
-        const double r = w.up1 ? q1 + m2 - A0 : q1 - m2 - A1;
-        if (w.B1 == 0.0) {
-            if (w.up1 ? (0.0 > r) : (0.0 < r)) return false;
+        const float q1 = __builtin_fmaf(-w.C1, dcn, w.P1);
+        const float m2 = 2.0f * w.m1;
+        const float r = w.up1 ? q1 + m2 - A0 : q1 - m2 - A1;
+        if (w.B1 == 0.0f) {
+            if (w.up1 ? (0.0f > r) : (0.0f < r)) return false;
         } else {
-            const double x = r * iB1;
-            if ((w.B1 > 0.0) != (bool)w.up1) lo = fmax(lo, x); else hi = fmin(hi, x);
+            const float x = r / w.B1;
+            if ((w.B1 > 0.0f) != (bool)w.up1) lo = fmaxf(lo, x); else hi = fminf(hi, x);
         }
     }
-    lo = fmax(lo, -1e9);
-    hi = fmin(hi, 1e9);
+    {   // ellipse: (1 + B0^2) Db^2 + 2 B0 C0 Dc Db + (1 + C0^2) Dc^2 < Rp^2
+        const float Dc = dcn - w.uc;
+        const float eb = __builtin_fmaf(w.B0, w.B0, 1.0f), ebc = w.B0 * w.C0 * Dc;
+        const float cc = __builtin_fmaf(__builtin_fmaf(w.C0, w.C0, 1.0f) * Dc, Dc, -w.Rp2);
+        const float D = __builtin_fmaf(ebc, ebc, -eb * cc);
+        if (!(D >= 0.0f)) return false;
+        const float sq = sqrtf(D), ie = 1.0f / eb;
+        lo = fmaxf(lo, w.ub + (-ebc - sq) * ie - 1e-3f);
+        hi = fminf(hi, w.ub + (-ebc + sq) * ie + 1e-3f);
+    }
+    lo = fmaxf(lo, -1e9f);
+    hi = fminf(hi, 1e9f);
     if (!(lo <= hi)) return false;
-    const int32_t l = w.lo_b + (int32_t)ceil(lo - 1e-6), h = w.lo_b + (int32_t)floor(hi + 1e-6);
+    const int32_t l = w.lo_b + (int32_t)ceilf(lo), h = w.lo_b + (int32_t)floorf(hi);
     bl = l > b0 ? l : b0;
     bh = h < b1 ? h : b1;
     return bl <= bh;
@@ -277,15 +297,15 @@ __device__ __forceinline__ bool row_band(const ColWalk &w, double iB0, double iB
 
 // Candidate a-range of column (ib, ic): returns the count, first index in *ia0.
 __device__ __forceinline__ int32_t column_range(const ColWalk &w, int32_t ib, int32_t ic, int32_t *ia0) {
-    const double db = (double)(ib - w.lo_b), dcn = (double)(ic - w.lo_c);
-    double flo = (double)(w.a0 - w.lo_a), fhi = (double)(w.a1 - w.lo_a);
-    const double p0 = __builtin_fma(-w.B0, db, __builtin_fma(-w.C0, dcn, w.P0));
-    if (w.up0) flo = fmax(flo, ceil(p0 - kPlaneMargin));
-    else fhi = fmin(fhi, floor(p0 + kPlaneMargin));
+    const float db = (float)(ib - w.lo_b), dcn = (float)(ic - w.lo_c);
+    float flo = (float)(w.a0 - w.lo_a), fhi = (float)(w.a1 - w.lo_a);
+    const float p0 = __builtin_fmaf(-w.B0, db, __builtin_fmaf(-w.C0, dcn, w.P0));
+    if (w.up0) flo = fmaxf(flo, ceilf(p0 - w.m0));
+    else fhi = fminf(fhi, floorf(p0 + w.m0));
     if (w.use1) {
-        const double p1 = __builtin_fma(-w.B1, db, __builtin_fma(-w.C1, dcn, w.P1));
-        if (w.up1) fhi = fmin(fhi, floor(p1 + kPlaneMargin));
-        else flo = fmax(flo, ceil(p1 - kPlaneMargin));
+        const float p1 = __builtin_fmaf(-w.B1, db, __builtin_fmaf(-w.C1, dcn, w.P1));
+        if (w.up1) fhi = fminf(fhi, floorf(p1 + w.m1));
+        else flo = fmaxf(flo, ceilf(p1 - w.m1));
     }
     *ia0 = w.lo_a + (int32_t)flo;
     return flo <= fhi ? (int32_t)(fhi - flo) + 1 : 0;
@@ -495,7 +515,6 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
             // ---- stage 0 (lane = pair): the sub-segment to registers, its column walk to LDS ----
             SegRec me;
             ColWalk w;
-            double iB0 = 0.0, iB1 = 0.0;
             int32_t b0 = 0, b1 = -1, c0 = 0, c1 = -1;
             bool have = pi < p1;
             {
@@ -538,28 +557,45 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                     b1 = pick3(chi, bx);
                     c0 = pick3(clo, cx);
                     c1 = pick3(chi, cx);
+                    const double idx = 1.0 / sg.dx;
                     const double t0a = pick3(sg.tci0, ax);
-                    const double ua = (pick3(sg.ci0, ax) - sg.start) / sg.dx + 1.0 - w.lo_a;
-                    const double ub = (pick3(sg.ci0, bx) - sg.start) / sg.dx + 1.0 - w.lo_b;
-                    const double uc = (pick3(sg.ci0, cx) - sg.start) / sg.dx + 1.0 - w.lo_c;
-                    w.B0 = pick3(sg.tci0, bx) / t0a;
-                    w.C0 = pick3(sg.tci0, cx) / t0a;
-                    w.P0 = (ua + w.B0 * ub) + w.C0 * uc;
+                    const double ua = (pick3(sg.ci0, ax) - sg.start) * idx + 1.0 - w.lo_a;
+                    const double ub = (pick3(sg.ci0, bx) - sg.start) * idx + 1.0 - w.lo_b;
+                    const double uc = (pick3(sg.ci0, cx) - sg.start) * idx + 1.0 - w.lo_c;
+                    const double B0 = pick3(sg.tci0, bx) / t0a, C0 = pick3(sg.tci0, cx) / t0a;
+                    const double Wb = pick3(sg.hi, bx) - pick3(sg.lo, bx), Wc = pick3(sg.hi, cx) - pick3(sg.lo, cx);
+                    w.B0 = (float)B0;
+                    w.C0 = (float)C0;
+                    w.P0 = (float)((ua + B0 * ub) + C0 * uc);
+                    w.m0 = plane_margin((ua + B0 * ub) + C0 * uc, B0, C0, Wb, Wc);
+                    w.m1 = 0.0f;
+                    w.ub = (float)ub;
+                    w.uc = (float)uc;
                     w.up0 = t0a > 0.0;
-                    iB0 = w.B0 != 0.0 ? 1.0 / w.B0 : 0.0;
                     const double t1a = pick3(sg.tci1, ax);
                     w.use1 = fabs(t1a) >= 0.05;
-                    w.P1 = w.B1 = w.C1 = 0.0;
+                    w.P1 = w.B1 = w.C1 = 0.0f;
                     w.up1 = 1;
+                    // cells between the planes within R of the chord line lie at most
+                    // (ndci0 + R tan(delta)) from the plane t0 = 0 (delta: angle between the plane
+                    // normals); their columns cross that plane within Rp of ci0
+                    const double cd = (sg.tci0[0] * sg.tci1[0] + sg.tci0[1] * sg.tci1[1]) + sg.tci0[2] * sg.tci1[2];
+                    double Rp = 1e30;
+                    if (cd > 0.2) {
+                        const double tand = sqrt(fmax(1.0 - cd * cd, 0.0)) / cd;
+                        Rp = (sg.R + (sg.ndci0 + sg.R * tand) * sqrt(fmax(1.0 - t0a * t0a, 0.0)) / fabs(t0a)) * idx;
+                    }
+                    w.Rp2 = (float)fmin(Rp * Rp * (1.0 + 1e-5) + 1e-3, 1e30);
                     if (w.use1) {
-                        const double va = (pick3(sg.ci1, ax) - sg.start) / sg.dx + 1.0 - w.lo_a;
-                        const double vb = (pick3(sg.ci1, bx) - sg.start) / sg.dx + 1.0 - w.lo_b;
-                        const double vc = (pick3(sg.ci1, cx) - sg.start) / sg.dx + 1.0 - w.lo_c;
-                        w.B1 = pick3(sg.tci1, bx) / t1a;
-                        w.C1 = pick3(sg.tci1, cx) / t1a;
-                        w.P1 = (va + w.B1 * vb) + w.C1 * vc;
+                        const double va = (pick3(sg.ci1, ax) - sg.start) * idx + 1.0 - w.lo_a;
+                        const double vb = (pick3(sg.ci1, bx) - sg.start) * idx + 1.0 - w.lo_b;
+                        const double vc = (pick3(sg.ci1, cx) - sg.start) * idx + 1.0 - w.lo_c;
+                        const double B1 = pick3(sg.tci1, bx) / t1a, C1 = pick3(sg.tci1, cx) / t1a;
+                        w.B1 = (float)B1;
+                        w.C1 = (float)C1;
+                        w.P1 = (float)((va + B1 * vb) + C1 * vc);
+                        w.m1 = plane_margin((va + B1 * vb) + C1 * vc, B1, C1, Wb, Wc);
                         w.up1 = t1a > 0.0;
-                        iB1 = w.B1 != 0.0 ? 1.0 / w.B1 : 0.0;
                     }
                     W.walks[lane] = w;
                 } else {
@@ -573,7 +609,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
             // ---- stage 1 (lane = pair): one row of the pair per step, non-empty bands queued ----
             for (int32_t ic = c0; __ballot(have && ic <= c1); ic++) {
                 int32_t bl = 0, bh = -1;
-                const bool sgok = have && ic <= c1 && row_band(w, iB0, iB1, b0, b1, ic, bl, bh);
+                const bool sgok = have && ic <= c1 && row_band(w, b0, b1, ic, bl, bh);
                 const unsigned long long m = __ballot(sgok);
                 if (sgok) W.segs[W.segn + __popcll(m & lanemask_lt())] =
                     RowSeg{(int16_t)ic, (int16_t)bl, (uint8_t)(bh - bl + 1), (uint8_t)lane, 0};

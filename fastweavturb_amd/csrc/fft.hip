@@ -377,14 +377,26 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl, int W, int nchunk,
 
 // Reduce the per-block partial rows (columns are independent), then derive up and the
 // normalised spectra.
-__global__ void k_reduce_columns(const double *__restrict__ partials, int64_t rows, int cols,
-                                 double *__restrict__ out) {
-    int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= cols) return;
+// Column sums of the per-block partial rows, in a fixed order: thread (x, y) of a 64 x 16 block
+// sums rows y, y + 16, ... of column 64 blockIdx.x + x, then the 16 partial sums are added in
+// order of y.
+__global__ __launch_bounds__(1024) void k_reduce_columns(const double *__restrict__ partials, int64_t rows, int cols,
+                                                         double *__restrict__ out) {
+    __shared__ double part[16][64];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int col = blockIdx.x * 64 + tx;
     partials += (int64_t)blockIdx.y * rows * cols;   // rows of instance blockIdx.y
     double s = 0.0;
-    for (int64_t r = 0; r < rows; r++) s += partials[r * cols + col];
-    out[(int64_t)blockIdx.y * cols + col] = s;
+    if (col < cols)
+        for (int64_t r = ty; r < rows; r += 16) s += partials[r * cols + col];
+    part[ty][tx] = s;
+    __syncthreads();
+    if (ty == 0 && col < cols) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) t += part[k][tx];
+        out[(int64_t)blockIdx.y * cols + col] = t;
+    }
 }
 
 __global__ void k_finalize(const double *__restrict__ sums, int nek, int mode, DeviceScalars *__restrict__ sc,
@@ -524,7 +536,7 @@ static hipError_t spectral_group(wt_ctx *c, int mode, int pos, int K, bool timed
     if (timed) kt_end(c, kSlotFftZ);
     WT_HIP(hipGetLastError());
     if (timed) kt_begin(c, kSlotReduce);
-    hipLaunchKernelGGL(k_reduce_columns, dim3((unsigned)((cols + 63) / 64), (unsigned)K), dim3(64), 0, st,
+    hipLaunchKernelGGL(k_reduce_columns, dim3((unsigned)((cols + 63) / 64), (unsigned)K), dim3(1024), 0, st,
                        (const double *)c->partials, zrows, cols, colsum);
     hipLaunchKernelGGL(k_finalize, dim3((unsigned)K), dim3(256), 0, st, (const double *)colsum, nek, mode,
                        c->scal + pos, c->E_w + I0.E_off, c->E_u + I0.E_off);

@@ -41,15 +41,26 @@ __global__ void k_chords(Geo g) {
 
 // tube.f90:418-437 sequential cumulative sum, one thread per tube (kept sequential so the
 // rounding matches the reference), then Len_Disc and length.
+// The chords are read 16 at a time into registers so that the serial chain waits on one memory
+// round trip per 16 additions.
 __global__ void k_cumsum(Geo g, double *__restrict__ tot_out) {
     int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= g.nline) return;
     int64_t b = g.pt_base[l], e = g.pt_base[l + 1];
     double acc = 0.0;
     g.cze[b] = 0.0;
-    for (int64_t i = b + 1; i < e; i++) {
-        acc = acc + g.chord[i];
-        g.cze[i] = acc;
+    constexpr int kChunk = 16;
+    for (int64_t i0 = b + 1; i0 < e; i0 += kChunk) {
+        double buf[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; k++) buf[k] = i0 + k < e ? g.chord[i0 + k] : 0.0;
+#pragma unroll
+        for (int k = 0; k < kChunk; k++) {
+            if (i0 + k < e) {
+                acc = acc + buf[k];
+                g.cze[i0 + k] = acc;
+            }
+        }
     }
     double tot = acc + g.chord[b];
     double LD = tot / kPi2;
