@@ -25,6 +25,7 @@ SOURCES = {
     "scan.hip": [],
     "fft.hip": [],
     "stats.hip": [],
+    "mlp.hip": [],
     "wt_api.cpp": ["-x", "hip"],
 }
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",

@@ -58,7 +58,7 @@ __device__ void hit_math_f64(const Geo &g, const SubSeg &sg, int32_t ip, int32_t
     const double LD = g.Len_Disc[l], len = g.length[l], Gamma = g.gamma[l];
     const double smin = g.sigma[l], smax = r.sigma_ratio * smin;
     const int nT = g.n_T[l];
-    const double *coe = g.coe + 18 * sg.sec;
+    const double *coe = g.coe + 18 * (int64_t)sg.sec;
     const double sfirst = g.czl[sg.sec];
     const double dzeta = r.dzeta_sup < g.dtf[sg.sec] ? r.dzeta_sup : g.dtf[sg.sec];
     double X[3] = {mesh(ip, sg), mesh(jp, sg), mesh(kp, sg)};
@@ -135,7 +135,7 @@ __device__ void hit_math_f64(const Geo &g, const SubSeg &sg, int32_t ip, int32_t
 __device__ void hit_math_f32(const Geo &g, const SubSeg &sg, int32_t ip, int32_t jp, int32_t kp, double *v) {
     const int l = sg.tube;
     const double LD = g.Len_Disc[l];
-    const double *coe = g.coe + 18 * sg.sec;
+    const double *coe = g.coe + 18 * (int64_t)sg.sec;
     double X[3] = {mesh(ip, sg), mesh(jp, sg), mesh(kp, sg)};
     double a[3] = {X[0] - sg.ci0[0], X[1] - sg.ci0[1], X[2] - sg.ci0[2]};
     double t0 = (a[0] * sg.tci0[0] + a[1] * sg.tci0[1]) + a[2] * sg.tci0[2];

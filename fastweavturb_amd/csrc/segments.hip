@@ -354,7 +354,7 @@ __global__ void k_subpoints(Geo g) {
     double sv = czl + g.dtf[src] * (double)jj;
     double coe[18];
 #pragma unroll
-    for (int k = 0; k < 18; k++) coe[k] = g.coe[18 * src + k];
+    for (int k = 0; k < 18; k++) coe[k] = g.coe[18 * (int64_t)src + k];
     double c3[3];
     spline_pos((sv - czl) * LD, coe, c3);
 #pragma unroll
@@ -442,6 +442,8 @@ hipError_t launch_segments(wt_ctx *c) {
     WT_HIP(hipStreamSynchronize(st));
     if (tmp) (void)hipFree(tmp);
     if (herr) return hipErrorInvalidValue;
+    // sub-point indices are 32-bit in the brick pair lists
+    if (nsub >= (int64_t)UINT32_MAX) return hipErrorInvalidImage;
     g.nsub_all = nsub;
     WT_HIP(dalloc(c, c->seg_allocs, &g.sub_sec, nsub));
     WT_HIP(dalloc(c, c->seg_allocs, &g.ci, 3 * nsub));

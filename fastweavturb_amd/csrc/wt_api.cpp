@@ -333,6 +333,7 @@ static int upload_tubes(wt_ctx *c) {
         sec[l + 1] = sec[l] + npt[l] + 1;
     }
     const int64_t S = sec[nline];
+    if (S >= INT32_MAX) return set_err(c, WT_ERANGE, "%lld sections (points + one per tube) exceed 2^31 - 1", (long long)S);
     size_t cap = mem_cap_bytes();
     size_t need = (size_t)P * 8 * 13 + (size_t)S * 8 * 24;
     if (cap && field_bytes(c->field_elems, c->spec_elems, c->precision) + need > cap)
@@ -446,6 +447,8 @@ int wt_build_segments(wt_ctx *c, wt_seg_stats *out) {
     hipError_t e = wt::launch_segments(c);
     if (e == hipErrorInvalidValue)
         return set_err(c, WT_ERANGE, "turning angle not finite (degenerate spline: repeated or cusped points)");
+    if (e == hipErrorInvalidImage)
+        return set_err(c, WT_ERANGE, "more than 2^32 - 2 sub-points in one context; split the batch");
     if (e != hipSuccess) return hip_err(c, e, "segments");
     HIPCALL(c, wt::launch_seg_stats(c, &c->seg_stats), "segment statistics");
     c->seg_stats.npoint_all = c->geo.npoint_all;

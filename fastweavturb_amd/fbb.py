@@ -164,3 +164,56 @@ def config_small(N: int = 32, seed: int = 7, ntubes=(1, 2), npts=(64, 48), radii
     """Tiny multi-level case for parity tests (seconds on the CPU oracle)."""
     levels = [Level(r, n, (p, p)) for r, n, p in zip(radii, ntubes, npts)]
     return woven_config(N, levels, seed, sigma_ratio=sigma_ratio, ntf_pre=ntf_pre, **kw)
+
+
+@dataclasses.dataclass
+class LevelSpec:
+    """One scale level of an ensemble or inverse-design instance: every tube of the level has
+    the same point count, so the level is drawn as one batch."""
+    radius: float          # core radius r; sigma = r / sqrt(2)
+    ntubes: int
+    npoints: int
+    hurst: float
+    gamma: float           # circulation of the level before the per-tube jitter
+
+
+def woven_levels(N: int, levels: list, seed: int, gamma_jitter: float = 0.2, wiggle: float = 0.35,
+                 Rtube_ratio: float = 3.0, ntf_times: int = 50, ntf_pre: int = 1000,
+                 dzeta_sup: float = 1e-4, chunk: int = 1024) -> tuple[RunParams, Tubes]:
+    """Tube table of one instance from LevelSpecs, with the construction of `woven_config`
+    (fBb bridge, step-rms normalisation, winding drift, shift and wrap) but one random stream per
+    level (seeded by (seed, level)) and the fGn of a level's tubes drawn in batches of `chunk`.
+    Subsets of tubes are therefore not reproducible on their own, unlike `woven_config`."""
+    npoint, xs, sig, gam = [], [], [], []
+    for li, lev in enumerate(levels):
+        rng = np.random.default_rng([int(seed), li])
+        axis = rng.standard_normal(3)
+        axis /= np.linalg.norm(axis)
+        w = lattice_winding(axis).astype(np.float64)
+        n = max(8, int(lev.npoints))
+        j = np.arange(1, n + 1, dtype=np.float64) / n
+        f = 0.5 * (j ** (2 * lev.hurst) + 1.0 - (1.0 - j) ** (2 * lev.hurst))
+        t = np.arange(n, dtype=np.float64) / n
+        dB = wiggle * 2.0 * np.pi * float(np.linalg.norm(w)) / n
+        for c0 in range(0, lev.ntubes, chunk):
+            cnt = min(chunk, lev.ntubes - c0)
+            g = fgn_davies_harte(n - 1, lev.hurst, rng, 3 * cnt).reshape(cnt, 3, n - 1)
+            B = np.concatenate([np.zeros((cnt, 3, 1)), np.cumsum(g, axis=2)], axis=2)
+            B -= f[None, None, :] * B[:, :, -1:]
+            rms = np.sum(np.diff(B, axis=2) ** 2, axis=(1, 2)) / (n - 1)
+            B *= (dB / np.sqrt(np.maximum(rms, 1e-300)))[:, None, None]
+            x0 = (rng.random((cnt, 3)) - 0.5) * 2.0 * np.pi
+            X = x0[:, :, None] + 2.0 * np.pi * w[None, :, None] * t[None, None, :] + B
+            X = np.mod(X + np.pi, 2.0 * np.pi) - np.pi
+            xs.append(X.transpose(1, 0, 2).reshape(3, cnt * n))
+            npoint.extend([n] * cnt)
+            sig.extend([lev.radius / math.sqrt(2.0)] * cnt)
+            gam.append(lev.gamma * rng.uniform(1 - gamma_jitter, 1 + gamma_jitter, cnt))
+    X = np.concatenate(xs, axis=1)
+    nt = len(npoint)
+    tb = Tubes(np.array(npoint, dtype=np.int32), X[0].copy(), X[1].copy(), X[2].copy(),
+               np.array(sig), np.concatenate(gam), np.ones(nt, dtype=np.int32), np.zeros(nt, dtype=np.int32))
+    p = RunParams(N=N, n_d=N, line_s=1, line_e=nt, sigma_ratio=1.0, eta_t=0.0, ntf_times=ntf_times,
+                  ntf_pre=ntf_pre, Rtube_ratio=Rtube_ratio, dzeta_sup=dzeta_sup)
+    tb.validate()
+    return p, tb

@@ -1,0 +1,394 @@
+"""Inverse design of woven fields: a learned surrogate that maps target statistics to per-level
+tube parameters, and the plain parameter sweep it is measured against.
+
+The reference sets its level parameters by closed-form scalings (FBB_with_parameters.m:25-41,
+185-193: radius ratio r_s, density, r_Gamma from r_E, Hurst index H) and has no inverse design.
+Here:
+
+* Design (24 numbers, 6 levels x 4): per level l
+    a_l = log2(Gamma_l / Gamma_0)   (a_0 = 0: the normalisation u' = 1 makes a common
+                                     circulation factor unobservable)
+    b_l = log2(density_l)           tubes_l = max(1, round(2^b_l * 4 * 4^l))
+    q_l = radius ratio              r_l = r_{l-1} / q_l, r_{-1} = 1.6
+    H_l = Hurst index of the level's fBb centerlines
+  points per tube n_l = clamp(round(51.2 / r_l), 32, 1024); sigma = r / sqrt(2).
+* Statistics (48 numbers) of the normalised velocity: 32 log shells of E(k) (Gaussian-weighted
+  in ln k around 32 centres log-spaced over 1 <= k <= 16) and the flatness F4 = S4/S2^2 and
+  F6 = S6/S2^3 of x-increments of u_x at the separations m * 2pi/32, m = 1,2,3,4,6,8,12,16.
+  Both are defined in box units, so instances of every grid size N in {64, 96, 128, 192, 256}
+  (and the 512^3 verification grid) describe the same scales.
+* Pairs (statistics -> design) come from ensembles of uneven instances generated by the batched
+  library path (one context per ensemble; geometry, binning and deposition as single launches).
+* The network (48 -> 3 x 256 GELU -> 24, BF16 MFMA kernels of the library) is trained on
+  standardised features and designs, with the slots of absent levels masked out of the loss.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+from concurrent.futures import ProcessPoolExecutor
+
+import numpy as np
+
+from . import fbb
+from .woven_io import RunParams, Tubes
+
+NLEV = 6
+NPAR = 4
+NSHELL = 32
+K_LO, K_HI = 1.0, 16.0
+FLAT_M = (1, 2, 3, 4, 6, 8, 12, 16)
+NFEAT = NSHELL + 2 * len(FLAT_M)
+GRIDS = (64, 96, 128, 192, 256)
+R_TOP = 1.6
+SURROGATE_SIZES = [NFEAT, 256, 256, 256, NLEV * NPAR]
+
+# parameter ranges of the designs (the box the sweep and the draws live in)
+RANGE = {"b": (-1.0, 1.0), "q": (1.6, 2.4), "H": (0.5, 0.9)}
+
+
+@dataclasses.dataclass
+class Design:
+    a: np.ndarray          # [L] log2 Gamma_l / Gamma_0, a[0] = 0
+    b: np.ndarray          # [L] log2 density
+    q: np.ndarray          # [L] radius ratio
+    H: np.ndarray          # [L] Hurst index
+
+    @property
+    def nlev(self) -> int:
+        return len(self.q)
+
+    def radii(self) -> np.ndarray:
+        return R_TOP / np.cumprod(self.q)
+
+    def levels(self) -> list:
+        r = self.radii()
+        out = []
+        for l in range(self.nlev):
+            nt = max(1, int(round(2.0 ** self.b[l] * 4 * 4 ** l)))
+            npts = int(min(1024, max(32, round(51.2 / r[l]))))
+            gam = r[0] ** (4.0 / 3.0) * 2.0 ** self.a[l]
+            out.append(fbb.LevelSpec(float(r[l]), nt, npts, float(self.H[l]), float(gam)))
+        return out
+
+    def vector(self) -> tuple[np.ndarray, np.ndarray]:
+        """(24-vector, mask): per level (a, b, q, H); absent levels are zero with mask 0."""
+        v = np.zeros((NLEV, NPAR))
+        m = np.zeros((NLEV, NPAR))
+        L = self.nlev
+        v[:L] = np.stack([self.a, self.b, self.q, self.H], axis=1)
+        m[:L] = 1.0
+        m[0, 0] = 0.0      # a_0 is 0 by definition
+        return v.ravel(), m.ravel()
+
+    @staticmethod
+    def from_vector(v, nlev: int = NLEV) -> "Design":
+        v = np.asarray(v, dtype=np.float64).reshape(NLEV, NPAR)[:nlev]
+        a = v[:, 0].copy()
+        a[0] = 0.0
+        return Design(a, np.clip(v[:, 1], *RANGE["b"]), np.clip(v[:, 2], *RANGE["q"]), np.clip(v[:, 3], *RANGE["H"]))
+
+
+def max_levels(N: int, q) -> int:
+    """Levels whose core radius is at least one grid spacing 2pi/N (at least 3)."""
+    r = R_TOP / np.cumprod(np.asarray(q, dtype=np.float64))
+    return max(3, int(np.sum(r >= 2.0 * np.pi / N)))
+
+
+def draw_design(rng: np.random.Generator, N: int, nlev: int | None = None) -> Design:
+    """Ensemble draw: instance-wide Hurst index U[0.5, 0.9], density U[0.5, 2] and circulation
+    exponent U[1.1, 1.5] (Gamma_l ~ r_l^gamma), each level jittered around them; radius ratios
+    U[1.6, 2.4]; 3..(levels resolved by N) levels when nlev is not given."""
+    q = rng.uniform(*RANGE["q"], NLEV)
+    if nlev is None:
+        nlev = int(rng.integers(3, min(NLEV, max_levels(N, q)) + 1))
+    q = q[:nlev]
+    H0, d0, g0 = rng.uniform(0.5, 0.9), rng.uniform(0.5, 2.0), rng.uniform(1.1, 1.5)
+    r = R_TOP / np.cumprod(q)
+    a = g0 * np.log2(r / r[0]) + rng.uniform(-0.25, 0.25, nlev)
+    a[0] = 0.0
+    b = np.clip(math.log2(d0) + rng.uniform(-0.3, 0.3, nlev), *RANGE["b"])
+    H = np.clip(H0 + rng.uniform(-0.08, 0.08, nlev), *RANGE["H"])
+    return Design(a, b, q, H)
+
+
+def instance_config(design: Design, N: int, seed: int, ntf_pre: int = 1000):
+    return fbb.woven_levels(N, design.levels(), seed, ntf_pre=ntf_pre)
+
+
+# ---- statistics -----------------------------------------------------------------------------
+def shell_weights(nek: int) -> np.ndarray:
+    """(32, nek) Gaussian weights in ln k of the integer shells around the log-spaced centres."""
+    j = np.arange(NSHELL)
+    lc = math.log(K_LO) + (math.log(K_HI) - math.log(K_LO)) * j / (NSHELL - 1)
+    s = (math.log(K_HI) - math.log(K_LO)) / (NSHELL - 1)
+    lk = np.log(np.arange(1, nek + 1, dtype=np.float64))
+    w = np.exp(-0.5 * ((lk[None, :] - lc[:, None]) / s) ** 2)
+    return w / w.sum(axis=1, keepdims=True)
+
+
+def log_shells(E) -> np.ndarray:
+    """32 log shells of an integer-shell spectrum E[i-1], i = 1..nek."""
+    E = np.asarray(E, dtype=np.float64)
+    return shell_weights(len(E)) @ E
+
+
+def flat_separations(N: int) -> np.ndarray:
+    if N % 32:
+        raise ValueError("flatness separations need N divisible by 32")
+    return np.array([m * N // 32 for m in FLAT_M], dtype=np.int32)
+
+
+def features(E, S) -> np.ndarray:
+    """48 statistics from the integer-shell spectrum and S[p-2][m] (p = 2..6) of x-increments."""
+    S = np.asarray(S, dtype=np.float64)
+    F4 = S[2] / S[0] ** 2
+    F6 = S[4] / S[0] ** 3
+    return np.concatenate([np.log10(np.maximum(log_shells(E), 1e-300)), np.log10(F4), np.log10(F6)])
+
+
+def measure(g, inst: int | None = None) -> np.ndarray:
+    """Statistics of instance `inst` of a generated WovenGPU context."""
+    if inst is not None:
+        g.select(inst)
+    E = g.spectrum(1)
+    S = g.structure_functions(1, 2, 6, flat_separations(g.N))
+    return features(E, S)
+
+
+# ---- fixed-seed centerline bank ---------------------------------------------------------------
+HB = (0.5, 0.6, 0.7, 0.8, 0.9)
+
+
+def level_points(l: int) -> int:
+    """Points per tube of level l (64 * 2^l, at most 1024); independent of the radius so that a
+    design's geometry changes continuously with its radius ratios."""
+    return int(min(1024, 64 * 2 ** l))
+
+
+def level_max_tubes(l: int) -> int:
+    return 2 * 4 ** (l + 1)
+
+
+class CenterlineBank:
+    """Unit fBb shapes of one tube seed. Per level l: the tubes of the largest density
+    (2 * 4^(l+1)), each a closed fBb (bridge as FBB_with_parameters.m:92-102) with unit step rms,
+    drawn for every Hurst index in HB from the same Gaussian numbers; the level's winding vector;
+    per tube a shift and a circulation jitter U(0.8, 1.2). A design takes the first tubes of each
+    level, interpolates linearly between the two nearest Hurst indices (then restores unit step
+    rms), scales the wiggle to 0.35 of the drift step (as `fbb.woven_config`), adds the drift and
+    the shift and wraps into [-pi, pi). Equal designs give identical fields; nearby designs give
+    nearby fields, apart from tubes added or removed by the density."""
+
+    def __init__(self, seed: int, nlev: int = NLEV, wiggle: float = 0.35, arrays: dict | None = None):
+        self.seed, self.nlev, self.wiggle = int(seed), nlev, wiggle
+        if arrays is not None:
+            self.a = arrays
+            return
+        self.a = {}
+        for l in range(nlev):
+            n, cmax = level_points(l), level_max_tubes(l)
+            r0 = np.random.default_rng([self.seed, l, 0])
+            axis = r0.standard_normal(3)
+            axis /= np.linalg.norm(axis)
+            self.a[f"w{l}"] = fbb.lattice_winding(axis).astype(np.float64)
+            self.a[f"x0_{l}"] = (r0.random((cmax, 3)) - 0.5) * 2.0 * np.pi
+            self.a[f"u{l}"] = r0.uniform(0.8, 1.2, cmax)
+            j = np.arange(1, n + 1, dtype=np.float64) / n
+            shapes = np.zeros((len(HB), cmax, 3, n), dtype=np.float32)
+            for h, H in enumerate(HB):
+                rng = np.random.default_rng([self.seed, l, 1])        # same numbers for every H
+                f = 0.5 * (j ** (2 * H) + 1.0 - (1.0 - j) ** (2 * H))
+                for c0 in range(0, cmax, 1024):
+                    cnt = min(1024, cmax - c0)
+                    g = fbb.fgn_davies_harte(n - 1, H, rng, 3 * cnt).reshape(cnt, 3, n - 1)
+                    B = np.concatenate([np.zeros((cnt, 3, 1)), np.cumsum(g, axis=2)], axis=2)
+                    B -= f[None, None, :] * B[:, :, -1:]
+                    rms = np.sqrt(np.sum(np.diff(B, axis=2) ** 2, axis=(1, 2)) / (n - 1))
+                    shapes[h, c0:c0 + cnt] = B / rms[:, None, None]
+            self.a[f"B{l}"] = shapes
+
+    def save(self, path: str):
+        np.savez(path, seed=self.seed, nlev=self.nlev, wiggle=self.wiggle, **self.a)
+
+    @staticmethod
+    def load(path: str, mmap: bool = True) -> "CenterlineBank":
+        z = np.load(path, mmap_mode="r" if mmap else None)
+        arrays = {k: z[k] for k in z.files if k not in ("seed", "nlev", "wiggle")}
+        return CenterlineBank(int(z["seed"]), int(z["nlev"]), float(z["wiggle"]), arrays)
+
+    def tubes(self, design: Design):
+        """Tube table of a design (sigma = r / sqrt(2), Gamma_l = r_0^(4/3) 2^a_l times the tube's
+        jitter, n_T = 1, no filter)."""
+        if design.nlev > self.nlev:
+            raise ValueError("design has more levels than the bank")
+        r = design.radii()
+        npoint, xs, sig, gam = [], [], [], []
+        for l in range(design.nlev):
+            n = level_points(l)
+            cnt = min(level_max_tubes(l), max(1, int(round(2.0 ** design.b[l] * 4 * 4 ** l))))
+            H = float(np.clip(design.H[l], HB[0], HB[-1]))
+            h = min(len(HB) - 2, int((H - HB[0]) / 0.1))
+            t = (H - HB[h]) / 0.1
+            Bs = self.a[f"B{l}"]
+            B = (1.0 - t) * Bs[h, :cnt].astype(np.float64) + t * Bs[h + 1, :cnt].astype(np.float64)
+            rms = np.sqrt(np.sum(np.diff(B, axis=2) ** 2, axis=(1, 2)) / (n - 1))
+            w = self.a[f"w{l}"]
+            dB = self.wiggle * 2.0 * np.pi * float(np.linalg.norm(w)) / n
+            tt = np.arange(n, dtype=np.float64) / n
+            X = (np.asarray(self.a[f"x0_{l}"][:cnt])[:, :, None] + 2.0 * np.pi * w[None, :, None] * tt[None, None, :]
+                 + B * (dB / rms)[:, None, None])
+            X = np.mod(X + np.pi, 2.0 * np.pi) - np.pi
+            xs.append(X.transpose(1, 0, 2).reshape(3, cnt * n))
+            npoint.extend([n] * cnt)
+            sig.extend([r[l] / math.sqrt(2.0)] * cnt)
+            gam.append(r[0] ** (4.0 / 3.0) * 2.0 ** design.a[l] * np.asarray(self.a[f"u{l}"][:cnt]))
+        X = np.concatenate(xs, axis=1)
+        nt = len(npoint)
+        return Tubes(np.array(npoint, dtype=np.int32), X[0].copy(), X[1].copy(), X[2].copy(), np.array(sig),
+                     np.concatenate(gam), np.ones(nt, dtype=np.int32), np.zeros(nt, dtype=np.int32))
+
+    def config(self, design: Design, N: int, ntf_pre: int = 1000):
+        t = self.tubes(design)
+        p = RunParams(N=N, n_d=N, line_s=1, line_e=t.nline, sigma_ratio=1.0, eta_t=0.0, ntf_times=50,
+                      ntf_pre=ntf_pre, Rtube_ratio=3.0, dzeta_sup=1e-4)
+        return p, t
+
+
+def family_design(gamma: float, density: float, hurst: float, nlev: int = NLEV) -> Design:
+    """The plain method's level law (the 512^3 workload's levels): r_l = 0.8 * 2^-l, one density,
+    one Hurst index, Gamma_l ~ r_l^gamma."""
+    q = np.full(nlev, 2.0)
+    r = R_TOP / np.cumprod(q)
+    a = gamma * np.log2(r / r[0])
+    return Design(a, np.full(nlev, math.log2(density)), q, np.full(nlev, hurst))
+
+
+def draw_inverse_design(rng: np.random.Generator, jitter: bool) -> Design:
+    """Prior of the inverse-design pairs: the plain method's family with gamma U[1.1, 1.5],
+    density U[0.5, 2], H U[0.5, 0.9]; with `jitter`, each level's (a, b, q, H) perturbed too."""
+    d = family_design(rng.uniform(1.1, 1.5), rng.uniform(0.5, 2.0), rng.uniform(0.5, 0.9))
+    if jitter:
+        L = d.nlev
+        d.a = d.a + np.concatenate([[0.0], rng.uniform(-0.3, 0.3, L - 1)])
+        d.b = np.clip(d.b + rng.uniform(-0.3, 0.3, L), *RANGE["b"])
+        d.q = np.clip(d.q + rng.uniform(-0.15, 0.15, L), *RANGE["q"])
+        d.H = np.clip(d.H + rng.uniform(-0.08, 0.08, L), *RANGE["H"])
+    return d
+
+
+# ---- pair generation ---------------------------------------------------------------------------
+_WORKER_BANK = None
+
+
+def _init_worker(bank_path):
+    global _WORKER_BANK
+    _WORKER_BANK = CenterlineBank.load(bank_path) if bank_path else None
+
+
+def _make_instance(args):
+    design, N, seed = args
+    if _WORKER_BANK is not None:
+        return _WORKER_BANK.config(design, N)
+    return instance_config(design, N, seed)
+
+
+def generate_pairs(jobs, ensemble: int = 64, workers: int = 8, log=None, bank_path: str | None = None):
+    """(statistics, design, mask, N) for each job (design, N, tube seed), from ensembles of
+    `ensemble` instances, each ensemble one batched context on the GPU. Tube tables are built on
+    `workers` host processes (spawned, so none inherits a GPU context) while the GPU runs the
+    previous ensemble; with `bank_path` they come from that centerline bank instead of fresh
+    draws of the job's seed."""
+    import multiprocessing as mp
+    from .api import WovenGPU
+    n = len(jobs)
+    X = np.zeros((n, NFEAT), dtype=np.float32)
+    Y = np.zeros((n, NLEV * NPAR), dtype=np.float32)
+    M = np.zeros((n, NLEV * NPAR), dtype=np.float32)
+    Ns = np.zeros(n, dtype=np.int32)
+    t0 = time.time()
+    ex = None
+    if workers > 1:
+        ex = ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn"),
+                                 initializer=_init_worker, initargs=(bank_path,))
+    else:
+        _init_worker(bank_path)
+    try:
+        def submit(lo):
+            chunk = jobs[lo:lo + ensemble]
+            return [ex.submit(_make_instance, j) for j in chunk] if ex else [_make_instance(j) for j in chunk]
+        pending = submit(0)
+        for lo in range(0, n, ensemble):
+            cfgs = [f.result() for f in pending] if ex else pending
+            pending = submit(lo + ensemble) if lo + ensemble < n else []
+            g = WovenGPU(precision=32, sizes=[p.N for p, _ in cfgs])
+            for k, (p, t) in enumerate(cfgs):
+                g.select(k)
+                g.set_params(p)
+                g.set_tubes(t)
+            g.generate()
+            for k in range(len(cfgs)):
+                i = lo + k
+                X[i] = measure(g, k)
+                Y[i], M[i] = jobs[i][0].vector()
+                Ns[i] = jobs[i][1]
+            g.close()
+            if log:
+                log(f"[pairs] {lo + len(cfgs)}/{n} in {time.time() - t0:.1f}s")
+    finally:
+        if ex:
+            ex.shutdown(cancel_futures=True)
+    return X, Y, M, Ns
+
+
+def ensemble_jobs(n: int, seed: int = 5000, grids=GRIDS):
+    """Uneven ensemble instances: grid size from `grids`, a design from `draw_design`, tube seed
+    seed + i."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        N = int(rng.choice(grids))
+        out.append((draw_design(rng, N), N, seed + i))
+    return out
+
+
+# ---- the network ---------------------------------------------------------------------------------
+class Surrogate:
+    """Standardisation + the library's MLP. Designs are predicted as 24-vectors."""
+
+    def __init__(self, seed: int = 0):
+        from .mlp import MLP
+        self.net = MLP(SURROGATE_SIZES, seed=seed)
+        self.xm = self.xs = self.ym = self.ys = None
+
+    def fit(self, X, Y, M, epochs: int = 200, batch: int = 256, lr: float = 2e-3, lr_final: float = 2e-5,
+            seed: int = 0) -> np.ndarray:
+        X = np.asarray(X, dtype=np.float64)
+        Y = np.asarray(Y, dtype=np.float64)
+        M = np.asarray(M, dtype=np.float64)
+        self.xm, self.xs = X.mean(axis=0), X.std(axis=0) + 1e-6
+        cnt = np.maximum(M.sum(axis=0), 1.0)
+        self.ym = (Y * M).sum(axis=0) / cnt
+        self.ys = np.sqrt(((Y - self.ym) ** 2 * M).sum(axis=0) / cnt) + 1e-6
+        Xn = (X - self.xm) / self.xs
+        Yn = (Y - self.ym) / self.ys * M
+        self.net.set_data(Xn, Yn, M)
+        return self.net.train(epochs, batch=batch, lr=lr, lr_final=lr_final, seed=seed)
+
+    def predict(self, X) -> np.ndarray:
+        Xn = (np.atleast_2d(np.asarray(X, dtype=np.float64)) - self.xm) / self.xs
+        return self.net.forward(Xn).astype(np.float64) * self.ys + self.ym
+
+    def state(self) -> dict:
+        return {"params": self.net.get_params(), "xm": self.xm, "xs": self.xs, "ym": self.ym, "ys": self.ys}
+
+    def load(self, st: dict):
+        self.net.set_params(st["params"])
+        self.xm, self.xs, self.ym, self.ys = st["xm"], st["xs"], st["ym"], st["ys"]
+
+
+def masked_rmse(P, Y, M) -> np.ndarray:
+    """Per-slot RMS error over the pairs where the slot is present."""
+    cnt = np.maximum(M.sum(axis=0), 1.0)
+    return np.sqrt(((P - Y) ** 2 * M).sum(axis=0) / cnt)

@@ -178,6 +178,37 @@ int wt_get_bins(wt_ctx *ctx, int32_t *lohi, double *R);
  * not run). Returns the number of entries available (15). */
 int wt_stage_times(wt_ctx *ctx, double *ms, int32_t cap);
 
+/* ---- Inverse-design surrogate network ------------------------------------------------------
+ * No counterpart in the reference, which has no learned component (its closest analogue is the
+ * closed-form level scaling of FBB_with_parameters.m:25-41, 185-193). A multilayer perceptron
+ * sizes[0] -> sizes[1] -> ... -> sizes[nlayer] with GELU after every hidden layer and a linear
+ * output, trained and evaluated on one GPU by hand-written BF16 MFMA kernels with FP32
+ * accumulation. Loss: masked mean squared error sum(mask (y - t)^2) / sum(mask). Optimiser:
+ * Adam (0.9, 0.999, 1e-8) on FP32 master weights. Parameters are exchanged as one flat FP32
+ * array: per layer the weights W[out][in] row-major, then the biases b[out]. */
+typedef struct wt_mlp wt_mlp;
+int wt_mlp_create(wt_mlp **m, int32_t nlayer, const int32_t *sizes, uint64_t seed, int32_t device);
+void wt_mlp_destroy(wt_mlp *m);
+const char *wt_mlp_last_error(const wt_mlp *m);
+int64_t wt_mlp_num_params(const wt_mlp *m);
+int wt_mlp_get_params(wt_mlp *m, float *flat);
+/* Replaces the parameters and resets the optimiser state. */
+int wt_mlp_set_params(wt_mlp *m, const float *flat);
+/* Y[n][out] = network(X[n][in]). */
+int wt_mlp_forward(wt_mlp *m, int32_t n, const float *X, float *Y);
+/* Loss and its gradient (flat parameter order) on one batch; mask may be NULL (all ones). */
+int wt_mlp_loss_grad(wt_mlp *m, int32_t n, const float *X, const float *T, const float *mask, double *loss,
+                     float *grad);
+/* One Adam step on one batch; *loss is the batch loss before the step. */
+int wt_mlp_train_step(wt_mlp *m, int32_t n, const float *X, const float *T, const float *mask, float lr,
+                      double *loss);
+/* Keeps a training set resident on the device (mask may be NULL). */
+int wt_mlp_set_data(wt_mlp *m, int32_t n, const float *X, const float *T, const float *mask);
+/* Mini-batch Adam over the resident set: `epochs` seeded shuffles, learning rate decaying from lr
+ * to lr_final on a cosine; loss_per_epoch[e] (may be NULL) = mean batch loss of epoch e. */
+int wt_mlp_train(wt_mlp *m, int32_t epochs, int32_t batch, float lr, float lr_final, uint64_t seed,
+                 double *loss_per_epoch);
+
 #ifdef __cplusplus
 }
 #endif
