@@ -132,6 +132,80 @@ def cpu_baseline(p, t, budget_s: float):
             "seconds_parts": {"geometry": t_geo, "deposition": t_dep, "fft": t_fft}}
 
 
+def time_to_target(n_targets: int, n_sweeps: int):
+    """Time-to-target-spectrum on 512^3 FP32 woven fields: hidden designs of the plain method's
+    family (seeded) are generated as targets; the surrogate (trained here on the GPU from the
+    shipped generator-produced pairs) predicts a design and refines it until all 32 log shells are
+    within +-5%; the plain sweep walks its (gamma, density, hurst) grid until one candidate is.
+    Each candidate costs one full 512^3 generation plus its statistics."""
+    from fastweavturb_amd import surrogate as S
+    t0 = time.perf_counter()
+    bank = S.CenterlineBank(S.PAIRS_RECIPE["bank_seed"])
+    t_bank = time.perf_counter() - t0
+    X, Y, M = S.load_pairs()
+    sur = S.Surrogate(seed=1)
+    t0 = time.perf_counter()
+    hist = sur.fit(X, Y, M, epochs=300)
+    t_train = time.perf_counter() - t0
+    log(f"[ttt] bank {t_bank:.1f}s; surrogate trained on {len(X)} pairs in {t_train:.1f}s "
+        f"(loss {hist[0]:.3f} -> {hist[-1]:.4f})")
+    ver = S.Verifier(bank, 512)
+    rows = []
+    try:
+        for k, d in enumerate(S.target_designs(n_targets)):
+            tgt = ver.run(d)
+            r = S.surrogate_design(sur, tgt, ver)
+            row = {"target": S.sweep_family(d), "surrogate_s": r["seconds"], "surrogate_runs": r["runs"],
+                   "surrogate_found": r["found"], "surrogate_error": r["errors"][-1]}
+            if k < n_sweeps:
+                sw = S.sweep_design(tgt, ver)
+                row.update({"sweep_s": sw["seconds"], "sweep_runs": sw["runs"], "sweep_found": sw["found"],
+                            "sweep_grid": sw["grid_size"]})
+            log(f"[ttt] target {k}: {json.dumps(row)}")
+            rows.append(row)
+    finally:
+        ver.close()
+    s_ok = [r for r in rows if r["surrogate_found"]]
+    both = [r for r in rows if "sweep_s" in r and r["surrogate_found"] and r["sweep_found"]]
+    out = {"N": 512, "tolerance": 0.05, "shells": S.NSHELL, "targets": len(rows),
+           "surrogate_found": len(s_ok), "surrogate_mean_s": float(np.mean([r["surrogate_s"] for r in s_ok])) if s_ok else None,
+           "surrogate_mean_runs": float(np.mean([r["surrogate_runs"] for r in s_ok])) if s_ok else None,
+           "sweep_targets": sum("sweep_s" in r for r in rows),
+           "sweep_mean_s": float(np.mean([r["sweep_s"] for r in both])) if both else None,
+           "sweep_mean_runs": float(np.mean([r["sweep_runs"] for r in both])) if both else None,
+           "speedup": (float(np.mean([r["sweep_s"] for r in both]) / np.mean([r["surrogate_s"] for r in both]))
+                       if both else None),
+           "training": {"pairs": int(len(X)), "train_s": t_train, "epochs": 300},
+           "per_target": rows}
+    return out
+
+
+def rocfft_compare(N: int, reps: int = 5) -> dict:
+    """Times rocFFT (through torch.fft, which ROCm builds on hipFFT/rocFFT) on the transforms of
+    the Biot-Savart stage: R2C of the three vorticity components and C2R of the three velocity
+    components at N^3 FP32, as one batched call each, on torch's stream with torch events."""
+    import torch
+    x = torch.randn(3, N, N, N, device="cuda:0", dtype=torch.float32)
+    X = torch.fft.rfftn(x, dim=(1, 2, 3))
+    y = torch.fft.irfftn(X, s=(N, N, N), dim=(1, 2, 3))
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fwd = inv = 0.0
+    for _ in range(reps):
+        e[0].record()
+        X = torch.fft.rfftn(x, dim=(1, 2, 3))
+        e[1].record()
+        y = torch.fft.irfftn(X, s=(N, N, N), dim=(1, 2, 3))
+        e[2].record()
+        torch.cuda.synchronize()
+        fwd += e[0].elapsed_time(e[1])
+        inv += e[1].elapsed_time(e[2])
+    del x, X, y
+    torch.cuda.empty_cache()
+    return {"rocfft_r2c_3x_ms": fwd / reps, "rocfft_c2r_3x_ms": inv / reps,
+            "rocfft_transforms_ms": (fwd + inv) / reps, "via": "torch.fft (hipFFT/rocFFT)", "N": N}
+
+
 def copy_probe(nbytes: int = 1 << 30, reps: int = 3):
     """A plain elementwise copy of known size: each launch reads and writes exactly nbytes."""
     import torch
@@ -166,6 +240,12 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c2", "c1", "small"])
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ttt-targets", type=int, default=4,
+                    help="targets of the time-to-target-spectrum measurement (0 skips it)")
+    ap.add_argument("--ttt-sweeps", type=int, default=1, help="targets also searched by the plain sweep")
+    ap.add_argument("--fft-compare", action="store_true",
+                    help="also time rocFFT's transforms of the same Biot-Savart stage")
+    ap.add_argument("--rocfft-only", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB -> 1 GiB device copy kernel (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -177,6 +257,20 @@ def main():
     if args.copy_probe:
         copy_probe()
         return
+    if args.rocfft_only:
+        print(json.dumps(rocfft_compare(args.rocfft_only)), flush=True)
+        return
+    fftc = None
+    if args.fft_compare:
+        # torch carries its own HIP runtime: it runs in a child process started before this
+        # process touches the GPU
+        import subprocess
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--rocfft-only", "512"],
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            log(f"[bench] rocFFT comparison failed: {r.stderr[-2000:]}")
+        else:
+            fftc = json.loads(r.stdout.strip().splitlines()[-1])
 
     from fastweavturb_amd.api import WovenGPU
     t0 = time.time()
@@ -217,6 +311,16 @@ def main():
         roof = {"kernel": dom, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
                 "frac": achieved / peak, "traffic": measured_traffic(wname, dom), "ms": kernels[dom],
                 "resource": resource}
+    if fftc is not None:
+        spectral = [k for k in ("fft_x", "fft_y", "fft_z_biot_savart", "fft_y_inv", "fft_x_inv", "spectral_reduce")
+                    if k in kernels]
+        fftc["ours_biot_savart_stage_ms"] = float(stage_ms[3] / args.steps)
+        fftc["ours_kernels_ms"] = float(sum(kernels[k] for k in spectral))
+        log(f"[bench] fft compare {json.dumps(fftc)}")
+    ttt = None
+    if args.ttt_targets > 0 and args.config == "c2":
+        g.close()
+        ttt = time_to_target(args.ttt_targets, args.ttt_sweeps)
     cpu = None
     if not args.no_cpu_baseline:
         t0 = time.time()
@@ -229,7 +333,8 @@ def main():
         "config": {"workload": wname, "N": p.N, "tubes": t.nline, "points": int(t.npoint_all),
                    "sub_segments": int(seg["nseg_all"]), "levels": 6 if args.config == "c2" else None},
         "stages_ms": dict(zip(["segments", "bin", "deposit", "biot_savart"], (stage_ms / args.steps).tolist())),
-        "kernels_ms": kernels, "roofline": roof, "cpu_baseline": cpu,
+        "kernels_ms": kernels, "roofline": roof, "cpu_baseline": cpu, "time_to_target": ttt,
+        "fft_vs_rocfft": fftc,
     }
     print(json.dumps(out), flush=True)
 

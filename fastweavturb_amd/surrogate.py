@@ -178,8 +178,9 @@ class CenterlineBank:
     per tube a shift and a circulation jitter U(0.8, 1.2). A design takes the first tubes of each
     level, interpolates linearly between the two nearest Hurst indices (then restores unit step
     rms), scales the wiggle to 0.35 of the drift step (as `fbb.woven_config`), adds the drift and
-    the shift and wraps into [-pi, pi). Equal designs give identical fields; nearby designs give
-    nearby fields, apart from tubes added or removed by the density."""
+    the shift and wraps into [-pi, pi). A non-integer tube count c gives ceil(c) tubes, the last
+    with its circulation scaled by the fraction, so the field changes continuously with every
+    design parameter, and equal designs give identical fields."""
 
     def __init__(self, seed: int, nlev: int = NLEV, wiggle: float = 0.35, arrays: dict | None = None):
         self.seed, self.nlev, self.wiggle = int(seed), nlev, wiggle
@@ -196,7 +197,7 @@ class CenterlineBank:
             self.a[f"x0_{l}"] = (r0.random((cmax, 3)) - 0.5) * 2.0 * np.pi
             self.a[f"u{l}"] = r0.uniform(0.8, 1.2, cmax)
             j = np.arange(1, n + 1, dtype=np.float64) / n
-            shapes = np.zeros((len(HB), cmax, 3, n), dtype=np.float32)
+            shapes = np.zeros((len(HB), 3, cmax, n), dtype=np.float32)
             for h, H in enumerate(HB):
                 rng = np.random.default_rng([self.seed, l, 1])        # same numbers for every H
                 f = 0.5 * (j ** (2 * H) + 1.0 - (1.0 - j) ** (2 * H))
@@ -206,8 +207,12 @@ class CenterlineBank:
                     B = np.concatenate([np.zeros((cnt, 3, 1)), np.cumsum(g, axis=2)], axis=2)
                     B -= f[None, None, :] * B[:, :, -1:]
                     rms = np.sqrt(np.sum(np.diff(B, axis=2) ** 2, axis=(1, 2)) / (n - 1))
-                    shapes[h, c0:c0 + cnt] = B / rms[:, None, None]
+                    shapes[h, :, c0:c0 + cnt] = (B / rms[:, None, None]).transpose(1, 0, 2)
             self.a[f"B{l}"] = shapes
+            # mean product of consecutive-H steps, for the step rms of an interpolated shape
+            d = np.diff(shapes.astype(np.float64), axis=3)
+            self.a[f"cross{l}"] = np.einsum("hcki,hcki->hk", d[:-1], d[1:]) / (n - 1)
+            self.a[f"sq{l}"] = np.einsum("hcki,hcki->hk", d, d) / (n - 1)
 
     def save(self, path: str):
         np.savez(path, seed=self.seed, nlev=self.nlev, wiggle=self.wiggle, **self.a)
@@ -224,30 +229,40 @@ class CenterlineBank:
         if design.nlev > self.nlev:
             raise ValueError("design has more levels than the bank")
         r = design.radii()
-        npoint, xs, sig, gam = [], [], [], []
+        npoint, cols, sig, gam = [], [[], [], []], [], []
+        tt = None
         for l in range(design.nlev):
             n = level_points(l)
-            cnt = min(level_max_tubes(l), max(1, int(round(2.0 ** design.b[l] * 4 * 4 ** l))))
+            creal = min(float(level_max_tubes(l)), max(1.0, 2.0 ** design.b[l] * 4 * 4 ** l))
+            cnt = int(math.ceil(creal - 1e-9))
             H = float(np.clip(design.H[l], HB[0], HB[-1]))
             h = min(len(HB) - 2, int((H - HB[0]) / 0.1))
             t = (H - HB[h]) / 0.1
             Bs = self.a[f"B{l}"]
-            B = (1.0 - t) * Bs[h, :cnt].astype(np.float64) + t * Bs[h + 1, :cnt].astype(np.float64)
-            rms = np.sqrt(np.sum(np.diff(B, axis=2) ** 2, axis=(1, 2)) / (n - 1))
+            ms = ((1 - t) ** 2 * np.asarray(self.a[f"sq{l}"][h, :cnt]) + t * t * np.asarray(self.a[f"sq{l}"][h + 1, :cnt])
+                  + 2 * t * (1 - t) * np.asarray(self.a[f"cross{l}"][h, :cnt]))
             w = self.a[f"w{l}"]
             dB = self.wiggle * 2.0 * np.pi * float(np.linalg.norm(w)) / n
+            scale = (dB / np.sqrt(ms)).astype(np.float32)
             tt = np.arange(n, dtype=np.float64) / n
-            X = (np.asarray(self.a[f"x0_{l}"][:cnt])[:, :, None] + 2.0 * np.pi * w[None, :, None] * tt[None, None, :]
-                 + B * (dB / rms)[:, None, None])
-            X = np.mod(X + np.pi, 2.0 * np.pi) - np.pi
-            xs.append(X.transpose(1, 0, 2).reshape(3, cnt * n))
+            x0 = np.asarray(self.a[f"x0_{l}"][:cnt])
+            for c in range(3):
+                B = (np.float32(1 - t) * Bs[h, c, :cnt] + np.float32(t) * Bs[h + 1, c, :cnt]) * scale[:, None]
+                X = B.astype(np.float64)
+                X += x0[:, c, None] + 2.0 * np.pi * w[c] * tt[None, :]
+                X += np.pi
+                np.mod(X, 2.0 * np.pi, out=X)
+                X -= np.pi
+                cols[c].append(X.ravel())
             npoint.extend([n] * cnt)
             sig.extend([r[l] / math.sqrt(2.0)] * cnt)
-            gam.append(r[0] ** (4.0 / 3.0) * 2.0 ** design.a[l] * np.asarray(self.a[f"u{l}"][:cnt]))
-        X = np.concatenate(xs, axis=1)
+            gl = r[0] ** (4.0 / 3.0) * 2.0 ** design.a[l] * np.array(self.a[f"u{l}"][:cnt], dtype=np.float64)
+            gl[-1] *= creal - (cnt - 1)
+            gam.append(gl)
         nt = len(npoint)
-        return Tubes(np.array(npoint, dtype=np.int32), X[0].copy(), X[1].copy(), X[2].copy(), np.array(sig),
-                     np.concatenate(gam), np.ones(nt, dtype=np.int32), np.zeros(nt, dtype=np.int32))
+        return Tubes(np.array(npoint, dtype=np.int32), np.concatenate(cols[0]), np.concatenate(cols[1]),
+                     np.concatenate(cols[2]), np.array(sig), np.concatenate(gam), np.ones(nt, dtype=np.int32),
+                     np.zeros(nt, dtype=np.int32))
 
     def config(self, design: Design, N: int, ntf_pre: int = 1000):
         t = self.tubes(design)
@@ -265,14 +280,29 @@ def family_design(gamma: float, density: float, hurst: float, nlev: int = NLEV) 
     return Design(a, np.full(nlev, math.log2(density)), q, np.full(nlev, hurst))
 
 
-def draw_inverse_design(rng: np.random.Generator, jitter: bool) -> Design:
+# Densities of the inverse design. Below ~k = 4 a log shell holds only tens of Fourier modes, so
+# the spectrum there is the interference pattern of the exact tube set: one tube more or less
+# moves a shell by tens of percent, and no smooth model or coarse sweep can land on a continuous
+# tube count. Density is therefore one of these values (the same tube set for every design that
+# shares it), and the smooth parameters (circulations, radii, Hurst indices) are searched on it.
+DENSITIES = tuple(float(v) for v in 2.0 ** np.arange(-1.0, 1.0001, 0.5))
+
+
+def snap_density(b) -> float:
+    """log2 density nearest to the mean of the given log2 densities."""
+    lv = np.log2(DENSITIES)
+    return float(lv[np.argmin(np.abs(lv - float(np.mean(b))))])
+
+
+def draw_inverse_design(rng: np.random.Generator, jitter: bool, nlev: int = NLEV) -> Design:
     """Prior of the inverse-design pairs: the plain method's family with gamma U[1.1, 1.5],
-    density U[0.5, 2], H U[0.5, 0.9]; with `jitter`, each level's (a, b, q, H) perturbed too."""
-    d = family_design(rng.uniform(1.1, 1.5), rng.uniform(0.5, 2.0), rng.uniform(0.5, 0.9))
+    density from DENSITIES, H U[0.5, 0.9]; with `jitter`, each level's circulation, radius ratio
+    and Hurst index perturbed too."""
+    d = family_design(rng.uniform(1.1, 1.5), DENSITIES[int(rng.integers(len(DENSITIES)))], rng.uniform(0.5, 0.9),
+                      nlev)
     if jitter:
         L = d.nlev
         d.a = d.a + np.concatenate([[0.0], rng.uniform(-0.3, 0.3, L - 1)])
-        d.b = np.clip(d.b + rng.uniform(-0.3, 0.3, L), *RANGE["b"])
         d.q = np.clip(d.q + rng.uniform(-0.15, 0.15, L), *RANGE["q"])
         d.H = np.clip(d.H + rng.uniform(-0.08, 0.08, L), *RANGE["H"])
     return d
@@ -288,10 +318,25 @@ def _init_worker(bank_path):
 
 
 def _make_instance(args):
+    """Builds one instance's tube table in a worker; the point arrays go back through a file in
+    /dev/shm (memory-backed) instead of the result pipe."""
+    import os
+    import tempfile
     design, N, seed = args
-    if _WORKER_BANK is not None:
-        return _WORKER_BANK.config(design, N)
-    return instance_config(design, N, seed)
+    p, t = _WORKER_BANK.config(design, N) if _WORKER_BANK is not None else instance_config(design, N, seed)
+    fd, path = tempfile.mkstemp(prefix="wt_inst_", suffix=".npy", dir="/dev/shm" if os.path.isdir("/dev/shm") else None)
+    with os.fdopen(fd, "wb") as f:
+        np.save(f, np.stack([t.cx, t.cy, t.cz]))
+    small = Tubes(t.npoint, np.zeros(0), np.zeros(0), np.zeros(0), t.sigma, t.gamma, t.n_T, t.if_filter)
+    return p, small, path
+
+
+def _take_instance(res):
+    import os
+    p, t, path = res
+    xyz = np.load(path)
+    os.unlink(path)
+    return p, Tubes(t.npoint, xyz[0], xyz[1], xyz[2], t.sigma, t.gamma, t.n_T, t.if_filter)
 
 
 def generate_pairs(jobs, ensemble: int = 64, workers: int = 8, log=None, bank_path: str | None = None):
@@ -320,7 +365,7 @@ def generate_pairs(jobs, ensemble: int = 64, workers: int = 8, log=None, bank_pa
             return [ex.submit(_make_instance, j) for j in chunk] if ex else [_make_instance(j) for j in chunk]
         pending = submit(0)
         for lo in range(0, n, ensemble):
-            cfgs = [f.result() for f in pending] if ex else pending
+            cfgs = [_take_instance(f.result()) for f in pending] if ex else [_take_instance(r) for r in pending]
             pending = submit(lo + ensemble) if lo + ensemble < n else []
             g = WovenGPU(precision=32, sizes=[p.N for p, _ in cfgs])
             for k, (p, t) in enumerate(cfgs):
@@ -392,3 +437,156 @@ def masked_rmse(P, Y, M) -> np.ndarray:
     """Per-slot RMS error over the pairs where the slot is present."""
     cnt = np.maximum(M.sum(axis=0), 1.0)
     return np.sqrt(((P - Y) ** 2 * M).sum(axis=0) / cnt)
+
+
+# ---- inverse design ------------------------------------------------------------------------------
+def shell_error(f, target) -> float:
+    """Largest relative deviation of the 32 log shells of statistics f from the target's."""
+    return float(np.max(np.abs(10.0 ** (np.asarray(f)[:NSHELL] - np.asarray(target)[:NSHELL]) - 1.0)))
+
+
+class Verifier:
+    """Runs designs as woven fields from a centerline bank on one GPU context of grid N and
+    measures their statistics; counts runs and GPU wall time."""
+
+    def __init__(self, bank: CenterlineBank, N: int = 512):
+        from .api import WovenGPU
+        self.bank, self.N = bank, N
+        self.nlev = bank.nlev
+        self.g = WovenGPU(N, 32)
+        self.runs = 0
+        self.seconds = 0.0
+
+    def run(self, design: Design) -> np.ndarray:
+        p, t = self.bank.config(design, self.N)
+        t0 = time.perf_counter()
+        self.g.set_params(p)
+        self.g.set_tubes(t)
+        self.g.generate()
+        f = measure(self.g)
+        self.seconds += time.perf_counter() - t0
+        self.runs += 1
+        return f
+
+    def close(self):
+        self.g.close()
+
+
+def surrogate_design(sur: Surrogate, target, verifier: Verifier, tol: float = 0.05, max_runs: int = 8,
+                     damping: float = 0.7, log=None) -> dict:
+    """Predict a design from the target statistics, verify it, and while some shell misses the
+    tolerance correct the design with the network as an approximate inverse:
+    theta <- theta + S(target) - S(measured)."""
+    t0 = time.perf_counter()
+    s_target = sur.predict(target)[0]
+    theta = s_target.copy()
+    bslots = np.arange(NLEV) * NPAR + 1
+    theta[bslots] = snap_density(theta[bslots])       # the density is chosen once, then kept
+    errs = []
+    for k in range(max_runs):
+        f = verifier.run(Design.from_vector(theta, verifier.nlev))
+        errs.append(shell_error(f, target))
+        if log:
+            log(f"[surrogate] run {k + 1}: max shell error {errs[-1] * 100:.2f}%")
+        if errs[-1] <= tol:
+            return {"found": True, "runs": k + 1, "seconds": time.perf_counter() - t0, "errors": errs,
+                    "design": theta}
+        step = damping * (s_target - sur.predict(f)[0])
+        step[bslots] = 0.0
+        theta = theta + step
+    return {"found": False, "runs": max_runs, "seconds": time.perf_counter() - t0, "errors": errs, "design": theta}
+
+
+SWEEP_GRID = {"gamma": np.round(np.arange(1.10, 1.5001, 0.02), 2),
+              "density": np.array(DENSITIES),
+              "hurst": np.round(np.arange(0.5, 0.9001, 0.1), 2)}
+
+
+def sweep_design(target, verifier: Verifier, tol: float = 0.05, max_runs: int | None = None, log=None,
+                 grid: dict | None = None) -> dict:
+    """The plain parameter sweep: the level law of the plain method over a grid of (gamma,
+    density, hurst) in lexicographic order, each candidate verified by one run, until every shell
+    is within the tolerance or the grid (or max_runs) is exhausted."""
+    grid = grid or SWEEP_GRID
+    t0 = time.perf_counter()
+    best = (np.inf, None)
+    k = 0
+    for gm in grid["gamma"]:
+        for dn in grid["density"]:
+            for hh in grid["hurst"]:
+                if max_runs is not None and k >= max_runs:
+                    return {"found": False, "runs": k, "seconds": time.perf_counter() - t0, "best": best,
+                            "grid_size": int(np.prod([len(v) for v in grid.values()]))}
+                f = verifier.run(family_design(float(gm), float(dn), float(hh), verifier.nlev))
+                k += 1
+                e = shell_error(f, target)
+                if e < best[0]:
+                    best = (e, (float(gm), float(dn), float(hh)))
+                if e <= tol:
+                    return {"found": True, "runs": k, "seconds": time.perf_counter() - t0, "best": best,
+                            "grid_size": int(np.prod([len(v) for v in grid.values()]))}
+    return {"found": False, "runs": k, "seconds": time.perf_counter() - t0, "best": best,
+            "grid_size": int(np.prod([len(v) for v in grid.values()]))}
+
+
+def family_jobs(n: int, seed: int, N: int, jitter_frac: float = 0.3, nlev: int = NLEV):
+    """Inverse-design pairs: designs from the plain method's family (a fraction with per-level
+    jitter), all on the bank's tube seed, at grid N."""
+    rng = np.random.default_rng(seed)
+    return [(draw_inverse_design(rng, rng.random() < jitter_frac, nlev), N, 0) for _ in range(n)]
+
+
+def target_designs(n: int, seed: int = 9000, nlev: int = NLEV):
+    """Hidden designs of the inverse-design targets: the plain method's family with gamma,
+    density and hurst drawn inside the sweep's range."""
+    rng = np.random.default_rng(seed)
+    return [family_design(rng.uniform(1.12, 1.48), DENSITIES[int(rng.integers(len(DENSITIES)))],
+                          rng.uniform(0.52, 0.88), nlev) for _ in range(n)]
+
+
+def sweep_family(design: Design, grid: dict | None = None) -> dict:
+    """Where a family design sits on the sweep grid: the index of its density and the nearest grid
+    points in gamma and hurst (for reporting)."""
+    grid = grid or SWEEP_GRID
+    r = design.radii()
+    g = float(design.a[1] / np.log2(r[1] / r[0])) if design.nlev > 1 else float("nan")
+    return {"gamma": g, "density": float(2.0 ** design.b[0]), "hurst": float(design.H[0])}
+
+
+# ---- training set of the shipped surrogate --------------------------------------------------------
+PAIRS_FILE = "data/inverse_pairs.npz"
+PAIRS_RECIPE = {"n": 8192, "jobs_seed": 77, "bank_seed": 1234, "N": 128, "jitter_frac": 0.3}
+
+
+def pairs_path() -> str:
+    import os
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), PAIRS_FILE)
+
+
+def load_pairs():
+    z = np.load(pairs_path())
+    return z["X"], z["Y"], z["M"]
+
+
+def make_pairs(out: str, workers: int = 12, log=print, **recipe):
+    """Regenerates the shipped training set on the GPU (about 15 minutes for 8192 pairs)."""
+    import os
+    import tempfile
+    rc = dict(PAIRS_RECIPE, **recipe)
+    bank = CenterlineBank(rc["bank_seed"])
+    with tempfile.TemporaryDirectory() as d:
+        bp = os.path.join(d, "bank.npz")
+        bank.save(bp)
+        jobs = family_jobs(rc["n"], rc["jobs_seed"], rc["N"], rc["jitter_frac"])
+        X, Y, M, _ = generate_pairs(jobs, ensemble=16, workers=workers, log=log, bank_path=bp)
+    np.savez_compressed(out, X=X, Y=Y, M=M)
+    return X, Y, M
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="regenerate the inverse-design training pairs on the GPU")
+    ap.add_argument("--out", default=pairs_path())
+    ap.add_argument("--workers", type=int, default=12)
+    a = ap.parse_args()
+    make_pairs(a.out, a.workers)
