@@ -206,6 +206,55 @@ def rocfft_compare(N: int, reps: int = 5) -> dict:
             "rocfft_transforms_ms": (fwd + inv) / reps, "via": "torch.fft (hipFFT/rocFFT)", "N": N}
 
 
+def ensemble_bench(steps: int, warmup: int, n: int = 64, graph: bool = True) -> dict:
+    """The ensemble workload: n uneven instances (grid sizes from {64, 96, 128, 192, 256}, 3-6
+    levels, Hurst index, density and circulation exponent drawn per instance, tube seed 5000 + i)
+    generated together in one context, the spectral stage replayed from one hipGraph, then the
+    statistics of every instance in one call (32 log shells of E(k) and the flatness F4, F6 of
+    x-increments at 8 separations)."""
+    from fastweavturb_amd import surrogate as S
+    from fastweavturb_amd.api import WovenGPU
+    t0 = time.time()
+    jobs = S.ensemble_jobs(n, seed=5000)
+    cfgs = [S.instance_config(d, N, seed) for d, N, seed in jobs]
+    sizes = [p.N for p, _ in cfgs]
+    log(f"[c4] {n} instances built in {time.time() - t0:.1f}s; sizes {dict(zip(*np.unique(sizes, return_counts=True)))}")
+    g = WovenGPU(precision=32, sizes=sizes)
+    g.set_graph(graph)
+    for i, (p, t) in enumerate(cfgs):
+        g.select(i)
+        g.set_params(p)
+        g.set_tubes(t)
+    for _ in range(max(1, warmup)):
+        g.generate()
+    t0 = time.perf_counter()
+    stage = np.zeros(4)
+    for _ in range(steps):
+        g.generate()
+        stage += g.stage_times()[:4]
+    dt = (time.perf_counter() - t0) / steps
+    seps = np.stack([S.flat_separations(N) for N in sizes])
+    t1 = time.perf_counter()
+    E, Sf = g.batch_statistics(2, 6, seps)
+    t_stats = time.perf_counter() - t1
+    feats = np.stack([S.features(E[i], Sf[i]) for i in range(n)])
+    seg = g.segment_stats()
+    dep = g.deposit_stats()
+    g.close()
+    pts = float(sum(N ** 3 for N in sizes))
+    return {"metric": "ensemble woven fields per second (64 uneven instances)", "value": n / dt,
+            "unit": "instances/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": dt * 1e3,
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "ensemble_64_uneven", "instances": n, "grid_points": pts,
+                       "sizes": {str(k): int(v) for k, v in zip(*np.unique(sizes, return_counts=True))},
+                       "tubes": int(sum(t.nline for _, t in cfgs)), "points": int(sum(t.npoint_all for _, t in cfgs)),
+                       "graph": graph},
+            "mpts_per_s": pts / dt / 1e6, "stages_ms": dict(zip(["segments", "bin", "deposit", "biot_savart"],
+                                                                 (stage / steps).tolist())),
+            "statistics_ms": t_stats * 1e3, "sub_segments": int(seg["nseg_all"]), "hits": int(dep["hits"]),
+            "features_first": feats[0].round(4).tolist()}
+
+
 def copy_probe(nbytes: int = 1 << 30, reps: int = 3):
     """A plain elementwise copy of known size: each launch reads and writes exactly nbytes."""
     import torch
@@ -237,7 +286,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c2", "c1", "small"])
+    ap.add_argument("--config", default="c2", choices=["c2", "c1", "small", "c4"])
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ttt-targets", type=int, default=4,
@@ -259,6 +308,9 @@ def main():
         return
     if args.rocfft_only:
         print(json.dumps(rocfft_compare(args.rocfft_only)), flush=True)
+        return
+    if args.config == "c4":
+        print(json.dumps(ensemble_bench(args.steps, args.warmup)), flush=True)
         return
     fftc = None
     if args.fft_compare:

@@ -97,6 +97,8 @@ def load_library():
     L.wt_instance_scalars.argtypes = [vp, dp, dp, dp]
     L.wt_get_seg_stats.argtypes = [vp, C.POINTER(_SegStats)]
     L.wt_get_deposit_stats.argtypes = [vp, C.POINTER(_DepStats)]
+    L.wt_batch_statistics.argtypes = [vp, i32, i32, i32, ip, dp, dp]
+    L.wt_set_graph.argtypes = [vp, i32]
     _lib = L
     return L
 
@@ -245,6 +247,23 @@ class WovenGPU:
         S = np.zeros((pmax - pmin + 1) * len(s))
         self._check(self._L.wt_structure_functions(self._h, mode, pmin, pmax, len(s), _ip(s), _dp(S)))
         return S.reshape(pmax - pmin + 1, len(s))
+
+    def batch_statistics(self, pmin: int, pmax: int, seps):
+        """(spectra, S) of every instance: spectra[i] is instance i's integer-shell spectrum of the
+        normalised velocity; S[i, p - pmin, m] its x-increment structure function of order p at
+        separation seps[i][m] (seps: one row per instance, or one row for all)."""
+        n = len(self.sizes)
+        sp = np.ascontiguousarray(np.broadcast_to(np.asarray(seps, dtype=np.int32), (n, np.shape(seps)[-1])))
+        nsep = sp.shape[1]
+        neks = [nek(N) for N in self.sizes]
+        E = np.zeros(sum(neks))
+        S = np.zeros((n, pmax - pmin + 1, nsep))
+        self._check(self._L.wt_batch_statistics(self._h, pmin, pmax, nsep, _ip(sp), _dp(E), _dp(S)))
+        offs = np.concatenate([[0], np.cumsum(neks)])
+        return [E[offs[i]:offs[i + 1]] for i in range(n)], S
+
+    def set_graph(self, enable: bool = True):
+        self._check(self._L.wt_set_graph(self._h, int(enable)))
 
     # --- fields --------------------------------------------------------------------------
     def get_field(self, which, fp64: bool = True) -> np.ndarray:

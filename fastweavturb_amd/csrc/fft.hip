@@ -17,6 +17,8 @@
 //
 // The 1-D FFTs run in LDS: iterative in-place decimation in time over radices 8/4/3/2, data
 // loaded in mixed-radix digit-reversed order, twiddles from a table exp(-2 pi i k / N).
+#include <algorithm>
+
 #include "wt_internal.h"
 
 namespace wt {
@@ -358,9 +360,28 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl, int W, int nchunk,
             if (sh >= 1 && sh <= nek) atomicAdd(shell + nek + sh - 1, 0.5 * wt * ew);
         }
     }
-    // block sums of the two energies
-    atomicAdd(shell + 2 * nek, e_w);
-    atomicAdd(shell + 2 * nek + 1, e_u);
+    // block sums of the two energies in a fixed order (they set up, hence every normalised value):
+    // a shuffle tree per wave, then the waves in order
+    __shared__ double esum[2][16];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        e_w += __shfl_xor(e_w, o, 64);
+        e_u += __shfl_xor(e_u, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        esum[0][threadIdx.x >> 6] = e_w;
+        esum[1][threadIdx.x >> 6] = e_u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int wv = 0; wv < (int)(blockDim.x >> 6); wv++) {
+            a += esum[0][wv];
+            b += esum[1][wv];
+        }
+        shell[2 * nek] = a;
+        shell[2 * nek + 1] = b;
+    }
     __syncthreads();
     double *row = partials + (int64_t)blockIdx.x * (2 * nek + 2);
     for (int i = threadIdx.x; i < 2 * nek + 2; i += blockDim.x) row[i] = shell[i];
@@ -467,6 +488,46 @@ static const void *twiddles_for(wt_ctx *c, int N) {
     return nullptr;
 }
 
+// Static LDS of k_fft_z (the fixed-order energy sums) left free when sizing its dynamic LDS.
+constexpr size_t kZStatic = 512;
+
+// Bytes of per-block partial rows the z pass of K instances of grid N needs.
+static size_t spectral_partials_bytes(wt_ctx *c, int N, int K, int nek) {
+    int dev_lds = 65536;
+    (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+    const size_t cs = c->precision == 64 ? sizeof(C2<double>) : sizeof(C2<float>);
+    const int H = N / 2 + 1;
+    size_t shell_bytes = sizeof(double) * (2 * nek + 2);
+    int Wz = 1;
+    while (Wz < 16 && 3 * (size_t)(2 * Wz) * N * cs + shell_bytes + kZStatic <= (size_t)dev_lds) Wz *= 2;
+    const int ncz = (H + Wz - 1) / Wz;
+    const int64_t zblocks = (int64_t)N * ncz * K;
+    const int cols = 2 * nek + 2;
+    return sizeof(double) * ((size_t)zblocks * cols + (size_t)K * cols);
+}
+
+// Grows the partial-row buffer to what every group of the context needs, so that a captured
+// spectral stage never reallocates.
+hipError_t spectral_reserve(wt_ctx *c) {
+    size_t need = 0;
+    const int n = c->ninst();
+    for (int pos = 0; pos < n;) {
+        int end = pos;
+        while (end < n && c->inst[c->order[end]].N == c->inst[c->order[pos]].N) end++;
+        const wt::Instance &I0 = c->inst[c->order[pos]];
+        need = std::max(need, spectral_partials_bytes(c, I0.N, end - pos, I0.nek));
+        pos = end;
+    }
+    if (need <= c->partials_cap) return hipSuccess;
+    WT_HIP(hipStreamSynchronize(c->stream));
+    if (c->partials) (void)hipFree(c->partials);
+    c->partials = nullptr;
+    c->partials_cap = 0;
+    WT_HIP(hipMalloc(&c->partials, need));
+    c->partials_cap = need;
+    return hipSuccess;
+}
+
 // Biot-Savart (mode 0) or the velocity spectrum (mode 1) of K instances of equal N whose fields,
 // spectra, scalars and spectra arrays are contiguous from arena position `pos`.
 template <typename T>
@@ -513,7 +574,7 @@ static hipError_t spectral_group(wt_ctx *c, int mode, int pos, int K, bool timed
     const int nek = I0.nek;
     size_t shell_bytes = sizeof(double) * (2 * nek + 2);
     int Wz = 1;
-    while (Wz < 16 && 3 * (size_t)(2 * Wz) * N * cs + shell_bytes <= (size_t)dev_lds) Wz *= 2;
+    while (Wz < 16 && 3 * (size_t)(2 * Wz) * N * cs + shell_bytes + kZStatic <= (size_t)dev_lds) Wz *= 2;
     int ncz = (H + Wz - 1) / Wz;
     size_t shz = 3 * (size_t)Wz * N * cs + shell_bytes;
     const int64_t zrows = (int64_t)N * ncz;    // blocks (partial rows) per instance
@@ -521,12 +582,9 @@ static hipError_t spectral_group(wt_ctx *c, int mode, int pos, int K, bool timed
     const int cols = 2 * nek + 2;
     size_t need = sizeof(double) * ((size_t)zblocks * cols + (size_t)K * cols);
     if (need > c->partials_cap) {
-        WT_HIP(hipStreamSynchronize(st));
-        if (c->partials) (void)hipFree(c->partials);
-        c->partials = nullptr;
-        c->partials_cap = 0;
-        WT_HIP(hipMalloc(&c->partials, need));
-        c->partials_cap = need;
+        if (c->capturing) return hipErrorInvalidValue;   // spectral_reserve runs before a capture
+        WT_HIP(spectral_reserve(c));
+        if (need > c->partials_cap) return hipErrorInvalidValue;
     }
     double *colsum = c->partials + (size_t)zblocks * cols;
     WT_HIP(hipFuncSetAttribute((const void *)k_fft_z<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shz));

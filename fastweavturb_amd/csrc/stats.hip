@@ -7,6 +7,9 @@
 // Increments are formed in the field's precision; powers and sums in FP64. Persistent blocks
 // accumulate per-(separation, order) sums in LDS and write one partial row each; the rows are
 // reduced in a fixed order, so results are reproducible.
+#include <algorithm>
+#include <vector>
+
 #include "wt_internal.h"
 
 namespace wt {
@@ -188,6 +191,142 @@ static hipError_t sf_t(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const 
     WT_HIP(hipFreeAsync(tmp, st));
     WT_HIP(hipFreeAsync(partials, st));
     return hipStreamSynchronize(st);
+}
+
+// ---- every instance of a batch in one launch ------------------------------------------------
+// Work item = up to 8192 points of x-lines of one instance; items of an instance are contiguous,
+// each writes one partial row (separation-major, order-minor), and the rows of an instance are
+// summed in item order.
+struct SfItem {
+    int32_t pos, nl;      // arena position, lines in the item
+    int64_t line0;        // first line within the instance
+};
+struct SfInst {
+    int32_t N, pad;
+    int64_t field_off;
+};
+constexpr int kSfItemPoints = 8192;
+
+template <typename T>
+__global__ __launch_bounds__(kSfThreads) void k_sf_x_batch(const T *__restrict__ u, const SfInst *__restrict__ inst,
+                                                           const SfItem *__restrict__ items, int64_t nitems, int pmin,
+                                                           int np, int nsep, const int32_t *__restrict__ seps,
+                                                           double *__restrict__ partials) {
+    extern __shared__ unsigned char smem_raw[];
+    double *lds_acc = (double *)smem_raw;                       // nsep * np
+    double *scratch = lds_acc + (size_t)nsep * np;               // 4 * kMaxOrder
+    T *line = (T *)(scratch + 4 * kMaxOrder);                    // kSfItemPoints
+    for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const SfItem w = items[it];
+        const SfInst I = inst[w.pos];
+        const int N = I.N;
+        for (int i = threadIdx.x; i < nsep * np; i += blockDim.x) lds_acc[i] = 0.0;
+        const T *src = u + I.field_off + w.line0 * N;             // u_x: first component
+        const int npts = w.nl * N;
+        for (int i = threadIdx.x; i < npts; i += blockDim.x) line[i] = src[i];
+        __syncthreads();
+        for (int m = 0; m < nsep; m++) {
+            const int s = ((seps[(int64_t)w.pos * nsep + m] % N) + N) % N;
+            double acc[kMaxOrder];
+            for (int p = 0; p < np; p++) acc[p] = 0.0;
+            for (int i = threadIdx.x; i < npts; i += blockDim.x) {
+                const int li = i / N, x = i - li * N;
+                int xs = x + s;
+                if (xs >= N) xs -= N;
+                const double d = (double)line[li * N + xs] - (double)line[i];
+                double pw = 1.0;
+                for (int p = 1; p < pmin; p++) pw *= d;
+                for (int p = 0; p < np; p++) {
+                    pw *= d;
+                    acc[p] += pw;
+                }
+            }
+            block_accumulate(acc, np, lds_acc + (size_t)m * np, scratch);
+        }
+        double *row = partials + it * nsep * np;
+        for (int i = threadIdx.x; i < nsep * np; i += blockDim.x) row[i] = lds_acc[i];
+        __syncthreads();
+    }
+}
+
+// out[(pos*np + p)*nsep + m] = sum over the position's items of column m*np + p, / N^3.
+__global__ void k_sf_reduce_items(const double *__restrict__ partials, const int64_t *__restrict__ ibase,
+                                  const SfInst *__restrict__ inst, int np, int nsep, double *__restrict__ out) {
+    const int pos = blockIdx.y;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cols = nsep * np;
+    if (col >= cols) return;
+    double s = 0.0;
+    for (int64_t r = ibase[pos]; r < ibase[pos + 1]; r++) s += partials[r * cols + col];
+    const double N = (double)inst[pos].N;
+    const int m = col / np, p = col - m * np;
+    out[((int64_t)pos * np + p) * nsep + m] = s / (N * N * N);
+}
+
+template <typename T>
+static hipError_t sf_batch_t(wt_ctx *c, int pmin, int pmax, int nsep, const int32_t *d_seps, double *d_out) {
+    hipStream_t st = c->stream;
+    const int np = pmax - pmin + 1;
+    const int npos = c->ninst();
+    // work items, built once per context (the layout is fixed at creation)
+    if (!c->sf_items) {
+        std::vector<SfItem> items;
+        std::vector<int64_t> ibase(npos + 1, 0);
+        std::vector<SfInst> inst(npos);
+        for (int pos = 0; pos < npos; pos++) {
+            const wt::Instance &I = c->inst[c->order[pos]];
+            inst[pos] = SfInst{I.N, 0, I.field_off};
+            const int lpg = kSfItemPoints / I.N > 0 ? kSfItemPoints / I.N : 1;
+            const int64_t nlines = (int64_t)I.N * I.N;
+            ibase[pos] = (int64_t)items.size();
+            for (int64_t l0 = 0; l0 < nlines; l0 += lpg)
+                items.push_back(SfItem{pos, (int32_t)std::min<int64_t>(lpg, nlines - l0), l0});
+        }
+        ibase[npos] = (int64_t)items.size();
+        const size_t bytes = sizeof(SfItem) * items.size() + sizeof(int64_t) * (npos + 1) + sizeof(SfInst) * npos;
+        void *d = nullptr;
+        WT_HIP(hipMalloc(&d, bytes));
+        unsigned char *b = (unsigned char *)d;
+        hipError_t e = hipMemcpy(b, items.data(), sizeof(SfItem) * items.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b + sizeof(SfItem) * items.size(), ibase.data(), sizeof(int64_t) * (npos + 1),
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b + sizeof(SfItem) * items.size() + sizeof(int64_t) * (npos + 1), inst.data(),
+                          sizeof(SfInst) * npos, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            return e;
+        }
+        c->sf_items = d;
+        c->sf_nitems = (int64_t)items.size();
+    }
+    const SfItem *d_items = (const SfItem *)c->sf_items;
+    const int64_t *d_ibase = (const int64_t *)(d_items + c->sf_nitems);
+    const SfInst *d_inst = (const SfInst *)(d_ibase + npos + 1);
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    int dev_lds = 65536;
+    (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+    const size_t sh = sizeof(double) * ((size_t)nsep * np + 4 * kMaxOrder) + sizeof(T) * kSfItemPoints;
+    if (sh > (size_t)dev_lds) return hipErrorInvalidValue;
+    const int64_t blocks = std::min<int64_t>(c->sf_nitems, (int64_t)cus * 4);
+    double *partials = nullptr;
+    WT_HIP(hipMallocAsync((void **)&partials, sizeof(double) * c->sf_nitems * nsep * np, st));
+    WT_HIP(hipFuncSetAttribute((const void *)k_sf_x_batch<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    hipLaunchKernelGGL(k_sf_x_batch<T>, dim3((unsigned)blocks), dim3(kSfThreads), sh, st, (const T *)c->u, d_inst,
+                       d_items, c->sf_nitems, pmin, np, nsep, d_seps, partials);
+    WT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sf_reduce_items, dim3((unsigned)((nsep * np + 63) / 64), (unsigned)npos), dim3(64), 0, st,
+                       (const double *)partials, d_ibase, d_inst, np, nsep, d_out);
+    WT_HIP(hipGetLastError());
+    return hipFreeAsync(partials, st);
+}
+
+hipError_t launch_sf_batch(wt_ctx *c, int pmin, int pmax, int nsep, const int32_t *d_seps, double *d_out) {
+    if (pmax - pmin + 1 > kMaxOrder || pmin < 1) return hipErrorInvalidValue;
+    return c->precision == 64 ? sf_batch_t<double>(c, pmin, pmax, nsep, d_seps, d_out)
+                              : sf_batch_t<float>(c, pmin, pmax, nsep, d_seps, d_out);
 }
 
 hipError_t launch_sf(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const int32_t *d_seps, double *d_out) {

@@ -112,7 +112,10 @@ void wt_destroy(wt_ctx *c) {
     free_list(c->bin_allocs);
     free_list(c->seg_allocs);
     free_list(c->geo_allocs);
-    void *bufs[] = {c->omega, c->u, c->spec, c->pairs_buf, c->partials, c->scal, c->E_w, c->E_u, c->runs_dev};
+    if (c->spec_exec) (void)hipGraphExecDestroy(c->spec_exec);
+    if (c->spec_graph) (void)hipGraphDestroy(c->spec_graph);
+    void *bufs[] = {c->omega, c->u, c->spec, c->pairs_buf, c->partials, c->scal, c->E_w, c->E_u, c->runs_dev,
+                    c->sf_items};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (void *p : c->tw_tab)
@@ -507,10 +510,50 @@ static int read_scalars(wt_ctx *c, int inst, wt::DeviceScalars *s) {
     return WT_OK;
 }
 
+// Records the spectral stage of every group (x, y, z + Biot-Savart + spectra, reductions, y and x
+// inverse) as a graph on the context stream. The stage only reads the context's fixed arenas, so
+// the graph stays valid for the context's lifetime.
+static hipError_t capture_spectral(wt_ctx *c) {
+    WT_HIP(wt::spectral_reserve(c));
+    WT_HIP(hipStreamSynchronize(c->stream));
+    WT_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    hipError_t e = wt::launch_spectral(c, 0, -1);
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e == hipSuccess) e = e2;
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return e;
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return e;
+    }
+    c->spec_graph = g;
+    c->spec_exec = x;
+    return hipSuccess;
+}
+
+int wt_set_graph(wt_ctx *c, int32_t enable) {
+    CHECK_CTX(c);
+    c->use_graph = enable != 0;
+    return WT_OK;
+}
+
 int wt_biot_savart(wt_ctx *c, double *etot, double *up, double *enstrophy) {
     CHECK_CTX(c);
     if (!c->have_omega) return set_err(c, WT_ESTATE, "wt_biot_savart needs a deposited or set vorticity");
-    hipError_t e = wt::launch_spectral(c, 0, -1);
+    hipError_t e;
+    if (c->use_graph && c->spec_exec) {
+        e = hipGraphLaunch(c->spec_exec, c->stream);
+    } else {
+        e = wt::launch_spectral(c, 0, -1);
+        if (e == hipSuccess && c->use_graph) e = capture_spectral(c);   // replayed from the next call on
+    }
     if (e != hipSuccess) return hip_err(c, e, "biot_savart");
     std::vector<wt::DeviceScalars> all(c->ninst());
     HIPCALL(c, hipMemcpyAsync(all.data(), c->scal, sizeof(wt::DeviceScalars) * c->ninst(), hipMemcpyDeviceToHost,
@@ -628,6 +671,58 @@ int wt_structure_functions(wt_ctx *c, int32_t mode, int32_t pmin, int32_t pmax, 
     if (d_out) (void)hipFree(d_out);
     if (e == hipErrorInvalidValue) return set_err(c, WT_ERANGE, "too many separations/orders for one block's LDS");
     if (e != hipSuccess) return hip_err(c, e, "structure functions");
+    return WT_OK;
+}
+
+int wt_batch_statistics(wt_ctx *c, int32_t pmin, int32_t pmax, int32_t nsep, const int32_t *seps, double *E,
+                        double *S) {
+    CHECK_CTX(c);
+    if (!c->have_u) return set_err(c, WT_ESTATE, "no velocity resident");
+    if (pmin < 1 || pmax < pmin || pmax - pmin + 1 > 12)
+        return set_err(c, WT_EINVAL, "need 1 <= pmin <= pmax, at most 12 orders");
+    if (nsep < 0 || (nsep > 0 && (!seps || !S)))
+        return set_err(c, WT_EINVAL, "need nsep >= 0 and, when nsep > 0, separations and an output array");
+    const int ni = c->ninst(), np = pmax - pmin + 1;
+    if (E) {
+        if (!c->have_spectra) {
+            for (int i = 0; i < ni; i++) {
+                hipError_t e = wt::launch_spectral(c, 1, i);
+                if (e != hipSuccess) return hip_err(c, e, "spectrum");
+            }
+            c->have_spectra = true;
+        }
+        std::vector<double> all(c->E_elems);
+        HIPCALL(c, hipMemcpyAsync(all.data(), c->E_u, sizeof(double) * c->E_elems, hipMemcpyDeviceToHost, c->stream),
+                "spectra");
+        HIPCALL(c, hipStreamSynchronize(c->stream), "spectra");
+        int64_t o = 0;
+        for (int i = 0; i < ni; i++) {
+            const wt::Instance &I = c->inst[i];
+            memcpy(E + o, all.data() + I.E_off, sizeof(double) * I.nek);
+            o += I.nek;
+        }
+    }
+    if (nsep == 0) return WT_OK;
+    std::vector<int32_t> sp((size_t)ni * nsep);
+    for (int pos = 0; pos < ni; pos++)
+        memcpy(sp.data() + (size_t)pos * nsep, seps + (size_t)c->order[pos] * nsep, sizeof(int32_t) * nsep);
+    int32_t *d_seps = nullptr;
+    double *d_out = nullptr;
+    HIPCALL(c, hipMallocAsync((void **)&d_seps, sizeof(int32_t) * sp.size(), c->stream), "alloc");
+    hipError_t e = hipMallocAsync((void **)&d_out, sizeof(double) * (size_t)ni * np * nsep, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_seps, sp.data(), sizeof(int32_t) * sp.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = wt::launch_sf_batch(c, pmin, pmax, nsep, d_seps, d_out);
+    std::vector<double> out((size_t)ni * np * nsep);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out.data(), d_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, c->stream);
+    if (d_seps) (void)hipFreeAsync(d_seps, c->stream);
+    if (d_out) (void)hipFreeAsync(d_out, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipErrorInvalidValue) return set_err(c, WT_ERANGE, "too many separations/orders for one block's LDS");
+    if (e != hipSuccess) return hip_err(c, e, "batch statistics");
+    for (int pos = 0; pos < ni; pos++)
+        memcpy(S + (size_t)c->order[pos] * np * nsep, out.data() + (size_t)pos * np * nsep, sizeof(double) * np * nsep);
     return WT_OK;
 }
 

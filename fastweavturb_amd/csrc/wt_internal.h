@@ -151,6 +151,15 @@ struct wt_ctx {
     // per-kernel timing slots (see wt::kSlot*), events recorded on `stream`
     hipEvent_t kev[16][2] = {};
     bool kvalid[16] = {};
+    // hipGraph of the spectral stage (all groups), captured once after the first eager run
+    bool use_graph = false;
+    bool capturing = false;
+    hipGraph_t spec_graph = nullptr;
+    hipGraphExec_t spec_exec = nullptr;
+    // batched structure functions: work items and per-position separations
+    void *sf_items = nullptr;
+    int64_t sf_nitems = 0;
+    int sf_items_lpg = 0;
     int ninst() const { return (int)inst.size(); }
 };
 
@@ -161,9 +170,11 @@ enum : int {
     kSlotFftX, kSlotFftY, kSlotFftZ, kSlotFftYInv, kSlotFftXInv, kSlotReduce, kNumSlots
 };
 inline void kt_begin(wt_ctx *c, int slot) {
+    if (c->capturing) return;
     if (c->kev[slot][0] && hipEventRecord(c->kev[slot][0], c->stream) == hipSuccess) c->kvalid[slot] = false;
 }
 inline void kt_end(wt_ctx *c, int slot) {
+    if (c->capturing) return;
     if (c->kev[slot][1] && hipEventRecord(c->kev[slot][1], c->stream) == hipSuccess) c->kvalid[slot] = true;
 }
 // segments.hip
@@ -176,11 +187,15 @@ hipError_t launch_deposit(wt_ctx *c);
 // fft.hip: mode 0 = Biot-Savart + normalisation of every instance; mode 1 = spectrum of the
 // resident velocity of instance `only`
 hipError_t launch_spectral(wt_ctx *c, int mode, int only);
+hipError_t spectral_reserve(wt_ctx *c);
 bool fft_size_supported(int N);
 void fft_reversal(int N, std::vector<int32_t> &rev);
 // stats.hip: structure functions of instance c->cur
 hipError_t launch_sf(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const int32_t *d_seps,
                      double *d_out);
+// x-increment structure functions of every instance in one launch: d_seps[pos*nsep + m] per arena
+// position, d_out[(pos*np + p)*nsep + m]
+hipError_t launch_sf_batch(wt_ctx *c, int pmin, int pmax, int nsep, const int32_t *d_seps, double *d_out);
 // scan.hip
 hipError_t exclusive_scan_i64(const int64_t *d_in, int64_t *d_out, int64_t n, hipStream_t s,
                               void **tmp, size_t *tmp_bytes);

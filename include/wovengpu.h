@@ -157,6 +157,20 @@ int wt_structure_functions(wt_ctx *ctx, int32_t mode, int32_t pmin, int32_t pmax
 /* Separations of Sp_calculate (subroutine.f90:78, 97-98). Returns the count. */
 int32_t wt_sp_separations(int32_t n, int32_t *r, int32_t cap);
 
+/* Statistics of every instance of a context in one call (no counterpart in the reference, which
+ * makes one field per run; these are the ensemble workload's outputs). E (may be NULL) receives
+ * each instance's integer-shell spectrum of the normalised velocity, instances in order, nek(N_i)
+ * values each (as wt_spectrum with field 1). S receives x-increment structure functions of u_x
+ * (as mode 1 of wt_structure_functions) for every instance in one launch:
+ * S[(i*(pmax-pmin+1) + p - pmin)*nsep + m] at separation seps[i*nsep + m] of instance i. */
+int wt_batch_statistics(wt_ctx *ctx, int32_t pmin, int32_t pmax, int32_t nsep, const int32_t *seps, double *E,
+                        double *S);
+
+/* Replay the spectral stage of wt_biot_savart / wt_generate (every FFT pass, Biot-Savart, spectra,
+ * reductions and the normalising inverse, for all instances) from one hipGraph: captured after
+ * the first eager run, launched as one graph afterwards (no counterpart in the reference). */
+int wt_set_graph(wt_ctx *ctx, int32_t enable);
+
 /* Copy a resident field component to/from host memory, Fortran order, FP64 or FP32 host
  * buffer (output_v/output_puredata copy-out, subroutine.f90:1342-1399). wt_set_field replaces
  * the resident omega or u (used to drive wt_biot_savart / statistics on a given field). */
