@@ -206,7 +206,7 @@ def rocfft_compare(N: int, reps: int = 5) -> dict:
             "rocfft_transforms_ms": (fwd + inv) / reps, "via": "torch.fft (hipFFT/rocFFT)", "N": N}
 
 
-def ensemble_bench(steps: int, warmup: int, n: int = 64, graph: bool = True) -> dict:
+def ensemble_bench(steps: int, warmup: int, n: int = 64, graph: bool = True, tile_cache: str | None = None) -> dict:
     """The ensemble workload: n uneven instances (grid sizes from {64, 96, 128, 192, 256}, 3-6
     levels, Hurst index, density and circulation exponent drawn per instance, tube seed 5000 + i)
     generated together in one context, the spectral stage replayed from one hipGraph, then the
@@ -221,6 +221,7 @@ def ensemble_bench(steps: int, warmup: int, n: int = 64, graph: bool = True) -> 
     log(f"[c4] {n} instances built in {time.time() - t0:.1f}s; sizes {dict(zip(*np.unique(sizes, return_counts=True)))}")
     g = WovenGPU(precision=32, sizes=sizes)
     g.set_graph(graph)
+    log("[c4] autotune: " + g.autotune(tile_cache).strip().replace("\n", "; "))
     for i, (p, t) in enumerate(cfgs):
         g.select(i)
         g.set_params(p)
@@ -292,6 +293,9 @@ def main():
     ap.add_argument("--ttt-targets", type=int, default=4,
                     help="targets of the time-to-target-spectrum measurement (0 skips it)")
     ap.add_argument("--ttt-sweeps", type=int, default=1, help="targets also searched by the plain sweep")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default FFT tile shapes")
+    ap.add_argument("--tile-cache", default=os.path.join(ROOT, "fastweavturb_amd", "data", "fft_tiles.json"),
+                    help="autotune cache (JSON): shapes recorded there are reused, new ones appended")
     ap.add_argument("--fft-compare", action="store_true",
                     help="also time rocFFT's transforms of the same Biot-Savart stage")
     ap.add_argument("--rocfft-only", type=int, default=0, help=argparse.SUPPRESS)
@@ -310,7 +314,7 @@ def main():
         print(json.dumps(rocfft_compare(args.rocfft_only)), flush=True)
         return
     if args.config == "c4":
-        print(json.dumps(ensemble_bench(args.steps, args.warmup)), flush=True)
+        print(json.dumps(ensemble_bench(args.steps, args.warmup, tile_cache=args.tile_cache)), flush=True)
         return
     fftc = None
     if args.fft_compare:
@@ -332,6 +336,8 @@ def main():
     g = WovenGPU(p.N, precision)
     g.set_params(p)
     g.set_tubes(t)
+    if not args.no_autotune:
+        log("[bench] autotune: " + g.autotune(args.tile_cache).strip().replace("\n", "; "))
     for i in range(args.warmup):
         g.generate()
         log(f"[bench] warmup {i} done, stages ms {np.round(g.stage_times()[:4], 2).tolist()}")

@@ -99,6 +99,7 @@ def load_library():
     L.wt_get_deposit_stats.argtypes = [vp, C.POINTER(_DepStats)]
     L.wt_batch_statistics.argtypes = [vp, i32, i32, i32, ip, dp, dp]
     L.wt_set_graph.argtypes = [vp, i32]
+    L.wt_autotune.argtypes = [vp, C.c_char_p, C.c_char_p, i32]
     _lib = L
     return L
 
@@ -261,6 +262,13 @@ class WovenGPU:
         self._check(self._L.wt_batch_statistics(self._h, pmin, pmax, nsep, _ip(sp), _dp(E), _dp(S)))
         offs = np.concatenate([[0], np.cumsum(neks)])
         return [E[offs[i]:offs[i + 1]] for i in range(n)], S
+
+    def autotune(self, cache_path: str | None = None) -> str:
+        """Times and keeps the fastest FFT tile shapes for this context's grid sizes; returns the
+        per-size report."""
+        buf = C.create_string_buffer(4096)
+        self._check(self._L.wt_autotune(self._h, cache_path.encode() if cache_path else None, buf, len(buf)))
+        return buf.value.decode()
 
     def set_graph(self, enable: bool = True):
         self._check(self._L.wt_set_graph(self._h, int(enable)))

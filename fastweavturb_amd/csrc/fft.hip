@@ -18,6 +18,8 @@
 // The 1-D FFTs run in LDS: iterative in-place decimation in time over radices 8/4/3/2, data
 // loaded in mixed-radix digit-reversed order, twiddles from a table exp(-2 pi i k / N).
 #include <algorithm>
+#include <cstdio>
+#include <string>
 
 #include "wt_internal.h"
 
@@ -42,6 +44,34 @@ struct FftPlan {
     const int32_t *rev;   // digit-reversal permutation, N entries
     const void *tw;       // N complex twiddles exp(-2 pi i k / N) of type C2<T>
 };
+
+// The twiddle and digit-reversal tables of a plan, copied into LDS at `area` (16-byte aligned,
+// plan_lds_bytes(N) long); the returned plan points at the copies. The caller synchronises.
+__host__ __device__ constexpr size_t plan_lds_bytes(int N, size_t cs) {
+    return ((size_t)N * cs + (size_t)N * sizeof(int32_t) + 15) / 16 * 16;
+}
+template <typename T>
+__device__ __forceinline__ FftPlan stage_plan(const FftPlan &pl, unsigned char *area) {
+    C2<T> *tw = (C2<T> *)area;
+    int32_t *rv = (int32_t *)(tw + pl.N);
+    const C2<T> *gtw = (const C2<T> *)pl.tw;
+    for (int i = threadIdx.x; i < pl.N; i += blockDim.x) {
+        tw[i] = gtw[i];
+        rv[i] = pl.rev[i];
+    }
+    FftPlan q = pl;
+    q.tw = tw;
+    q.rev = rv;
+    return q;
+}
+
+
+// LDS layout of the line buffers: element j of a line sits at pidx(j) = j + j/16 (one padding
+// element per 16), and lines are lds_pitch(N) = N + N/16 + 1 elements apart (1 mod 32). Butterfly
+// strides (powers of the radices) and the transposed tiles of the y and z passes (lanes one line
+// apart) then spread over the LDS banks instead of piling onto a few.
+__host__ __device__ constexpr int lds_pitch(int N) { return N + N / 16 + 1; }
+__device__ __forceinline__ int pidx(int j) { return j + (j >> 4); }
 
 template <typename T, bool INV>
 __device__ __forceinline__ void dft4(C2<T> *v) {
@@ -99,15 +129,26 @@ __device__ __forceinline__ void fft_pass(C2<T> *buf, int nlines, int ldb, int N,
     const int nbf = N / R;
     const int total = nlines * nbf;
     const int step = N / (L * R);
+    const bool pow2 = (N & (N - 1)) == 0;   // then nbf and L are powers of two: shifts, not divisions
+    const int snbf = __ffs(nbf) - 1, sL = __ffs(L) - 1;
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
-        int line = t / nbf;
-        int b = t - line * nbf;
-        int g = b / L;
-        int k = b - g * L;
-        C2<T> *p = buf + line * ldb + g * L * R + k;
+        int line, b, g, k;
+        if (pow2) {
+            line = t >> snbf;
+            b = t & (nbf - 1);
+            g = b >> sL;
+            k = b & (L - 1);
+        } else {
+            line = t / nbf;
+            b = t - line * nbf;
+            g = b / L;
+            k = b - g * L;
+        }
+        C2<T> *p = buf + line * ldb;
+        const int j0 = g * L * R + k;
         C2<T> v[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) v[r] = p[r * L];
+        for (int r = 0; r < R; r++) v[r] = p[pidx(j0 + r * L)];
         if (L > 1) {
 #pragma unroll
             for (int r = 1; r < R; r++) {
@@ -118,7 +159,7 @@ __device__ __forceinline__ void fft_pass(C2<T> *buf, int nlines, int ldb, int N,
         }
         dft_small<T, R, INV>(v);
 #pragma unroll
-        for (int r = 0; r < R; r++) p[r * L] = v[r];
+        for (int r = 0; r < R; r++) p[pidx(j0 + r * L)] = v[r];
     }
 }
 
@@ -148,15 +189,26 @@ __device__ __forceinline__ void fft_pass_dif(C2<T> *buf, int nlines, int ldb, in
     const int nbf = N / R;
     const int total = nlines * nbf;
     const int step = N / (L * R);
+    const bool pow2 = (N & (N - 1)) == 0;   // then nbf and L are powers of two: shifts, not divisions
+    const int snbf = __ffs(nbf) - 1, sL = __ffs(L) - 1;
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
-        int line = t / nbf;
-        int b = t - line * nbf;
-        int g = b / L;
-        int k = b - g * L;
-        C2<T> *p = buf + line * ldb + g * L * R + k;
+        int line, b, g, k;
+        if (pow2) {
+            line = t >> snbf;
+            b = t & (nbf - 1);
+            g = b >> sL;
+            k = b & (L - 1);
+        } else {
+            line = t / nbf;
+            b = t - line * nbf;
+            g = b / L;
+            k = b - g * L;
+        }
+        C2<T> *p = buf + line * ldb;
+        const int j0 = g * L * R + k;
         C2<T> v[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) v[r] = p[r * L];
+        for (int r = 0; r < R; r++) v[r] = p[pidx(j0 + r * L)];
         dft_small<T, R, INV>(v);
         if (L > 1) {
 #pragma unroll
@@ -167,7 +219,7 @@ __device__ __forceinline__ void fft_pass_dif(C2<T> *buf, int nlines, int ldb, in
             }
         }
 #pragma unroll
-        for (int r = 0; r < R; r++) p[r * L] = v[r];
+        for (int r = 0; r < R; r++) p[pidx(j0 + r * L)] = v[r];
     }
 }
 
@@ -190,25 +242,27 @@ __device__ void fft_lds_dif(C2<T> *buf, int nlines, int ldb, const FftPlan &pl) 
 
 // ---- x pass, forward: two real lines -> two half spectra --------------------------------
 template <typename T>
-__global__ void k_fft_x_r2c(const T *__restrict__ f, C2<T> *__restrict__ spec, FftPlan pl, int pairs_per_block,
+__global__ void k_fft_x_r2c(const T *__restrict__ f, C2<T> *__restrict__ spec, FftPlan pl_g, int pairs_per_block,
                             int64_t npairs, T scale) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = (C2<T> *)smem_raw;
-    const int N = pl.N, H = N / 2 + 1;
+    const int N = pl_g.N, H = N / 2 + 1, P = lds_pitch(N);
+    const FftPlan pl = stage_plan<T>(pl_g, smem_raw + sizeof(C2<T>) * (size_t)pairs_per_block * P);
+    __syncthreads();
     const int64_t pair0 = (int64_t)blockIdx.x * pairs_per_block;
     int np = (int)((npairs - pair0) < pairs_per_block ? (npairs - pair0) : pairs_per_block);
     for (int i = threadIdx.x; i < np * N; i += blockDim.x) {
         int pp = i / N, x = i - pp * N;
         int64_t line = 2 * (pair0 + pp);
         T a = f[line * N + x], b = f[(line + 1) * N + x];
-        buf[pp * N + pl.rev[x]] = C2<T>{a, b};
+        buf[pp * P + pidx(pl.rev[x])] = C2<T>{a, b};
     }
     __syncthreads();
-    fft_lds<T, false>(buf, np, N, pl);
+    fft_lds<T, false>(buf, np, P, pl);
     for (int i = threadIdx.x; i < np * H; i += blockDim.x) {
         int pp = i / H, k = i - pp * H;
-        C2<T> z = buf[pp * N + (k == N / 2 ? N / 2 : k)];
-        C2<T> zc = buf[pp * N + ((N - k) % N)];
+        C2<T> z = buf[pp * P + pidx(k == N / 2 ? N / 2 : k)];
+        C2<T> zc = buf[pp * P + pidx((N - k) % N)];
         // A = (Z(k) + conj Z(N-k)) / 2, B = (Z(k) - conj Z(N-k)) / (2i)
         C2<T> A = {(z.x + zc.x) * (T)0.5, (z.y - zc.y) * (T)0.5};
         C2<T> B = {(z.y + zc.y) * (T)0.5, (zc.x - z.x) * (T)0.5};
@@ -220,12 +274,14 @@ __global__ void k_fft_x_r2c(const T *__restrict__ f, C2<T> *__restrict__ spec, F
 
 // ---- x pass, inverse: two half spectra -> two real lines, times inv_up; omega too --------
 template <typename T>
-__global__ void k_fft_x_c2r(const C2<T> *__restrict__ spec, T *__restrict__ u, T *__restrict__ omega, FftPlan pl,
+__global__ void k_fft_x_c2r(const C2<T> *__restrict__ spec, T *__restrict__ u, T *__restrict__ omega, FftPlan pl_g,
                             int pairs_per_block, int64_t npairs, const DeviceScalars *__restrict__ sc,
                             int64_t pairs_per_inst) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = (C2<T> *)smem_raw;
-    const int N = pl.N, H = N / 2 + 1;
+    const int N = pl_g.N, H = N / 2 + 1, P = lds_pitch(N);
+    const FftPlan pl = stage_plan<T>(pl_g, smem_raw + sizeof(C2<T>) * (size_t)pairs_per_block * P);
+    __syncthreads();
     const int64_t pair0 = (int64_t)blockIdx.x * pairs_per_block;
     int np = (int)((npairs - pair0) < pairs_per_block ? (npairs - pair0) : pairs_per_block);
     for (int i = threadIdx.x; i < np * N; i += blockDim.x) {
@@ -236,16 +292,16 @@ __global__ void k_fft_x_c2r(const C2<T> *__restrict__ spec, T *__restrict__ u, T
         if (k == 0 || k == N / 2) { A.y = 0; B.y = 0; }
         if (k > N / 2) { A.y = -A.y; B.y = -B.y; }   // Hermitian extension
         // Z = A + i B
-        buf[pp * N + k] = C2<T>{A.x - B.y, A.y + B.x};
+        buf[pp * P + pidx(k)] = C2<T>{A.x - B.y, A.y + B.x};
     }
     __syncthreads();
-    fft_lds_dif<T, true>(buf, np, N, pl);
+    fft_lds_dif<T, true>(buf, np, P, pl);
     // a block's pairs may straddle two instances of the group
     for (int i = threadIdx.x; i < np * N; i += blockDim.x) {
         int pp = i / N, x = i - pp * N;
         int64_t line = 2 * (pair0 + pp);
         const T s = (T)sc[(pair0 + pp) / pairs_per_inst].inv_up;
-        C2<T> z = buf[pp * N + pl.rev[x]];
+        C2<T> z = buf[pp * P + pidx(pl.rev[x])];
         u[line * N + x] = z.x * s;
         u[(line + 1) * N + x] = z.y * s;
         if (omega) {
@@ -257,10 +313,12 @@ __global__ void k_fft_x_c2r(const C2<T> *__restrict__ spec, T *__restrict__ u, T
 
 // ---- y pass: tiles of W kx columns x N y values ------------------------------------------
 template <typename T, bool INV>
-__global__ void k_fft_y(C2<T> *__restrict__ spec, FftPlan pl, int W, int nchunk) {
-    extern __shared__ unsigned char smem_raw[];
+__global__ void k_fft_y(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchunk) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = (C2<T> *)smem_raw;
-    const int N = pl.N, H = N / 2 + 1;
+    const int N = pl_g.N, H = N / 2 + 1, P = lds_pitch(N);
+    const FftPlan pl = stage_plan<T>(pl_g, smem_raw + sizeof(C2<T>) * (size_t)W * P);
+    __syncthreads();
     int chunk = blockIdx.x % nchunk;
     int64_t plane = blockIdx.x / nchunk;  // c*N + z
     int kx0 = chunk * W;
@@ -268,14 +326,14 @@ __global__ void k_fft_y(C2<T> *__restrict__ spec, FftPlan pl, int W, int nchunk)
     C2<T> *base = spec + plane * (int64_t)N * H;
     for (int i = threadIdx.x; i < W * N; i += blockDim.x) {
         int w = i % W, y = i / W;
-        if (w < w_valid) buf[w * N + (INV ? y : pl.rev[y])] = base[(int64_t)y * H + kx0 + w];
+        if (w < w_valid) buf[w * P + pidx(INV ? y : pl.rev[y])] = base[(int64_t)y * H + kx0 + w];
     }
     __syncthreads();
-    if (INV) fft_lds_dif<T, true>(buf, w_valid, N, pl);
-    else fft_lds<T, false>(buf, w_valid, N, pl);
+    if (INV) fft_lds_dif<T, true>(buf, w_valid, P, pl);
+    else fft_lds<T, false>(buf, w_valid, P, pl);
     for (int i = threadIdx.x; i < W * N; i += blockDim.x) {
         int w = i % W, y = i / W;
-        if (w < w_valid) base[(int64_t)y * H + kx0 + w] = buf[w * N + (INV ? pl.rev[y] : y)];
+        if (w < w_valid) base[(int64_t)y * H + kx0 + w] = buf[w * P + pidx(INV ? pl.rev[y] : y)];
     }
 }
 
@@ -293,12 +351,15 @@ __device__ __forceinline__ int shell_of(int64_t k2) {
 // partials row per block: [0, nek) shells of w_hat (mode 0) | [nek, 2nek) shells of u_hat |
 // [2nek] sum |w_hat|^2 weighted | [2nek+1] sum |u_hat|^2 weighted
 template <typename T>
-__global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl, int W, int nchunk, int mode, int nek,
+__global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchunk, int mode, int nek,
                         double *__restrict__ partials) {
-    extern __shared__ unsigned char smem_raw[];
-    C2<T> *buf = (C2<T> *)smem_raw;                       // 3 * W * N
-    double *shell = (double *)(buf + 3 * W * pl.N);       // 2 * nek + 2
-    const int N = pl.N, H = N / 2 + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2<T> *buf = (C2<T> *)smem_raw;                       // 3 * W * P
+    const int N = pl_g.N, H = N / 2 + 1, P = lds_pitch(N);
+    unsigned char *plan_area = smem_raw + sizeof(C2<T>) * (size_t)3 * W * P;
+    double *shell = (double *)(plan_area + plan_lds_bytes(N, sizeof(C2<T>)));   // 2 * nek + 2
+    const FftPlan pl = stage_plan<T>(pl_g, plan_area);
+    __syncthreads();
     const int64_t N3h = (int64_t)N * N * H;
     int chunk = blockIdx.x % nchunk;
     int y = (blockIdx.x / nchunk) % N;
@@ -311,10 +372,10 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl, int W, int nchunk,
         int z = rest % N, c = rest / N;
         C2<T> v = {0, 0};
         if (w < w_valid) v = spec[c * N3h + ((int64_t)z * N + y) * H + kx0 + w];
-        buf[(c * W + w) * N + pl.rev[z]] = v;
+        buf[(c * W + w) * P + pidx(pl.rev[z])] = v;
     }
     __syncthreads();
-    fft_lds<T, false>(buf, 3 * W, N, pl);
+    fft_lds<T, false>(buf, 3 * W, P, pl);
     // pointwise: (w, kz) pairs
     const int ky = wavenum(y, N);
     double e_w = 0.0, e_u = 0.0;
@@ -324,7 +385,8 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl, int W, int nchunk,
         int kx = wavenum(kxi, N), kz = wavenum(z, N);
         int64_t k2 = (int64_t)kx * kx + (int64_t)ky * ky + (int64_t)kz * kz;
         double wt = (kxi == 0 || kxi == N / 2) ? 1.0 : 2.0;   // half-spectrum multiplicity
-        C2<T> a = buf[(0 * W + w) * N + z], b = buf[(1 * W + w) * N + z], c = buf[(2 * W + w) * N + z];
+        const int zp = pidx(z);
+        C2<T> a = buf[(0 * W + w) * P + zp], b = buf[(1 * W + w) * P + zp], c = buf[(2 * W + w) * P + zp];
         double ew = (double)a.x * a.x + (double)a.y * a.y + (double)b.x * b.x + (double)b.y * b.y +
                     (double)c.x * c.x + (double)c.y * c.y;
         int sh = shell_of(k2);
@@ -352,9 +414,9 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl, int W, int nchunk,
                         (double)uz.x * uz.x + (double)uz.y * uz.y;
             e_u += wt * eu;
             if (sh >= 1 && sh <= nek) atomicAdd(shell + nek + sh - 1, 0.5 * wt * eu);
-            buf[(0 * W + w) * N + z] = ux;
-            buf[(1 * W + w) * N + z] = uy;
-            buf[(2 * W + w) * N + z] = uz;
+            buf[(0 * W + w) * P + zp] = ux;
+            buf[(1 * W + w) * P + zp] = uy;
+            buf[(2 * W + w) * P + zp] = uz;
         } else {
             e_u += wt * ew;
             if (sh >= 1 && sh <= nek) atomicAdd(shell + nek + sh - 1, 0.5 * wt * ew);
@@ -387,12 +449,12 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl, int W, int nchunk,
     for (int i = threadIdx.x; i < 2 * nek + 2; i += blockDim.x) row[i] = shell[i];
     if (mode != 0) return;
     __syncthreads();
-    fft_lds_dif<T, true>(buf, 3 * W, N, pl);
+    fft_lds_dif<T, true>(buf, 3 * W, P, pl);
     for (int i = threadIdx.x; i < 3 * W * N; i += blockDim.x) {
         int w = i % W, rest = i / W;
         int z = rest % N, c = rest / N;
         if (w < w_valid)
-            spec[c * N3h + ((int64_t)z * N + y) * H + kx0 + w] = buf[(c * W + w) * N + pl.rev[z]];
+            spec[c * N3h + ((int64_t)z * N + y) * H + kx0 + w] = buf[(c * W + w) * P + pidx(pl.rev[z])];
     }
 }
 
@@ -491,15 +553,51 @@ static const void *twiddles_for(wt_ctx *c, int N) {
 // Static LDS of k_fft_z (the fixed-order energy sums) left free when sizing its dynamic LDS.
 constexpr size_t kZStatic = 512;
 
-// Bytes of per-block partial rows the z pass of K instances of grid N needs.
-static size_t spectral_partials_bytes(wt_ctx *c, int N, int K, int nek) {
+// Tile shapes of the three passes: x lines (pairs) per block and threads, y columns per tile and
+// threads, z columns per tile and threads.
+struct FftTiles {
+    int ppb, tx, wy, ty, wz, tz;
+};
+
+static int lds_limit(wt_ctx *c) {
     int dev_lds = 65536;
     (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+    return dev_lds;
+}
+static size_t lds_x(int N, int ppb, size_t cs) { return (size_t)ppb * lds_pitch(N) * cs + plan_lds_bytes(N, cs); }
+static size_t lds_y(int N, int wy, size_t cs) { return (size_t)wy * lds_pitch(N) * cs + plan_lds_bytes(N, cs); }
+static size_t lds_z(int N, int wz, size_t cs, int nek) {
+    return 3 * (size_t)wz * lds_pitch(N) * cs + plan_lds_bytes(N, cs) + sizeof(double) * (2 * nek + 2);
+}
+
+// The shapes used without tuning: as many lines/columns as fit, up to 8192 points (x) or 16
+// columns (y, z), 256 threads.
+static FftTiles default_tiles(wt_ctx *c, int N, int nek) {
     const size_t cs = c->precision == 64 ? sizeof(C2<double>) : sizeof(C2<float>);
+    const size_t lim = (size_t)lds_limit(c);
+    FftTiles t{};
+    t.ppb = (int)(8192 / N) > 1 ? (int)(8192 / N) : 1;
+    while (t.ppb > 1 && lds_x(N, t.ppb, cs) > lim) t.ppb /= 2;
+    t.wy = 1;
+    while (t.wy < 16 && lds_y(N, 2 * t.wy, cs) <= lim) t.wy *= 2;
+    t.wz = 1;
+    while (t.wz < 16 && lds_z(N, 2 * t.wz, cs, nek) + kZStatic <= lim) t.wz *= 2;
+    t.tx = t.ty = t.tz = 256;
+    return t;
+}
+
+static FftTiles tiles_for(wt_ctx *c, int N, int nek) {
+    for (size_t i = 0; i + 7 <= c->fft_tiles.size(); i += 7)
+        if (c->fft_tiles[i] == N) {
+            const int32_t *v = &c->fft_tiles[i + 1];
+            return FftTiles{v[0], v[1], v[2], v[3], v[4], v[5]};
+        }
+    return default_tiles(c, N, nek);
+}
+
+// Bytes of per-block partial rows the z pass of K instances of grid N needs.
+static size_t spectral_partials_bytes(wt_ctx *c, int N, int K, int nek, int Wz) {
     const int H = N / 2 + 1;
-    size_t shell_bytes = sizeof(double) * (2 * nek + 2);
-    int Wz = 1;
-    while (Wz < 16 && 3 * (size_t)(2 * Wz) * N * cs + shell_bytes + kZStatic <= (size_t)dev_lds) Wz *= 2;
     const int ncz = (H + Wz - 1) / Wz;
     const int64_t zblocks = (int64_t)N * ncz * K;
     const int cols = 2 * nek + 2;
@@ -515,7 +613,7 @@ hipError_t spectral_reserve(wt_ctx *c) {
         int end = pos;
         while (end < n && c->inst[c->order[end]].N == c->inst[c->order[pos]].N) end++;
         const wt::Instance &I0 = c->inst[c->order[pos]];
-        need = std::max(need, spectral_partials_bytes(c, I0.N, end - pos, I0.nek));
+        need = std::max(need, spectral_partials_bytes(c, I0.N, end - pos, I0.nek, tiles_for(c, I0.N, I0.nek).wz));
         pos = end;
     }
     if (need <= c->partials_cap) return hipSuccess;
@@ -540,43 +638,37 @@ static hipError_t spectral_group(wt_ctx *c, int mode, int pos, int K, bool timed
     pl.tw = twiddles_for(c, N);
     if (!pl.tw) return hipErrorInvalidValue;
     pl.rev = (const int32_t *)((const char *)pl.tw + sizeof(C2<T>) * N);
-    int dev_lds = 65536;
-    (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
     const size_t cs = sizeof(C2<T>);
     T *field = (T *)(mode == 0 ? c->omega : c->u) + I0.field_off;
     C2<T> *spec = (C2<T> *)c->spec + I0.spec_off;
+    const int nek = I0.nek;
+    const FftTiles tl = tiles_for(c, N, nek);
     // x forward
-    int ppb = (int)(8192 / N) > 1 ? (int)(8192 / N) : 1;  // pairs per block
-    while ((size_t)ppb * N * cs > (size_t)dev_lds) ppb /= 2;
-    if (ppb < 1) ppb = 1;
+    const int ppb = tl.ppb;  // pairs per block
     const int64_t pairs_per_inst = (int64_t)3 * N * N / 2;
     const int64_t npairs = pairs_per_inst * K;
     unsigned gx = (unsigned)((npairs + ppb - 1) / ppb);
     T scale = (T)(1.0 / ((double)N * N * N));
-    size_t shx = (size_t)ppb * N * cs;
+    size_t shx = lds_x(N, ppb, cs);
     WT_HIP(hipFuncSetAttribute((const void *)k_fft_x_r2c<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shx));
     if (timed) kt_begin(c, kSlotFftX);
-    hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(gx), dim3(256), shx, st, (const T *)field, spec, pl, ppb, npairs, scale);
+    hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(gx), dim3(tl.tx), shx, st, (const T *)field, spec, pl, ppb, npairs, scale);
     if (timed) kt_end(c, kSlotFftX);
     WT_HIP(hipGetLastError());
     // y forward
-    int Wy = 1;
-    while (Wy < 16 && (size_t)(2 * Wy) * N * cs <= (size_t)dev_lds) Wy *= 2;
+    const int Wy = tl.wy;
     int ncy = (H + Wy - 1) / Wy;
-    size_t shy = (size_t)Wy * N * cs;
+    size_t shy = lds_y(N, Wy, cs);
     WT_HIP(hipFuncSetAttribute((const void *)k_fft_y<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shy));
     WT_HIP(hipFuncSetAttribute((const void *)k_fft_y<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shy));
     if (timed) kt_begin(c, kSlotFftY);
-    hipLaunchKernelGGL((k_fft_y<T, false>), dim3((unsigned)(3 * K * N * ncy)), dim3(256), shy, st, spec, pl, Wy, ncy);
+    hipLaunchKernelGGL((k_fft_y<T, false>), dim3((unsigned)(3 * K * N * ncy)), dim3(tl.ty), shy, st, spec, pl, Wy, ncy);
     if (timed) kt_end(c, kSlotFftY);
     WT_HIP(hipGetLastError());
     // z fused
-    const int nek = I0.nek;
-    size_t shell_bytes = sizeof(double) * (2 * nek + 2);
-    int Wz = 1;
-    while (Wz < 16 && 3 * (size_t)(2 * Wz) * N * cs + shell_bytes + kZStatic <= (size_t)dev_lds) Wz *= 2;
+    const int Wz = tl.wz;
     int ncz = (H + Wz - 1) / Wz;
-    size_t shz = 3 * (size_t)Wz * N * cs + shell_bytes;
+    size_t shz = lds_z(N, Wz, cs, nek);
     const int64_t zrows = (int64_t)N * ncz;    // blocks (partial rows) per instance
     const int64_t zblocks = zrows * K;
     const int cols = 2 * nek + 2;
@@ -589,7 +681,7 @@ static hipError_t spectral_group(wt_ctx *c, int mode, int pos, int K, bool timed
     double *colsum = c->partials + (size_t)zblocks * cols;
     WT_HIP(hipFuncSetAttribute((const void *)k_fft_z<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shz));
     if (timed) kt_begin(c, kSlotFftZ);
-    hipLaunchKernelGGL(k_fft_z<T>, dim3((unsigned)zblocks), dim3(256), shz, st, spec, pl, Wz, ncz, mode, nek,
+    hipLaunchKernelGGL(k_fft_z<T>, dim3((unsigned)zblocks), dim3(tl.tz), shz, st, spec, pl, Wz, ncz, mode, nek,
                        c->partials);
     if (timed) kt_end(c, kSlotFftZ);
     WT_HIP(hipGetLastError());
@@ -603,13 +695,13 @@ static hipError_t spectral_group(wt_ctx *c, int mode, int pos, int K, bool timed
     if (mode != 0) return hipSuccess;
     // y inverse
     if (timed) kt_begin(c, kSlotFftYInv);
-    hipLaunchKernelGGL((k_fft_y<T, true>), dim3((unsigned)(3 * K * N * ncy)), dim3(256), shy, st, spec, pl, Wy, ncy);
+    hipLaunchKernelGGL((k_fft_y<T, true>), dim3((unsigned)(3 * K * N * ncy)), dim3(tl.ty), shy, st, spec, pl, Wy, ncy);
     if (timed) kt_end(c, kSlotFftYInv);
     WT_HIP(hipGetLastError());
     // x inverse (C2R), scaled by 1/up, omega scaled too
     WT_HIP(hipFuncSetAttribute((const void *)k_fft_x_c2r<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shx));
     if (timed) kt_begin(c, kSlotFftXInv);
-    hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(gx), dim3(256), shx, st, (const C2<T> *)spec, (T *)c->u + I0.field_off,
+    hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(gx), dim3(tl.tx), shx, st, (const C2<T> *)spec, (T *)c->u + I0.field_off,
                        (T *)c->omega + I0.field_off, pl, ppb, npairs, (const DeviceScalars *)(c->scal + pos),
                        pairs_per_inst);
     if (timed) kt_end(c, kSlotFftXInv);
@@ -640,6 +732,140 @@ static hipError_t spectral_t(wt_ctx *c, int mode, int only) {
         pos = end;
     }
     return hipSuccess;
+}
+
+// ---- tile autotuning -----------------------------------------------------------------------
+// Times every candidate shape of the x (forward), y (forward) and z (Biot-Savart) passes of one
+// group on the context's own arenas and keeps the fastest of each. The x pass only reads omega;
+// y and z work in place on the half spectrum, whose contents the next x pass rewrites; z also
+// writes the partial rows. No result buffer (omega, u, scalars, spectra) is touched. The
+// inverse passes use the forward passes' shapes.
+template <typename T>
+static hipError_t tune_group(wt_ctx *c, int pos, int K, FftTiles &best, std::string &rep) {
+    hipStream_t st = c->stream;
+    const wt::Instance &I0 = c->inst[c->order[pos]];
+    const int N = I0.N, H = N / 2 + 1, nek = I0.nek;
+    FftPlan pl;
+    make_plan(N, pl);
+    pl.tw = twiddles_for(c, N);
+    if (!pl.tw) return hipErrorInvalidValue;
+    pl.rev = (const int32_t *)((const char *)pl.tw + sizeof(C2<T>) * N);
+    const size_t cs = sizeof(C2<T>), lim = (size_t)lds_limit(c);
+    T *field = (T *)c->omega + I0.field_off;
+    C2<T> *spec = (C2<T> *)c->spec + I0.spec_off;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    WT_HIP(hipEventCreate(&e0));
+    hipError_t err = hipEventCreate(&e1);
+    if (err != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return err;
+    }
+    auto timed = [&](auto launch, float *out) -> hipError_t {
+        float b = 1e30f;
+        for (int r = 0; r < 3; r++) {
+            WT_HIP(hipEventRecord(e0, st));
+            launch();
+            WT_HIP(hipGetLastError());
+            WT_HIP(hipEventRecord(e1, st));
+            WT_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            WT_HIP(hipEventElapsedTime(&ms, e0, e1));
+            b = ms < b ? ms : b;
+        }
+        *out = b;
+        return hipSuccess;
+    };
+    const int threads[3] = {256, 512, 1024};
+    best = default_tiles(c, N, nek);
+    float bx = 1e30f, by = 1e30f, bz = 1e30f, t = 0.f;
+    const int64_t npairs = (int64_t)3 * N * N / 2 * K;
+    const T scale = (T)(1.0 / ((double)N * N * N));
+    for (int ppb = 1; ppb <= 64 && err == hipSuccess; ppb *= 2) {
+        const size_t sh = lds_x(N, ppb, cs);
+        if (sh > lim || (int64_t)ppb * N > 32768) continue;
+        err = hipFuncSetAttribute((const void *)k_fft_x_r2c<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        const unsigned gx = (unsigned)((npairs + ppb - 1) / ppb);
+        for (int tx : threads) {
+            if (err != hipSuccess) break;
+            err = timed([&] { hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(gx), dim3(tx), sh, st, (const T *)field, spec, pl,
+                                                 ppb, npairs, scale); }, &t);
+            if (err == hipSuccess && t < bx) { bx = t; best.ppb = ppb; best.tx = tx; }
+        }
+    }
+    for (int wy = 1; wy <= 16 && err == hipSuccess; wy *= 2) {
+        const size_t sh = lds_y(N, wy, cs);
+        if (sh > lim) continue;
+        err = hipFuncSetAttribute((const void *)k_fft_y<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        const int ncy = (H + wy - 1) / wy;
+        for (int ty : threads) {
+            if (err != hipSuccess) break;
+            err = timed([&] { hipLaunchKernelGGL((k_fft_y<T, false>), dim3((unsigned)(3 * K * N * ncy)), dim3(ty), sh, st,
+                                                 spec, pl, wy, ncy); }, &t);
+            if (err == hipSuccess && t < by) { by = t; best.wy = wy; best.ty = ty; }
+        }
+    }
+    // partial rows for the narrowest candidate
+    if (err == hipSuccess) {
+        const size_t need = spectral_partials_bytes(c, N, K, nek, 1);
+        if (need > c->partials_cap) {
+            err = hipStreamSynchronize(st);
+            if (c->partials) (void)hipFree(c->partials);
+            c->partials = nullptr;
+            c->partials_cap = 0;
+            if (err == hipSuccess) err = hipMalloc(&c->partials, need);
+            if (err == hipSuccess) c->partials_cap = need;
+        }
+    }
+    for (int wz = 1; wz <= 16 && err == hipSuccess; wz *= 2) {
+        const size_t sh = lds_z(N, wz, cs, nek);
+        if (sh + kZStatic > lim) continue;
+        err = hipFuncSetAttribute((const void *)k_fft_z<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        const int ncz = (H + wz - 1) / wz;
+        const int64_t zblocks = (int64_t)N * ncz * K;
+        for (int tz : threads) {
+            if (err != hipSuccess) break;
+            err = timed([&] { hipLaunchKernelGGL(k_fft_z<T>, dim3((unsigned)zblocks), dim3(tz), sh, st, spec, pl, wz, ncz,
+                                                 0, nek, c->partials); }, &t);
+            if (err == hipSuccess && t < bz) { bz = t; best.wz = wz; best.tz = tz; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err != hipSuccess) return err;
+    char line[256];
+    snprintf(line, sizeof line, "N=%d x(%d lines, %d threads) %.3f ms; y(%d cols, %d threads) %.3f ms; z(%d cols, %d threads) %.3f ms\n",
+             N, best.ppb, best.tx, bx, best.wy, best.ty, by, best.wz, best.tz, bz);
+    rep += line;
+    return hipSuccess;
+}
+
+hipError_t spectral_autotune(wt_ctx *c, const std::vector<int32_t> &cached, std::string &report) {
+    const int n = c->ninst();
+    for (int pos = 0; pos < n;) {
+        int end = pos;
+        while (end < n && c->inst[c->order[end]].N == c->inst[c->order[pos]].N) end++;
+        const int N = c->inst[c->order[pos]].N;
+        bool have = false;
+        for (size_t i = 0; i + 7 <= cached.size(); i += 7)
+            if (cached[i] == N) {
+                for (size_t j = 0; j + 7 <= c->fft_tiles.size(); j += 7)
+                    if (c->fft_tiles[j] == N) c->fft_tiles.erase(c->fft_tiles.begin() + j, c->fft_tiles.begin() + j + 7);
+                c->fft_tiles.insert(c->fft_tiles.end(), cached.begin() + i, cached.begin() + i + 7);
+                have = true;
+                report += "N=" + std::to_string(N) + " from cache\n";
+            }
+        if (!have) {
+            FftTiles b{};
+            WT_HIP(c->precision == 64 ? tune_group<double>(c, pos, end - pos, b, report)
+                                      : tune_group<float>(c, pos, end - pos, b, report));
+            for (size_t j = 0; j + 7 <= c->fft_tiles.size(); j += 7)
+                if (c->fft_tiles[j] == N) c->fft_tiles.erase(c->fft_tiles.begin() + j, c->fft_tiles.begin() + j + 7);
+            const int32_t v[7] = {N, b.ppb, b.tx, b.wy, b.ty, b.wz, b.tz};
+            c->fft_tiles.insert(c->fft_tiles.end(), v, v + 7);
+        }
+        pos = end;
+    }
+    return spectral_reserve(c);
 }
 
 hipError_t launch_spectral(wt_ctx *c, int mode, int only) {

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <new>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "wt_internal.h"
@@ -536,6 +537,88 @@ static hipError_t capture_spectral(wt_ctx *c) {
     c->spec_graph = g;
     c->spec_exec = x;
     return hipSuccess;
+}
+
+// Autotune cache file: one JSON object, key "<arch>:<precision>:<N>" -> {"ppb": .., "tx": ..,
+// "wy": .., "ty": .., "wz": .., "tz": ..}. Entries of other devices, precisions and sizes are kept.
+static const char *kTileFields[6] = {"ppb", "tx", "wy", "ty", "wz", "tz"};
+
+static std::vector<std::pair<std::string, std::vector<int>>> read_tile_cache(const char *path) {
+    std::vector<std::pair<std::string, std::vector<int>>> out;
+    FILE *f = fopen(path, "rb");
+    if (!f) return out;
+    std::string txt;
+    char buf[4096];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof buf, f)) > 0) txt.append(buf, n);
+    fclose(f);
+    size_t p = txt.find('{');
+    while (p != std::string::npos) {
+        size_t k0 = txt.find('"', p + 1);
+        if (k0 == std::string::npos) break;
+        size_t k1 = txt.find('"', k0 + 1);
+        size_t b0 = txt.find('{', k1);
+        size_t b1 = txt.find('}', b0);
+        if (k1 == std::string::npos || b0 == std::string::npos || b1 == std::string::npos) break;
+        std::string key = txt.substr(k0 + 1, k1 - k0 - 1), body = txt.substr(b0, b1 - b0);
+        std::vector<int> v;
+        for (const char *fld : kTileFields) {
+            size_t q = body.find(std::string("\"") + fld + "\"");
+            int x = 0;
+            if (q == std::string::npos || sscanf(body.c_str() + body.find(':', q) + 1, "%d", &x) != 1) break;
+            v.push_back(x);
+        }
+        if (v.size() == 6) out.emplace_back(key, v);
+        p = b1;
+    }
+    return out;
+}
+
+int wt_autotune(wt_ctx *c, const char *cache_path, char *report, int32_t report_cap) {
+    CHECK_CTX(c);
+    hipDeviceProp_t prop;
+    HIPCALL(c, hipGetDeviceProperties(&prop, c->device), "device properties");
+    std::string arch = prop.gcnArchName;
+    if (arch.find(':') != std::string::npos) arch = arch.substr(0, arch.find(':'));
+    const std::string pre = arch + ":" + std::to_string(c->precision) + ":";
+    auto entries = cache_path ? read_tile_cache(cache_path) : decltype(read_tile_cache(nullptr)){};
+    std::vector<int32_t> cached;
+    for (auto &e : entries)
+        if (e.first.compare(0, pre.size(), pre) == 0) {
+            cached.push_back(atoi(e.first.c_str() + pre.size()));
+            cached.insert(cached.end(), e.second.begin(), e.second.end());
+        }
+    std::string rep;
+    hipError_t e = wt::spectral_autotune(c, cached, rep);
+    if (e != hipSuccess) return hip_err(c, e, "autotune");
+    // the captured spectral stage used the old shapes
+    if (c->spec_exec) (void)hipGraphExecDestroy(c->spec_exec);
+    if (c->spec_graph) (void)hipGraphDestroy(c->spec_graph);
+    c->spec_exec = nullptr;
+    c->spec_graph = nullptr;
+    c->have_spectra = false;
+    if (cache_path) {
+        for (size_t i = 0; i + 7 <= c->fft_tiles.size(); i += 7) {
+            const std::string key = pre + std::to_string(c->fft_tiles[i]);
+            std::vector<int> v(c->fft_tiles.begin() + i + 1, c->fft_tiles.begin() + i + 7);
+            bool found = false;
+            for (auto &en : entries)
+                if (en.first == key) { en.second = v; found = true; }
+            if (!found) entries.emplace_back(key, v);
+        }
+        FILE *f = fopen(cache_path, "wb");
+        if (!f) return set_err(c, WT_EINVAL, "cannot write the autotune cache %s", cache_path);
+        fprintf(f, "{\n");
+        for (size_t i = 0; i < entries.size(); i++) {
+            fprintf(f, "  \"%s\": {", entries[i].first.c_str());
+            for (int j = 0; j < 6; j++) fprintf(f, "%s\"%s\": %d", j ? ", " : "", kTileFields[j], entries[i].second[j]);
+            fprintf(f, "}%s\n", i + 1 < entries.size() ? "," : "");
+        }
+        fprintf(f, "}\n");
+        fclose(f);
+    }
+    if (report && report_cap > 0) snprintf(report, (size_t)report_cap, "%s", rep.c_str());
+    return WT_OK;
 }
 
 int wt_set_graph(wt_ctx *c, int32_t enable) {

@@ -156,6 +156,9 @@ struct wt_ctx {
     bool capturing = false;
     hipGraph_t spec_graph = nullptr;
     hipGraphExec_t spec_exec = nullptr;
+    // FFT tile shapes chosen by wt_autotune, 7 ints per grid size: N, x lines per block, x threads,
+    // y columns per tile, y threads, z columns per tile, z threads (defaults for other N)
+    std::vector<int32_t> fft_tiles;
     // batched structure functions: work items and per-position separations
     void *sf_items = nullptr;
     int64_t sf_nitems = 0;
@@ -188,6 +191,10 @@ hipError_t launch_deposit(wt_ctx *c);
 // resident velocity of instance `only`
 hipError_t launch_spectral(wt_ctx *c, int mode, int only);
 hipError_t spectral_reserve(wt_ctx *c);
+// Times the candidate tile shapes of the x, y and z passes for every grid size of the context and
+// keeps the fastest in c->fft_tiles; `cached` (7 ints per N, may be empty) is used instead of
+// measuring for the sizes it holds. `report` receives one line per grid size.
+hipError_t spectral_autotune(wt_ctx *c, const std::vector<int32_t> &cached, std::string &report);
 bool fft_size_supported(int N);
 void fft_reversal(int N, std::vector<int32_t> &rev);
 // stats.hip: structure functions of instance c->cur

@@ -171,6 +171,14 @@ int wt_batch_statistics(wt_ctx *ctx, int32_t pmin, int32_t pmax, int32_t nsep, c
  * the first eager run, launched as one graph afterwards (no counterpart in the reference). */
 int wt_set_graph(wt_ctx *ctx, int32_t enable);
 
+/* Autotuner (no counterpart in the reference): times the candidate tile shapes of the FFT passes
+ * (x lines per block, y and z columns per tile, threads per block) for every grid size of the
+ * context on the device and keeps the fastest for later calls. With cache_path, shapes already
+ * recorded there for this GPU architecture and precision are reused without measuring, and the
+ * chosen shapes are written back (JSON). report (may be NULL) receives one line per grid size.
+ * The half spectrum is overwritten; call before wt_generate / wt_biot_savart. */
+int wt_autotune(wt_ctx *ctx, const char *cache_path, char *report, int32_t report_cap);
+
 /* Copy a resident field component to/from host memory, Fortran order, FP64 or FP32 host
  * buffer (output_v/output_puredata copy-out, subroutine.f90:1342-1399). wt_set_field replaces
  * the resident omega or u (used to drive wt_biot_savart / statistics on a given field). */
