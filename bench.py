@@ -256,6 +256,54 @@ def ensemble_bench(steps: int, warmup: int, n: int = 64, graph: bool = True, til
             "features_first": feats[0].round(4).tolist()}
 
 
+def c3_bench(densities=(0.5, 1.0, 2.0), tile_cache: str | None = None) -> dict:
+    """The 1024^3 workload: 8 levels (r = 2^-l), density swept over `densities`, fields resident
+    on the GPU, structure functions of orders 2-6 of u_x at every separation 1..512."""
+    from fastweavturb_amd import fbb
+    from fastweavturb_amd.api import WovenGPU
+    g = WovenGPU(1024, 32)
+    log("[c3] autotune: " + g.autotune(tile_cache).strip().replace("\n", "; "))
+    seps = np.arange(1, 513, dtype=np.int32)
+    runs = []
+    for d in densities:
+        t0 = time.time()
+        p, t = fbb.config_c3(d)
+        log(f"[c3] density {d}: {t.nline} tubes, {t.npoint_all} points (built in {time.time() - t0:.1f}s)")
+        g.set_params(p)
+        g.set_tubes(t)
+        t0 = time.perf_counter()
+        etot, up, ens = g.generate()
+        t_gen = time.perf_counter() - t0
+        E = g.spectrum(1)
+        t0 = time.perf_counter()
+        S = g.structure_functions(1, 2, 6, seps)
+        t_sf = time.perf_counter() - t0
+        free, total = _device_memory()
+        dep = g.deposit_stats()
+        row = {"density": d, "tubes": t.nline, "points": int(t.npoint_all), "generate_s": t_gen,
+               "mpts_per_s": 1024 ** 3 / t_gen / 1e6, "stages_ms": g.stage_times()[:4].tolist(),
+               "sf_s": t_sf, "hits": int(dep["hits"]), "pairs": int(dep["pairs"]), "up": up,
+               "energy_in_shells": float(E.sum()), "F4_at_1": float(S[2, 0] / S[0, 0] ** 2),
+               "F4_at_512": float(S[2, -1] / S[0, -1] ** 2), "device_used_gb": (total - free) / 1e9}
+        log(f"[c3] {json.dumps(row)}")
+        runs.append(row)
+    g.close()
+    return {"metric": "Mpts/s woven field at 1024^3 FP32 (8 levels)", "workload": "c3_1024_fp32", "runs": runs}
+
+
+def _device_memory():
+    """(free, total) bytes of device 0, read through the HIP runtime the library uses."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        free, total = ctypes.c_size_t(), ctypes.c_size_t()
+        if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0:
+            return free.value, total.value
+    except OSError:
+        pass
+    return 0, 0
+
+
 def copy_probe(nbytes: int = 1 << 30, reps: int = 3):
     """A plain elementwise copy of known size: each launch reads and writes exactly nbytes."""
     import torch
@@ -287,7 +335,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c2", "c1", "small", "c4"])
+    ap.add_argument("--config", default="c2", choices=["c2", "c1", "small", "c4", "c3"])
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ttt-targets", type=int, default=4,
@@ -312,6 +360,9 @@ def main():
         return
     if args.rocfft_only:
         print(json.dumps(rocfft_compare(args.rocfft_only)), flush=True)
+        return
+    if args.config == "c3":
+        print(json.dumps(c3_bench(tile_cache=args.tile_cache)), flush=True)
         return
     if args.config == "c4":
         print(json.dumps(ensemble_bench(args.steps, args.warmup, tile_cache=args.tile_cache)), flush=True)

@@ -217,3 +217,12 @@ def woven_levels(N: int, levels: list, seed: int, gamma_jitter: float = 0.2, wig
                   ntf_pre=ntf_pre, Rtube_ratio=Rtube_ratio, dzeta_sup=dzeta_sup)
     tb.validate()
     return p, tb
+
+
+def config_c3(density: float = 1.0, seed: int = 1234, N: int = 1024, nlev: int = 8, max_points: int = 1024):
+    """1024^3: nlev levels, r = 2^-l, round(density * 4 * 4^l) tubes, min(256 * 2^l, max_points) points
+    per tube (the cap keeps the centerlines at ~90M points), H = 0.75, Gamma ~ r^(4/3)."""
+    levels = [LevelSpec(2.0 ** -l, max(1, int(round(density * 4 * 4 ** l))), int(min(256 * 2 ** l, max_points)),
+                        0.75, (2.0 ** -l) ** (4.0 / 3.0)) for l in range(nlev)]
+    return woven_levels(N, levels, seed)
+
