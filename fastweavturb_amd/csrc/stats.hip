@@ -3,10 +3,9 @@
 // mode 0 (Sp_calculate): S_p(r) = (1/3N^3) sum over all points of
 //   (ux(i+r,j,k)-ux)^p + (uy(i,j+r,k)-uy)^p + (uz(i,j,k+r)-uz)^p, p = 1..pmax.
 // mode 1 (x-only, getStructureFunc_x generalised): S_p(r) = (1/N^3) sum (ux(i+r)-ux(i))^p,
-//   p = pmin..pmax, x-lines staged in LDS.
-// Increments are formed in the field's precision; powers and sums in FP64. Persistent blocks
-// accumulate per-(separation, order) sums in LDS and write one partial row each; the rows are
-// reduced in a fixed order, so results are reproducible.
+//   p = pmin..pmax, x-lines staged in LDS, one separation per thread.
+// Increments are formed in the field's precision; powers and sums in FP64. Blocks write one
+// partial row each and the rows are reduced in a fixed order, so results are reproducible.
 #include <algorithm>
 #include <vector>
 
@@ -39,44 +38,91 @@ __device__ __forceinline__ void block_accumulate(const double *acc, int np, doub
     __syncthreads();
 }
 
-template <typename T>
-__global__ __launch_bounds__(kSfThreads) void k_sf_x(const T *__restrict__ ux, int N, int64_t nlines, int lines_per_group,
-                                                     int pmin, int np, int nsep, const int32_t *__restrict__ seps,
-                                                     double *__restrict__ partials) {
-    extern __shared__ unsigned char smem_raw[];
-    double *lds_acc = (double *)smem_raw;                       // nsep * np
-    double *scratch = lds_acc + (size_t)nsep * np;               // 4 * kMaxOrder
-    T *line = (T *)(scratch + 4 * kMaxOrder);                    // lines_per_group * N
-    for (int i = threadIdx.x; i < nsep * np; i += blockDim.x) lds_acc[i] = 0.0;
-    __syncthreads();
-    const int64_t ngroups = (nlines + lines_per_group - 1) / lines_per_group;
-    for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-        int64_t l0 = grp * lines_per_group;
-        int nl = (int)((nlines - l0) < lines_per_group ? (nlines - l0) : lines_per_group);
-        for (int i = threadIdx.x; i < nl * N; i += blockDim.x) line[i] = ux[l0 * N + i];
-        __syncthreads();
-        const int npts = nl * N;
-        for (int m = 0; m < nsep; m++) {
-            int s = ((seps[m] % N) + N) % N;
-            double acc[kMaxOrder];
-            for (int p = 0; p < np; p++) acc[p] = 0.0;
-            for (int i = threadIdx.x; i < npts; i += blockDim.x) {
-                int li = i / N, x = i - li * N;
-                int xs = x + s;
-                if (xs >= N) xs -= N;
-                double d = (double)line[li * N + xs] - (double)line[i];
-                double pw = 1.0;
-                for (int p = 1; p < pmin; p++) pw *= d;
-                for (int p = 0; p < np; p++) {
-                    pw *= d;
-                    acc[p] += pw;
-                }
+// x-increments with one separation per thread: thread t takes separation m = t % nsep and point
+// phase g = t / nsep (G = blockDim / nsep phases), accumulates the orders' power sums of its
+// separation over the points x = g, g + G, ... of every staged line of every group the block
+// takes, in registers, and the phases are summed once per block in a fixed order. Lanes of a wave
+// read the same base point (an LDS broadcast) and consecutive shifted points.
+template <typename T, int NP>
+__device__ __forceinline__ void sf_accumulate_n(const T *line, int nl, int N, int s, int g, int G, int pmin,
+                                                double *acc) {
+    for (int li = 0; li < nl; li++) {
+        const T *ln = line + li * N;
+        for (int x = g; x < N; x += G) {
+            int xs = x + s;
+            if (xs >= N) xs -= N;
+            const double d = (double)ln[xs] - (double)ln[x];
+            double pw = 1.0;
+            for (int p = 1; p < pmin; p++) pw *= d;
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                pw *= d;
+                acc[p] += pw;
             }
-            block_accumulate(acc, np, lds_acc + (size_t)m * np, scratch);
         }
     }
-    double *row = partials + (int64_t)blockIdx.x * nsep * np;
-    for (int i = threadIdx.x; i < nsep * np; i += blockDim.x) row[i] = lds_acc[i];
+}
+
+// The order count as a compile-time constant, so that only the requested powers are formed.
+template <typename T>
+__device__ __forceinline__ void sf_accumulate(const T *line, int nl, int N, int s, int g, int G, int pmin, int np,
+                                              double *acc) {
+    switch (np) {
+        case 1: sf_accumulate_n<T, 1>(line, nl, N, s, g, G, pmin, acc); break;
+        case 2: sf_accumulate_n<T, 2>(line, nl, N, s, g, G, pmin, acc); break;
+        case 3: sf_accumulate_n<T, 3>(line, nl, N, s, g, G, pmin, acc); break;
+        case 4: sf_accumulate_n<T, 4>(line, nl, N, s, g, G, pmin, acc); break;
+        case 5: sf_accumulate_n<T, 5>(line, nl, N, s, g, G, pmin, acc); break;
+        case 6: sf_accumulate_n<T, 6>(line, nl, N, s, g, G, pmin, acc); break;
+        case 7: sf_accumulate_n<T, 7>(line, nl, N, s, g, G, pmin, acc); break;
+        case 8: sf_accumulate_n<T, 8>(line, nl, N, s, g, G, pmin, acc); break;
+        case 9: sf_accumulate_n<T, 9>(line, nl, N, s, g, G, pmin, acc); break;
+        case 10: sf_accumulate_n<T, 10>(line, nl, N, s, g, G, pmin, acc); break;
+        case 11: sf_accumulate_n<T, 11>(line, nl, N, s, g, G, pmin, acc); break;
+        default: sf_accumulate_n<T, 12>(line, nl, N, s, g, G, pmin, acc); break;
+    }
+}
+
+// Sums the G phases' accumulators into out[m*np + p] (all threads call it).
+__device__ __forceinline__ void sf_reduce_phases(const double *acc, double *red, int nsep, int np, int m, int g,
+                                                 bool active, int G, double *out) {
+    if (active)
+#pragma unroll
+        for (int p = 0; p < kMaxOrder; p++)
+            if (p < np) red[((size_t)g * nsep + m) * np + p] = acc[p];
+    __syncthreads();
+    for (int i = threadIdx.x; i < nsep * np; i += blockDim.x) {
+        double v = 0.0;
+        for (int q = 0; q < G; q++) v += red[(size_t)q * nsep * np + i];
+        out[i] = v;
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_sf_x_sep(const T *__restrict__ ux, int N, int64_t nlines, int lpg, int pmin,
+                                                   int np, int nsep, const int32_t *__restrict__ seps, int row_stride,
+                                                   int col0, double *__restrict__ partials) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double *red = (double *)smem_raw;                                   // G * nsep * np
+    const int G = (int)blockDim.x / nsep > 0 ? (int)blockDim.x / nsep : 1;
+    T *line = (T *)(red + (size_t)G * nsep * np);                      // lpg * N
+    const int m = threadIdx.x % nsep, g = threadIdx.x / nsep;
+    const bool active = g < G;
+    const int s = ((seps[m] % N) + N) % N;
+    double acc[kMaxOrder];
+#pragma unroll
+    for (int p = 0; p < kMaxOrder; p++) acc[p] = 0.0;
+    const int64_t ngroups = (nlines + lpg - 1) / lpg;
+    for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        const int64_t l0 = grp * lpg;
+        const int nl = (int)((nlines - l0) < lpg ? (nlines - l0) : lpg);
+        for (int i = threadIdx.x; i < nl * N; i += blockDim.x) line[i] = ux[l0 * N + i];
+        __syncthreads();
+        if (active) sf_accumulate(line, nl, N, s, g, G, pmin, np, acc);
+        __syncthreads();
+    }
+    sf_reduce_phases(acc, red, nsep, np, m, g, active, G, partials + (int64_t)blockIdx.x * row_stride + col0);
 }
 
 template <typename T>
@@ -150,17 +196,28 @@ static hipError_t sf_t(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const 
     int64_t blocks;
     double *partials = nullptr;
     if (mode == 1) {
-        int lpg = 8192 / N > 0 ? 8192 / N : 1;
-        while (lpg > 1 && acc_bytes + (size_t)lpg * N * sizeof(T) > (size_t)dev_lds) lpg /= 2;
-        size_t sh = acc_bytes + (size_t)lpg * N * sizeof(T);
-        if (sh > (size_t)dev_lds) return hipErrorInvalidValue;
-        int64_t nlines = (int64_t)N * N;
-        int64_t ngroups = (nlines + lpg - 1) / lpg;
-        blocks = ngroups < (int64_t)cus * 4 ? ngroups : (int64_t)cus * 4;
+        // separations in chunks of at most 1024, one thread per separation (and point phase)
+        const int64_t nlines = (int64_t)N * N;
+        blocks = (int64_t)cus * 2;
         WT_HIP(hipMallocAsync((void **)&partials, sizeof(double) * blocks * nsep * np, st));
-        WT_HIP(hipFuncSetAttribute((const void *)k_sf_x<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-        hipLaunchKernelGGL(k_sf_x<T>, dim3((unsigned)blocks), dim3(kSfThreads), sh, st, u, N, nlines, lpg, pmin, np,
-                           nsep, d_seps, partials);
+        // rows of blocks beyond a chunk's group count stay zero
+        WT_HIP(hipMemsetAsync(partials, 0, sizeof(double) * blocks * nsep * np, st));
+        for (int s0 = 0; s0 < nsep; s0 += 1024) {
+            const int nc = nsep - s0 < 1024 ? nsep - s0 : 1024;
+            const int threads = nc <= 256 ? 256 : (nc + 63) / 64 * 64;
+            const int G = threads / nc;
+            const size_t red = sizeof(double) * (size_t)G * nc * np;
+            int lpg = 8192 / N > 0 ? 8192 / N : 1;
+            while (lpg > 1 && red + (size_t)lpg * N * sizeof(T) > (size_t)dev_lds) lpg /= 2;
+            const size_t sh = red + (size_t)lpg * N * sizeof(T);
+            if (sh > (size_t)dev_lds) return hipErrorInvalidValue;
+            const int64_t ngroups = (nlines + lpg - 1) / lpg;
+            const int64_t b = ngroups < blocks ? ngroups : blocks;
+            WT_HIP(hipFuncSetAttribute((const void *)k_sf_x_sep<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+            hipLaunchKernelGGL(k_sf_x_sep<T>, dim3((unsigned)b), dim3(threads), sh, st, u, N, nlines, lpg, pmin, np, nc,
+                               d_seps + s0, nsep * np, s0 * np, partials);
+            WT_HIP(hipGetLastError());
+        }
     } else {
         if (acc_bytes > (size_t)dev_lds) return hipErrorInvalidValue;
         int rpg = kSfThreads * 8 / N > 0 ? kSfThreads * 8 / N : 1;
@@ -208,44 +265,31 @@ struct SfInst {
 constexpr int kSfItemPoints = 8192;
 
 template <typename T>
-__global__ __launch_bounds__(kSfThreads) void k_sf_x_batch(const T *__restrict__ u, const SfInst *__restrict__ inst,
-                                                           const SfItem *__restrict__ items, int64_t nitems, int pmin,
-                                                           int np, int nsep, const int32_t *__restrict__ seps,
-                                                           double *__restrict__ partials) {
-    extern __shared__ unsigned char smem_raw[];
-    double *lds_acc = (double *)smem_raw;                       // nsep * np
-    double *scratch = lds_acc + (size_t)nsep * np;               // 4 * kMaxOrder
-    T *line = (T *)(scratch + 4 * kMaxOrder);                    // kSfItemPoints
+__global__ __launch_bounds__(1024) void k_sf_x_batch(const T *__restrict__ u, const SfInst *__restrict__ inst,
+                                                     const SfItem *__restrict__ items, int64_t nitems, int pmin,
+                                                     int np, int nsep, const int32_t *__restrict__ seps,
+                                                     double *__restrict__ partials) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double *red = (double *)smem_raw;                                   // G * nsep * np
+    const int G = (int)blockDim.x / nsep > 0 ? (int)blockDim.x / nsep : 1;
+    T *line = (T *)(red + (size_t)G * nsep * np);                      // kSfItemPoints
+    const int m = threadIdx.x % nsep, g = threadIdx.x / nsep;
+    const bool active = g < G;
     for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
         const SfItem w = items[it];
         const SfInst I = inst[w.pos];
         const int N = I.N;
-        for (int i = threadIdx.x; i < nsep * np; i += blockDim.x) lds_acc[i] = 0.0;
         const T *src = u + I.field_off + w.line0 * N;             // u_x: first component
-        const int npts = w.nl * N;
-        for (int i = threadIdx.x; i < npts; i += blockDim.x) line[i] = src[i];
+        for (int i = threadIdx.x; i < w.nl * N; i += blockDim.x) line[i] = src[i];
         __syncthreads();
-        for (int m = 0; m < nsep; m++) {
+        double acc[kMaxOrder];
+#pragma unroll
+        for (int p = 0; p < kMaxOrder; p++) acc[p] = 0.0;
+        if (active) {
             const int s = ((seps[(int64_t)w.pos * nsep + m] % N) + N) % N;
-            double acc[kMaxOrder];
-            for (int p = 0; p < np; p++) acc[p] = 0.0;
-            for (int i = threadIdx.x; i < npts; i += blockDim.x) {
-                const int li = i / N, x = i - li * N;
-                int xs = x + s;
-                if (xs >= N) xs -= N;
-                const double d = (double)line[li * N + xs] - (double)line[i];
-                double pw = 1.0;
-                for (int p = 1; p < pmin; p++) pw *= d;
-                for (int p = 0; p < np; p++) {
-                    pw *= d;
-                    acc[p] += pw;
-                }
-            }
-            block_accumulate(acc, np, lds_acc + (size_t)m * np, scratch);
+            sf_accumulate(line, w.nl, N, s, g, G, pmin, np, acc);
         }
-        double *row = partials + it * nsep * np;
-        for (int i = threadIdx.x; i < nsep * np; i += blockDim.x) row[i] = lds_acc[i];
-        __syncthreads();
+        sf_reduce_phases(acc, red, nsep, np, m, g, active, G, partials + it * nsep * np);
     }
 }
 
@@ -308,13 +352,15 @@ static hipError_t sf_batch_t(wt_ctx *c, int pmin, int pmax, int nsep, const int3
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
     int dev_lds = 65536;
     (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
-    const size_t sh = sizeof(double) * ((size_t)nsep * np + 4 * kMaxOrder) + sizeof(T) * kSfItemPoints;
+    if (nsep > 1024) return hipErrorInvalidValue;
+    const int threads = nsep <= 256 ? 256 : (nsep + 63) / 64 * 64;
+    const size_t sh = sizeof(double) * (size_t)(threads / nsep) * nsep * np + sizeof(T) * kSfItemPoints;
     if (sh > (size_t)dev_lds) return hipErrorInvalidValue;
     const int64_t blocks = std::min<int64_t>(c->sf_nitems, (int64_t)cus * 4);
     double *partials = nullptr;
     WT_HIP(hipMallocAsync((void **)&partials, sizeof(double) * c->sf_nitems * nsep * np, st));
     WT_HIP(hipFuncSetAttribute((const void *)k_sf_x_batch<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    hipLaunchKernelGGL(k_sf_x_batch<T>, dim3((unsigned)blocks), dim3(kSfThreads), sh, st, (const T *)c->u, d_inst,
+    hipLaunchKernelGGL(k_sf_x_batch<T>, dim3((unsigned)blocks), dim3(threads), sh, st, (const T *)c->u, d_inst,
                        d_items, c->sf_nitems, pmin, np, nsep, d_seps, partials);
     WT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sf_reduce_items, dim3((unsigned)((nsep * np + 63) / 64), (unsigned)npos), dim3(64), 0, st,
