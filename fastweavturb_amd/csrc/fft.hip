@@ -2,7 +2,7 @@
 // 1550-1569, 1604-1638) with a hand-written pencil FFT.
 //
 // Transform layout: the half spectrum (kx = 0..N/2) of each component, spec[c][z][y][kx]
-// with row stride N/2+1, complex<T>. Passes:
+// with the row stride padded to whole 128-byte lines (spec_row), complex<T>. Passes:
 //   x : R2C of two real lines at once packed as one complex line, scaled by 1/N^3
 //   y : C2C along y on tiles of W kx-columns staged in LDS
 //   z : C2C forward along z, then u_hat = i k~ x w_hat / k^2 (k~ = k with Nyquist components
@@ -50,6 +50,13 @@ struct FftPlan {
 __host__ __device__ constexpr size_t plan_lds_bytes(int N, size_t cs) {
     return ((size_t)N * cs + (size_t)N * sizeof(int32_t) + 15) / 16 * 16;
 }
+// Row stride of the half spectrum in HBM: N/2 + 1 columns rounded up to whole 128-byte lines, so
+// the column tiles of the y and z passes start on line boundaries.
+template <typename T>
+__host__ __device__ constexpr int spec_row(int N) {
+    return (N / 2 + 1 + (int)(128 / sizeof(C2<T>)) - 1) / (int)(128 / sizeof(C2<T>)) * (int)(128 / sizeof(C2<T>));
+}
+
 template <typename T>
 __device__ __forceinline__ FftPlan stage_plan(const FftPlan &pl, unsigned char *area) {
     C2<T> *tw = (C2<T> *)area;
@@ -246,7 +253,7 @@ __global__ void k_fft_x_r2c(const T *__restrict__ f, C2<T> *__restrict__ spec, F
                             int64_t npairs, T scale) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = (C2<T> *)smem_raw;
-    const int N = pl_g.N, H = N / 2 + 1, P = lds_pitch(N);
+    const int N = pl_g.N, H = N / 2 + 1, P = lds_pitch(N), Hs = spec_row<T>(N);
     const FftPlan pl = stage_plan<T>(pl_g, smem_raw + sizeof(C2<T>) * (size_t)pairs_per_block * P);
     __syncthreads();
     const int64_t pair0 = (int64_t)blockIdx.x * pairs_per_block;
@@ -267,8 +274,8 @@ __global__ void k_fft_x_r2c(const T *__restrict__ f, C2<T> *__restrict__ spec, F
         C2<T> A = {(z.x + zc.x) * (T)0.5, (z.y - zc.y) * (T)0.5};
         C2<T> B = {(z.y + zc.y) * (T)0.5, (zc.x - z.x) * (T)0.5};
         int64_t line = 2 * (pair0 + pp);
-        spec[line * H + k] = C2<T>{A.x * scale, A.y * scale};
-        spec[(line + 1) * H + k] = C2<T>{B.x * scale, B.y * scale};
+        spec[line * Hs + k] = C2<T>{A.x * scale, A.y * scale};
+        spec[(line + 1) * Hs + k] = C2<T>{B.x * scale, B.y * scale};
     }
 }
 
@@ -279,7 +286,7 @@ __global__ void k_fft_x_c2r(const C2<T> *__restrict__ spec, T *__restrict__ u, T
                             int64_t pairs_per_inst) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = (C2<T> *)smem_raw;
-    const int N = pl_g.N, H = N / 2 + 1, P = lds_pitch(N);
+    const int N = pl_g.N, P = lds_pitch(N), Hs = spec_row<T>(N);
     const FftPlan pl = stage_plan<T>(pl_g, smem_raw + sizeof(C2<T>) * (size_t)pairs_per_block * P);
     __syncthreads();
     const int64_t pair0 = (int64_t)blockIdx.x * pairs_per_block;
@@ -288,7 +295,7 @@ __global__ void k_fft_x_c2r(const C2<T> *__restrict__ spec, T *__restrict__ u, T
         int pp = i / N, k = i - pp * N;
         int64_t line = 2 * (pair0 + pp);
         int kk = k <= N / 2 ? k : N - k;
-        C2<T> A = spec[line * H + kk], B = spec[(line + 1) * H + kk];
+        C2<T> A = spec[line * Hs + kk], B = spec[(line + 1) * Hs + kk];
         if (k == 0 || k == N / 2) { A.y = 0; B.y = 0; }
         if (k > N / 2) { A.y = -A.y; B.y = -B.y; }   // Hermitian extension
         // Z = A + i B
@@ -316,24 +323,24 @@ template <typename T, bool INV>
 __global__ void k_fft_y(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchunk) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = (C2<T> *)smem_raw;
-    const int N = pl_g.N, H = N / 2 + 1, P = lds_pitch(N);
+    const int N = pl_g.N, H = N / 2 + 1, P = lds_pitch(N), Hs = spec_row<T>(N);
     const FftPlan pl = stage_plan<T>(pl_g, smem_raw + sizeof(C2<T>) * (size_t)W * P);
     __syncthreads();
     int chunk = blockIdx.x % nchunk;
     int64_t plane = blockIdx.x / nchunk;  // c*N + z
     int kx0 = chunk * W;
     int w_valid = H - kx0 < W ? H - kx0 : W;
-    C2<T> *base = spec + plane * (int64_t)N * H;
+    C2<T> *base = spec + plane * (int64_t)N * Hs;
     for (int i = threadIdx.x; i < W * N; i += blockDim.x) {
         int w = i % W, y = i / W;
-        if (w < w_valid) buf[w * P + pidx(INV ? y : pl.rev[y])] = base[(int64_t)y * H + kx0 + w];
+        if (w < w_valid) buf[w * P + pidx(INV ? y : pl.rev[y])] = base[(int64_t)y * Hs + kx0 + w];
     }
     __syncthreads();
     if (INV) fft_lds_dif<T, true>(buf, w_valid, P, pl);
     else fft_lds<T, false>(buf, w_valid, P, pl);
     for (int i = threadIdx.x; i < W * N; i += blockDim.x) {
         int w = i % W, y = i / W;
-        if (w < w_valid) base[(int64_t)y * H + kx0 + w] = buf[w * P + pidx(INV ? pl.rev[y] : y)];
+        if (w < w_valid) base[(int64_t)y * Hs + kx0 + w] = buf[w * P + pidx(INV ? pl.rev[y] : y)];
     }
 }
 
@@ -355,12 +362,12 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
                         double *__restrict__ partials) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = (C2<T> *)smem_raw;                       // 3 * W * P
-    const int N = pl_g.N, H = N / 2 + 1, P = lds_pitch(N);
+    const int N = pl_g.N, H = N / 2 + 1, P = lds_pitch(N), Hs = spec_row<T>(N);
     unsigned char *plan_area = smem_raw + sizeof(C2<T>) * (size_t)3 * W * P;
     double *shell = (double *)(plan_area + plan_lds_bytes(N, sizeof(C2<T>)));   // 2 * nek + 2
     const FftPlan pl = stage_plan<T>(pl_g, plan_area);
     __syncthreads();
-    const int64_t N3h = (int64_t)N * N * H;
+    const int64_t N3h = (int64_t)N * N * Hs;
     int chunk = blockIdx.x % nchunk;
     int y = (blockIdx.x / nchunk) % N;
     spec += (int64_t)3 * N3h * (blockIdx.x / ((int64_t)nchunk * N));  // instance of the group
@@ -371,7 +378,7 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
         int w = i % W, rest = i / W;
         int z = rest % N, c = rest / N;
         C2<T> v = {0, 0};
-        if (w < w_valid) v = spec[c * N3h + ((int64_t)z * N + y) * H + kx0 + w];
+        if (w < w_valid) v = spec[c * N3h + ((int64_t)z * N + y) * Hs + kx0 + w];
         buf[(c * W + w) * P + pidx(pl.rev[z])] = v;
     }
     __syncthreads();
@@ -454,7 +461,7 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
         int w = i % W, rest = i / W;
         int z = rest % N, c = rest / N;
         if (w < w_valid)
-            spec[c * N3h + ((int64_t)z * N + y) * H + kx0 + w] = buf[(c * W + w) * P + pidx(pl.rev[z])];
+            spec[c * N3h + ((int64_t)z * N + y) * Hs + kx0 + w] = buf[(c * W + w) * P + pidx(pl.rev[z])];
     }
 }
 
@@ -517,6 +524,11 @@ static void make_plan(int N, FftPlan &pl) {
     while (n % 4 == 0) { pl.radix[pl.npass++] = 4; n /= 4; }
     while (n % 2 == 0) { pl.radix[pl.npass++] = 2; n /= 2; }
     while (n % 3 == 0) { pl.radix[pl.npass++] = 3; n /= 3; }
+}
+
+int64_t spec_elems_of(int N, int precision) {
+    const int Hs = precision == 64 ? spec_row<double>(N) : spec_row<float>(N);
+    return (int64_t)3 * N * N * Hs;
 }
 
 bool fft_size_supported(int N) {

@@ -162,7 +162,7 @@ int wt_create_batch(wt_ctx **out, int32_t ninst, const int32_t *n, int32_t preci
         I.nek = (int)wt::nek_of((int)N);
         I.E_off = eo;
         fo += 3 * N * N * N;
-        so += 3 * N * N * (N / 2 + 1);
+        so += wt::spec_elems_of((int)N, precision);
         eo += I.nek;
     }
     c->field_elems = fo;

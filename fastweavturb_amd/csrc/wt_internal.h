@@ -130,7 +130,7 @@ struct wt_ctx {
     size_t pairs_cap = 0;
     void *omega = nullptr;                // arena: sum of 3*N_i^3 values (float or double)
     void *u = nullptr;
-    void *spec = nullptr;                 // arena: sum of 3*N_i*N_i*(N_i/2+1) complex
+    void *spec = nullptr;                 // arena: three half spectra per instance, rows padded to 128 B
     int64_t field_elems = 0, spec_elems = 0;
     double *partials = nullptr;           // per-block partial sums of the spectral pass
     size_t partials_cap = 0;
@@ -196,6 +196,7 @@ hipError_t spectral_reserve(wt_ctx *c);
 // measuring for the sizes it holds. `report` receives one line per grid size.
 hipError_t spectral_autotune(wt_ctx *c, const std::vector<int32_t> &cached, std::string &report);
 bool fft_size_supported(int N);
+int64_t spec_elems_of(int N, int precision);   // complex elements of one instance's three half spectra
 void fft_reversal(int N, std::vector<int32_t> &rev);
 // stats.hip: structure functions of instance c->cur
 hipError_t launch_sf(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const int32_t *d_seps,
