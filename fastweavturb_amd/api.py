@@ -100,6 +100,7 @@ def load_library():
     L.wt_batch_statistics.argtypes = [vp, i32, i32, i32, ip, dp, dp]
     L.wt_set_graph.argtypes = [vp, i32]
     L.wt_autotune.argtypes = [vp, C.c_char_p, C.c_char_p, i32]
+    L.wt_pdf.argtypes = [vp, i32, i32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int64)]
     _lib = L
     return L
 
@@ -269,6 +270,16 @@ class WovenGPU:
         buf = C.create_string_buffer(4096)
         self._check(self._L.wt_autotune(self._h, cache_path.encode() if cache_path else None, buf, len(buf)))
         return buf.value.decode()
+
+    def pdf(self, field: int = 1, nbins: int = 1000):
+        """(centres, pdf, counts) of the three components of omega (0) or u (1), as cal_pdf."""
+        val = np.zeros(nbins, dtype=np.float32)
+        pdf = np.zeros(nbins, dtype=np.float32)
+        cnt = np.zeros(nbins, dtype=np.int64)
+        fp = C.POINTER(C.c_float)
+        self._check(self._L.wt_pdf(self._h, field, nbins, val.ctypes.data_as(fp), pdf.ctypes.data_as(fp),
+                                   cnt.ctypes.data_as(C.POINTER(C.c_int64))))
+        return val, pdf, cnt
 
     def set_graph(self, enable: bool = True):
         self._check(self._L.wt_set_graph(self._h, int(enable)))

@@ -7,6 +7,7 @@
 // Increments are formed in the field's precision; powers and sums in FP64. Blocks write one
 // partial row each and the rows are reduced in a fixed order, so results are reproducible.
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "wt_internal.h"
@@ -373,6 +374,138 @@ hipError_t launch_sf_batch(wt_ctx *c, int pmin, int pmax, int nsep, const int32_
     if (pmax - pmin + 1 > kMaxOrder || pmin < 1) return hipErrorInvalidValue;
     return c->precision == 64 ? sf_batch_t<double>(c, pmin, pmax, nsep, d_seps, d_out)
                               : sf_batch_t<float>(c, pmin, pmax, nsep, d_seps, d_out);
+}
+
+// ---- PDFs (cal_pdf, subroutine.f90:1-40) -----------------------------------------------------
+// Values are taken as FP32 (the reference copies the fields to real(4), tube.f90:1169-1176).
+template <typename T>
+__global__ __launch_bounds__(256) void k_minmax(const T *__restrict__ x, int64_t n, float *__restrict__ out) {
+    __shared__ float red[2][4];
+    float mn = __builtin_huge_valf(), mx = -__builtin_huge_valf();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = (float)x[i];
+        mn = fminf(mn, v);
+        mx = fmaxf(mx, v);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        mn = fminf(mn, __shfl_xor(mn, o, 64));
+        mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = mn;
+        red[1][threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            mn = fminf(mn, red[0][w]);
+            mx = fmaxf(mx, red[1][w]);
+        }
+        out[2 * blockIdx.x] = red[0][0] < mn ? red[0][0] : mn;
+        out[2 * blockIdx.x + 1] = red[1][0] > mx ? red[1][0] : mx;
+    }
+}
+
+// Each value goes to the first bin l with lo[l] <= v < hi[l] (the reference's linear search).
+// Both edge tables are non-decreasing in l (every step that forms them rounds monotonically), so
+// that bin is the smallest L with hi[L] > v, provided lo[L] <= v, and no bin matches otherwise.
+// L is found by walking from the estimate (v - mn) / bs, which is within a step or two of it unless
+// bs is below the rounding unit of the values. Per-block counts in LDS, then one atomic per bin and
+// block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pdf_hist(const T *__restrict__ x, int64_t n, int nb, float mn, float bs,
+                                                  const float *__restrict__ edges, unsigned long long *__restrict__ counts) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *lo = (float *)smem_raw, *hi = lo + nb;
+    unsigned int *h = (unsigned int *)(hi + nb);
+    for (int l = threadIdx.x; l < nb; l += blockDim.x) {
+        lo[l] = edges[l];
+        hi[l] = edges[nb + l];
+        h[l] = 0u;
+    }
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = (float)x[i];
+        const float t = bs > 0.0f ? (v - mn) / bs : 0.0f;
+        int l = !(t >= 0.0f) ? 0 : (t >= (float)nb ? nb - 1 : (int)t);
+        while (l > 0 && hi[l - 1] > v) l--;
+        while (l < nb && !(hi[l] > v)) l++;
+        if (l < nb && lo[l] <= v) atomicAdd(&h[l], 1u);
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < nb; l += blockDim.x)
+        if (h[l]) atomicAdd(&counts[l], (unsigned long long)h[l]);
+}
+
+template <typename T>
+static hipError_t pdf_t(wt_ctx *c, int field, int nb, float *val, float *pdf, int64_t *counts_out) {
+    hipStream_t st = c->stream;
+    const wt::Instance &I = c->inst[c->cur];
+    const int N = I.N;
+    const int64_t n = (int64_t)3 * N * N * N;
+    const T *x = (const T *)(field == 0 ? c->omega : c->u) + I.field_off;
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    const int blocks = cus * 4;
+    float *d_mm = nullptr, *d_edges = nullptr;
+    unsigned long long *d_cnt = nullptr;
+    WT_HIP(hipMallocAsync((void **)&d_mm, sizeof(float) * 2 * blocks, st));
+    hipError_t e = hipSuccess;
+    std::vector<float> mm(2 * blocks);
+    hipLaunchKernelGGL(k_minmax<T>, dim3(blocks), dim3(256), 0, st, x, n, d_mm);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(mm.data(), d_mm, sizeof(float) * 2 * blocks, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFreeAsync(d_mm, st);
+    if (e != hipSuccess) return e;
+    float mn = mm[0], mx = mm[1];
+    for (int b = 1; b < blocks; b++) {
+        mn = fminf(mn, mm[2 * b]);
+        mx = fmaxf(mx, mm[2 * b + 1]);
+    }
+    // subroutine.f90:16-20: bin_size = (max - min) / real(bin_num);
+    // val(l) = min + 0.5d0*bin_size + real(l-1)*bin_size (double sum of a real(4) product, stored
+    // real(4)); the interval test uses val(l) -/+ 0.5*bin_size in real(4) (:27)
+    const float bs = (mx - mn) / (float)nb;
+    std::vector<float> edges(2 * (size_t)nb);
+    for (int l = 0; l < nb; l++) {
+        const float prod = (float)l * bs;
+        val[l] = (float)(((double)mn + 0.5 * (double)bs) + (double)prod);
+        edges[l] = val[l] - 0.5f * bs;
+        edges[nb + l] = val[l] + 0.5f * bs;
+    }
+    WT_HIP(hipMallocAsync((void **)&d_edges, sizeof(float) * 2 * nb, st));
+    e = hipMallocAsync((void **)&d_cnt, sizeof(unsigned long long) * nb, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_edges, edges.data(), sizeof(float) * 2 * nb, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long) * nb, st);
+    const size_t sh = (sizeof(float) * 2 + sizeof(unsigned int)) * (size_t)nb;
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_pdf_hist<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_pdf_hist<T>, dim3(blocks), dim3(256), sh, st, x, n, nb, mn, bs, (const float *)d_edges,
+                           d_cnt);
+        e = hipGetLastError();
+    }
+    std::vector<unsigned long long> cnt(nb);
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_cnt, sizeof(unsigned long long) * nb, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (d_edges) (void)hipFreeAsync(d_edges, st);
+    if (d_cnt) (void)hipFreeAsync(d_cnt, st);
+    if (e != hipSuccess) return e;
+    // subroutine.f90:37-39: pdf_v(l) / (bin_size*nx*ny*nzp*nproc) in real(4), nzp = 3 N (the three
+    // components stacked along z), nproc = 1
+    const float denom = (((bs * (float)N) * (float)N) * (float)(3 * N)) * 1.0f;
+    for (int l = 0; l < nb; l++) {
+        pdf[l] = (float)cnt[l] / denom;
+        if (counts_out) counts_out[l] = (int64_t)cnt[l];
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_pdf(wt_ctx *c, int field, int nbins, float *val, float *pdf, int64_t *counts) {
+    return c->precision == 64 ? pdf_t<double>(c, field, nbins, val, pdf, counts)
+                              : pdf_t<float>(c, field, nbins, val, pdf, counts);
 }
 
 hipError_t launch_sf(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const int32_t *d_seps, double *d_out) {

@@ -809,6 +809,16 @@ int wt_batch_statistics(wt_ctx *c, int32_t pmin, int32_t pmax, int32_t nsep, con
     return WT_OK;
 }
 
+int wt_pdf(wt_ctx *c, int32_t field, int32_t nbins, float *val, float *pdf, int64_t *counts) {
+    CHECK_CTX(c);
+    if (field != 0 && field != 1) return set_err(c, WT_EINVAL, "field must be 0 (vorticity) or 1 (velocity)");
+    if (nbins < 1 || nbins > 8192 || !val || !pdf) return set_err(c, WT_EINVAL, "need 1 <= nbins <= 8192 and outputs");
+    if (field == 1 ? !c->have_u : !c->have_omega) return set_err(c, WT_ESTATE, "the field is not resident");
+    hipError_t e = wt::launch_pdf(c, field, nbins, val, pdf, counts);
+    if (e != hipSuccess) return hip_err(c, e, "pdf");
+    return WT_OK;
+}
+
 static void *field_ptr(wt_ctx *c, int which) {
     const wt::Instance &I = c->inst[c->cur];
     size_t N3 = (size_t)I.N * I.N * I.N, es = c->precision == 64 ? 8 : 4;

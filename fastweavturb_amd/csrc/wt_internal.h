@@ -201,6 +201,8 @@ void fft_reversal(int N, std::vector<int32_t> &rev);
 // stats.hip: structure functions of instance c->cur
 hipError_t launch_sf(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const int32_t *d_seps,
                      double *d_out);
+// PDF of the three components of omega (field 0) or u (field 1) of instance c->cur (cal_pdf)
+hipError_t launch_pdf(wt_ctx *c, int field, int nbins, float *val, float *pdf, int64_t *counts);
 // x-increment structure functions of every instance in one launch: d_seps[pos*nsep + m] per arena
 // position, d_out[(pos*np + p)*nsep + m]
 hipError_t launch_sf_batch(wt_ctx *c, int pmin, int pmax, int nsep, const int32_t *d_seps, double *d_out);

@@ -7,6 +7,7 @@
 !   output/test_data.dat       per tube ntf_average / ntf_max       (tube.f90:673-676)
 !   output/stat/spectrum_vor_temp.dat  spectrum of raw vorticity    (tube.f90:972-978)
 !   output/stat/spectrum.dat   spectrum of normalised velocity      (tube.f90:1158-1164)
+!   output/stat/pdf_um.dat, pdf_wm.dat  PDFs of u and omega, 1000 bins (tube.f90:1166-1185)
 !   output/stat/Sp.dat         structure functions, orders 1..10    (subroutine.f90:64-129)
 !   output/field_wh.dat        |omega| and omega.u, downsampled, raw FP32 (tube.f90:1055-1073)
 !   output/info.dat            summary                              (tube.f90:1197-1211)
@@ -16,7 +17,7 @@ program weave_host
    use wovengpu
    implicit none
 
-   integer, parameter :: Sp_order = 10
+   integer, parameter :: Sp_order = 10, bin_num = 1000
    integer :: nline, npointall, i, j, k, i_line, nx, n_d, nek, nsep, r_d, nb, prec, ios
    integer(c_int32_t), allocatable :: npointlist(:), n_T_line(:), if_filter_line(:), seps(:)
    integer(c_int32_t), allocatable :: ntf(:)
@@ -25,6 +26,7 @@ program weave_host
    real(c_double), allocatable, target :: f(:, :, :, :)
    real(c_float), allocatable :: data_box(:, :, :, :)
    real(c_float) :: sigma_out, Gamma_out
+   real(c_float) :: vari(bin_num), pdf(bin_num)
    type(wt_params) :: p
    type(wt_seg_stats) :: st
    type(c_ptr) :: ctx
@@ -166,6 +168,20 @@ program weave_host
       write (49, *) i, real(E(i))
    end do
    close (49)
+
+   ! pdf_um.dat, pdf_wm.dat (export_pdf, subroutine.f90:42-62)
+   open (3201, file='./output/stat/pdf_um.dat', status='replace')
+   call wt_check(ctx, wt_pdf(ctx, 1_c_int32_t, int(bin_num, c_int32_t), vari, pdf, c_null_ptr), 'wt_pdf')
+   do i = 1, bin_num
+      write (3201, *) vari(i), pdf(i)
+   end do
+   close (3201)
+   open (3204, file='./output/stat/pdf_wm.dat', status='replace')
+   call wt_check(ctx, wt_pdf(ctx, 0_c_int32_t, int(bin_num, c_int32_t), vari, pdf, c_null_ptr), 'wt_pdf')
+   do i = 1, bin_num
+      write (3204, *) vari(i), pdf(i)
+   end do
+   close (3204)
 
    ! Sp.dat (subroutine.f90:114-123)
    nsep = wt_sp_separations(int(nx, c_int32_t), seps, 0_c_int32_t)

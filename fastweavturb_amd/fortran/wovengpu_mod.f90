@@ -92,6 +92,13 @@ module wovengpu
          integer(c_int32_t), intent(in) :: seps(*)
          real(c_double), intent(out) :: S(*)
       end function
+      integer(c_int) function wt_pdf(ctx, field, nbins, val, pdf, counts) bind(C, name="wt_pdf")
+         import :: c_ptr, c_int, c_int32_t, c_float
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: field, nbins
+         real(c_float), intent(out) :: val(*), pdf(*)
+         type(c_ptr), value :: counts
+      end function
       integer(c_int) function wt_get_field(ctx, which, host, host_is_fp64) bind(C, name="wt_get_field")
          import :: c_ptr, c_int, c_int32_t
          type(c_ptr), value :: ctx

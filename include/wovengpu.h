@@ -179,6 +179,16 @@ int wt_set_graph(wt_ctx *ctx, int32_t enable);
  * The half spectrum is overwritten; call before wt_generate / wt_biot_savart. */
 int wt_autotune(wt_ctx *ctx, const char *cache_path, char *report, int32_t report_cap);
 
+/* Replaces cal_pdf / export_pdf (subroutine.f90:1-62) as called at tube.f90:1166-1185: the PDF
+ * of all three components of the resident omega (field 0) or u (field 1), taken as FP32 values
+ * like the reference's real(4) copy, over nbins bins between their minimum and maximum
+ * (the reference uses bin_num = 1000, tube.f90:129). val[l] are the bin centres and pdf[l] the
+ * densities in the reference's FP32 arithmetic; counts (may be NULL) the number of values in each
+ * bin. A value goes to the first bin whose interval [val - bs/2, val + bs/2) holds it, so values in
+ * no bin (the maximum, usually) are not counted, as in the reference. Counts are exact; the
+ * reference counts in FP32, which stops incrementing at 2^24 per bin and rank. */
+int wt_pdf(wt_ctx *ctx, int32_t field, int32_t nbins, float *val, float *pdf, int64_t *counts);
+
 /* Copy a resident field component to/from host memory, Fortran order, FP64 or FP32 host
  * buffer (output_v/output_puredata copy-out, subroutine.f90:1342-1399). wt_set_field replaces
  * the resident omega or u (used to drive wt_biot_savart / statistics on a given field). */

@@ -40,6 +40,15 @@ def test_fortran_host_outputs_match_oracle(oracle, tmp_path, precision):
     Sref, r = oracle.sp_calculate(u, 10)
     np.testing.assert_allclose(Sp[:, 0], r * 2 * np.pi / 16, rtol=1e-5)
     np.testing.assert_allclose(Sp[:, 1:].T, Sref, rtol=5e-5, atol=1e-9)
+    from oracle import pdf_oracle as PO
+    for fname, fld in (("pdf_um.dat", u), ("pdf_wm.dat", om)):
+        pd = np.loadtxt(tmp_path / "output/stat" / fname)
+        rv, rp, rc = PO.cal_pdf(fld.astype(np.float32), 1000, (16, 16, 48))
+        assert pd.shape == (1000, 2)
+        span = float(rv[-1] - rv[0])
+        np.testing.assert_allclose(pd[:, 0], rv, rtol=0, atol=1e-4 * span)
+        bs = span / 999
+        assert abs(pd[:, 1].sum() * bs - rc.sum() / fld.size) < 1e-3
     wh = np.fromfile(tmp_path / "output/field_wh.dat", dtype=np.float32).reshape(2, 16, 16, 16)
     wmag = np.sqrt((om ** 2).sum(axis=0))
     hel = (om * u).sum(axis=0)
