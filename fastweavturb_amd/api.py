@@ -101,6 +101,7 @@ def load_library():
     L.wt_set_graph.argtypes = [vp, i32]
     L.wt_autotune.argtypes = [vp, C.c_char_p, C.c_char_p, i32]
     L.wt_pdf.argtypes = [vp, i32, i32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int64)]
+    L.wt_filter_spec.argtypes = [vp, i32]
     _lib = L
     return L
 
@@ -280,6 +281,10 @@ class WovenGPU:
         self._check(self._L.wt_pdf(self._h, field, nbins, val.ctypes.data_as(fp), pdf.ctypes.data_as(fp),
                                    cnt.ctypes.data_as(C.POINTER(C.c_int64))))
         return val, pdf, cnt
+
+    def filter_spec(self, k_c: int):
+        """Sharp spectral cut of the resident vorticity at k_c (filter_spec)."""
+        self._check(self._L.wt_filter_spec(self._h, int(k_c)))
 
     def set_graph(self, enable: bool = True):
         self._check(self._L.wt_set_graph(self._h, int(enable)))

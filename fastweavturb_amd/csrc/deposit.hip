@@ -43,6 +43,7 @@ struct DepArgs {
     double inv_scale;             // 2^-shift
     unsigned long long *counters; // [0] columns [1] candidates [2] hits [3] overflow
     unsigned int *next;           // brick queue cursor
+    int accumulate;               // 0: the brick's cells are written; 1: added to omega
 };
 
 __device__ __forceinline__ unsigned long long lanemask_lt() {
@@ -623,7 +624,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
             if (W.hitn > 0) stage_hits<F64, FAST>(W, me, W.hitn);
         }
         __syncthreads();
-        // the brick's cells are owned by this workgroup: plain stores, once
+        // the brick's cells are owned by this workgroup: plain stores (or read-add-store), once
         const int64_t N3 = (int64_t)N * N * N;
         for (int i = threadIdx.x; i < kCells; i += kDepThreads) {
             const int x = i % kBrickX, y = (i / kBrickX) % kBrickY, z = i / (kBrickX * kBrickY);
@@ -631,9 +632,15 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                 const int64_t idx = ri.field_off + (ox + x) + (int64_t)N * ((oy + y) + (int64_t)N * (oz + z));
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    if (F64) ((double *)A.omega)[idx + c * N3] = (double)acc[c * kCells + i];
-                    else ((float *)A.omega)[idx + c * N3] =
-                        (float)((double)(long long)acc[c * kCells + i] * A.inv_scale);
+                    if (F64) {
+                        double *o = (double *)A.omega + idx + c * N3;
+                        const double v = (double)acc[c * kCells + i];
+                        *o = A.accumulate ? *o + v : v;
+                    } else {
+                        float *o = (float *)A.omega + idx + c * N3;
+                        const double v = (double)(long long)acc[c * kCells + i] * A.inv_scale;
+                        *o = (float)(A.accumulate ? (double)*o + v : v);
+                    }
                 }
             }
         }
@@ -676,7 +683,7 @@ static hipError_t deposit_t(wt_ctx *c, DepArgs &A) {
     return hipGetLastError();
 }
 
-hipError_t launch_deposit(wt_ctx *c) {
+hipError_t launch_deposit(wt_ctx *c, bool accumulate) {
     hipStream_t st = c->stream;
     const bool f64 = c->precision == 64;
     unsigned long long *cnt = nullptr;
@@ -689,6 +696,7 @@ hipError_t launch_deposit(wt_ctx *c) {
     A.omega = c->omega;
     A.counters = cnt;
     A.next = (unsigned int *)(cnt + 4);
+    A.accumulate = accumulate ? 1 : 0;
     if (!f64) {
         A.fx_scale = ldexp(1.0, c->fixed_shift);
         A.inv_scale = ldexp(1.0, -c->fixed_shift);

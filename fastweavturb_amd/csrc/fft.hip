@@ -279,7 +279,8 @@ __global__ void k_fft_x_r2c(const T *__restrict__ f, C2<T> *__restrict__ spec, F
     }
 }
 
-// ---- x pass, inverse: two half spectra -> two real lines, times inv_up; omega too --------
+// ---- x pass, inverse: two half spectra -> two real lines, times inv_up; omega too (sc null:
+// unscaled, omega untouched) ----------------------------------------------------------------
 template <typename T>
 __global__ void k_fft_x_c2r(const C2<T> *__restrict__ spec, T *__restrict__ u, T *__restrict__ omega, FftPlan pl_g,
                             int pairs_per_block, int64_t npairs, const DeviceScalars *__restrict__ sc,
@@ -307,7 +308,7 @@ __global__ void k_fft_x_c2r(const C2<T> *__restrict__ spec, T *__restrict__ u, T
     for (int i = threadIdx.x; i < np * N; i += blockDim.x) {
         int pp = i / N, x = i - pp * N;
         int64_t line = 2 * (pair0 + pp);
-        const T s = (T)sc[(pair0 + pp) / pairs_per_inst].inv_up;
+        const T s = sc ? (T)sc[(pair0 + pp) / pairs_per_inst].inv_up : (T)1;
         C2<T> z = buf[pp * P + pidx(pl.rev[x])];
         u[line * N + x] = z.x * s;
         u[(line + 1) * N + x] = z.y * s;
@@ -354,12 +355,13 @@ __device__ __forceinline__ int shell_of(int64_t k2) {
     return m;
 }
 
-// ---- z pass: forward, Biot-Savart (mode 0) or spectrum only (mode 1), inverse ------------
-// partials row per block: [0, nek) shells of w_hat (mode 0) | [nek, 2nek) shells of u_hat |
-// [2nek] sum |w_hat|^2 weighted | [2nek+1] sum |u_hat|^2 weighted
+// ---- z pass: forward, Biot-Savart (mode 0), spectrum only (mode 1) or the sharp spectral
+// filter (mode 2: modes with k2 > kc2 set to zero, filter_spec, subroutine.f90:1225-1277),
+// inverse. Partials row per block (modes 0 and 1): [0, nek) shells of w_hat (mode 0) |
+// [nek, 2nek) shells of u_hat | [2nek] sum |w_hat|^2 weighted | [2nek+1] sum |u_hat|^2 weighted
 template <typename T>
 __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchunk, int mode, int nek,
-                        double *__restrict__ partials) {
+                        double *__restrict__ partials, int64_t kc2) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *buf = (C2<T> *)smem_raw;                       // 3 * W * P
     const int N = pl_g.N, H = N / 2 + 1, P = lds_pitch(N), Hs = spec_row<T>(N);
@@ -383,8 +385,19 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
     }
     __syncthreads();
     fft_lds<T, false>(buf, 3 * W, P, pl);
-    // pointwise: (w, kz) pairs
     const int ky = wavenum(y, N);
+    if (mode == 2) {
+        for (int i = threadIdx.x; i < w_valid * N; i += blockDim.x) {
+            int w = i / N, z = i - w * N;
+            int kx = wavenum(kx0 + w, N), kz = wavenum(z, N);
+            if ((int64_t)kx * kx + (int64_t)ky * ky + (int64_t)kz * kz > kc2) {
+                const int zp = pidx(z);
+#pragma unroll
+                for (int c = 0; c < 3; c++) buf[(c * W + w) * P + zp] = C2<T>{0, 0};
+            }
+        }
+    } else {
+    // pointwise: (w, kz) pairs
     double e_w = 0.0, e_u = 0.0;
     for (int i = threadIdx.x; i < w_valid * N; i += blockDim.x) {
         int w = i / N, z = i - w * N;
@@ -455,6 +468,7 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
     double *row = partials + (int64_t)blockIdx.x * (2 * nek + 2);
     for (int i = threadIdx.x; i < 2 * nek + 2; i += blockDim.x) row[i] = shell[i];
     if (mode != 0) return;
+    }
     __syncthreads();
     fft_lds_dif<T, true>(buf, 3 * W, P, pl);
     for (int i = threadIdx.x; i < 3 * W * N; i += blockDim.x) {
@@ -641,7 +655,7 @@ hipError_t spectral_reserve(wt_ctx *c) {
 // Biot-Savart (mode 0) or the velocity spectrum (mode 1) of K instances of equal N whose fields,
 // spectra, scalars and spectra arrays are contiguous from arena position `pos`.
 template <typename T>
-static hipError_t spectral_group(wt_ctx *c, int mode, int pos, int K, bool timed) {
+static hipError_t spectral_group(wt_ctx *c, int mode, int pos, int K, bool timed, int64_t kc2 = 0) {
     hipStream_t st = c->stream;
     const wt::Instance &I0 = c->inst[c->order[pos]];
     const int N = I0.N, H = N / 2 + 1;
@@ -651,7 +665,7 @@ static hipError_t spectral_group(wt_ctx *c, int mode, int pos, int K, bool timed
     if (!pl.tw) return hipErrorInvalidValue;
     pl.rev = (const int32_t *)((const char *)pl.tw + sizeof(C2<T>) * N);
     const size_t cs = sizeof(C2<T>);
-    T *field = (T *)(mode == 0 ? c->omega : c->u) + I0.field_off;
+    T *field = (T *)(mode == 1 ? c->u : c->omega) + I0.field_off;
     C2<T> *spec = (C2<T> *)c->spec + I0.spec_off;
     const int nek = I0.nek;
     const FftTiles tl = tiles_for(c, N, nek);
@@ -685,18 +699,26 @@ static hipError_t spectral_group(wt_ctx *c, int mode, int pos, int K, bool timed
     const int64_t zblocks = zrows * K;
     const int cols = 2 * nek + 2;
     size_t need = sizeof(double) * ((size_t)zblocks * cols + (size_t)K * cols);
-    if (need > c->partials_cap) {
+    if (mode != 2 && need > c->partials_cap) {            // the filter writes no partial rows
         if (c->capturing) return hipErrorInvalidValue;   // spectral_reserve runs before a capture
         WT_HIP(spectral_reserve(c));
         if (need > c->partials_cap) return hipErrorInvalidValue;
     }
-    double *colsum = c->partials + (size_t)zblocks * cols;
+    double *colsum = c->partials ? c->partials + (size_t)zblocks * cols : nullptr;
     WT_HIP(hipFuncSetAttribute((const void *)k_fft_z<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shz));
     if (timed) kt_begin(c, kSlotFftZ);
     hipLaunchKernelGGL(k_fft_z<T>, dim3((unsigned)zblocks), dim3(tl.tz), shz, st, spec, pl, Wz, ncz, mode, nek,
-                       c->partials);
+                       c->partials, kc2);
     if (timed) kt_end(c, kSlotFftZ);
     WT_HIP(hipGetLastError());
+    if (mode == 2) {   // filtered omega: y inverse, x inverse unscaled into omega
+        hipLaunchKernelGGL((k_fft_y<T, true>), dim3((unsigned)(3 * K * N * ncy)), dim3(tl.ty), shy, st, spec, pl, Wy, ncy);
+        WT_HIP(hipGetLastError());
+        WT_HIP(hipFuncSetAttribute((const void *)k_fft_x_c2r<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shx));
+        hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(gx), dim3(tl.tx), shx, st, (const C2<T> *)spec, field, (T *)nullptr, pl,
+                           ppb, npairs, (const DeviceScalars *)nullptr, pairs_per_inst);
+        return hipGetLastError();
+    }
     if (timed) kt_begin(c, kSlotReduce);
     hipLaunchKernelGGL(k_reduce_columns, dim3((unsigned)((cols + 63) / 64), (unsigned)K), dim3(1024), 0, st,
                        (const double *)c->partials, zrows, cols, colsum);
@@ -837,7 +859,7 @@ static hipError_t tune_group(wt_ctx *c, int pos, int K, FftTiles &best, std::str
         for (int tz : threads) {
             if (err != hipSuccess) break;
             err = timed([&] { hipLaunchKernelGGL(k_fft_z<T>, dim3((unsigned)zblocks), dim3(tz), sh, st, spec, pl, wz, ncz,
-                                                 0, nek, c->partials); }, &t);
+                                                 0, nek, c->partials, (int64_t)0); }, &t);
             if (err == hipSuccess && t < bz) { bz = t; best.wz = wz; best.tz = tz; }
         }
     }
@@ -878,6 +900,17 @@ hipError_t spectral_autotune(wt_ctx *c, const std::vector<int32_t> &cached, std:
         pos = end;
     }
     return spectral_reserve(c);
+}
+
+template <typename T>
+static hipError_t filter_t(wt_ctx *c, int inst, int64_t kc2) {
+    for (int pos = 0; pos < c->ninst(); pos++)
+        if (c->order[pos] == inst) return spectral_group<T>(c, 2, pos, 1, false, kc2);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_filter(wt_ctx *c, int inst, int64_t kc2) {
+    return c->precision == 64 ? filter_t<double>(c, inst, kc2) : filter_t<float>(c, inst, kc2);
 }
 
 hipError_t launch_spectral(wt_ctx *c, int mode, int only) {

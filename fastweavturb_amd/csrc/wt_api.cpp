@@ -272,8 +272,6 @@ int wt_set_tubes(wt_ctx *c, int32_t nline, const int32_t *npoint, const double *
         if (npoint[l] < 3) return set_err(c, WT_EINVAL, "tube %d has %d points; need at least 3", l + 1, npoint[l]);
         if (!(sigma[l] > 0) || !std::isfinite(sigma[l]) || !std::isfinite(gamma[l]))
             return set_err(c, WT_EINVAL, "tube %d: sigma must be positive and finite, Gamma finite", l + 1);
-        if (if_filter && if_filter[l] != 0)
-            return set_err(c, WT_EINVAL, "tube %d: if_filter=1 (filter_spec) is not supported", l + 1);
         P += npoint[l];
     }
     for (int64_t i = 0; i < P; i++)
@@ -282,6 +280,9 @@ int wt_set_tubes(wt_ctx *c, int32_t nline, const int32_t *npoint, const double *
     wt::Instance &I = c->inst[c->cur];
     I.npoint.assign(npoint, npoint + nline);
     I.n_T.assign(n_T, n_T + nline);
+    I.if_filter.assign(nline, 0);
+    if (if_filter)
+        for (int l = 0; l < nline; l++) I.if_filter[l] = if_filter[l] != 0;
     I.sigma.assign(sigma, sigma + nline);
     I.gamma.assign(gamma, gamma + nline);
     I.cx.assign(cx, cx + P);
@@ -483,15 +484,108 @@ int wt_bin(wt_ctx *c) {
     return WT_OK;
 }
 
+// Tubes line_s..line_e of instance 0 with if_filter set (the reference's loop visits only those,
+// tube.f90:529, 926).
+static std::vector<int> filter_tubes(const wt_ctx *c) {
+    std::vector<int> f;
+    if (c->ninst() != 1) return f;
+    const wt::Instance &I = c->inst[0];
+    for (int l = I.params.line_s - 1; l <= I.params.line_e - 1 && l < (int)I.if_filter.size(); l++)
+        if (I.if_filter[l]) f.push_back(l);
+    return f;
+}
+
+static bool batch_has_filters(const wt_ctx *c) {
+    for (const auto &I : c->inst)
+        for (int32_t v : I.if_filter)
+            if (v) return true;
+    return false;
+}
+
+// The reference deposits tube by tube and filters the accumulated vorticity after each tube with
+// if_filter = 1, with k_c = ceiling(4/sigma) of that tube (tube.f90:926-929). Here the tubes
+// between two filters deposit as one segment (binned and deposited in one launch each, added to
+// the field after the first), then the field is filtered.
+static int deposit_filtered(wt_ctx *c, const std::vector<int> &flt) {
+    wt::Run &r = c->runs_host[0];
+    const int32_t s0 = r.line_s, e0 = r.line_e;
+    struct Seg { int32_t a, b; int64_t kc2; };
+    std::vector<Seg> segs;
+    int32_t a = s0;
+    for (int l : flt) {
+        const double kc = std::ceil(4.0 / c->inst[0].sigma[l]);
+        if (!(kc < 46341.0)) return set_err(c, WT_ERANGE, "tube %d: k_c = ceiling(4/sigma) exceeds 46340", l + 1);
+        segs.push_back({a, l, (int64_t)kc * (int64_t)kc});
+        a = l + 1;
+    }
+    if (a <= e0) segs.push_back({a, e0, -1});
+    wt_deposit_stats tot{};
+    auto set_range = [&](int32_t lo, int32_t hi) -> hipError_t {
+        r.line_s = lo;
+        r.line_e = hi;
+        c->run = r;
+        hipError_t e = hipMemcpyAsync(c->runs_dev, &r, sizeof r, hipMemcpyHostToDevice, c->stream);
+        return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
+    };
+    int rc = WT_OK;
+    for (size_t i = 0; i < segs.size() && rc == WT_OK; i++) {
+        hipError_t e = set_range(segs[i].a, segs[i].b);
+        if (e == hipSuccess) e = wt::launch_bins(c);
+        if (e == hipErrorInvalidValue) {
+            rc = set_err(c, WT_ERANGE, "a tube's cut-off radius spans more than the periodic box");
+            break;
+        }
+        if (e == hipSuccess) e = wt::launch_deposit(c, i > 0);
+        if (e == hipSuccess && segs[i].kc2 >= 0) e = wt::launch_filter(c, 0, segs[i].kc2);
+        if (e != hipSuccess) {
+            rc = hip_err(c, e, "filtered deposit");
+            break;
+        }
+        tot.columns += c->dep_stats.columns;
+        tot.candidates += c->dep_stats.candidates;
+        tot.hits += c->dep_stats.hits;
+        tot.pairs += c->dep_stats.pairs;
+        tot.overflow |= c->dep_stats.overflow;
+        tot.fixed_point_shift = c->dep_stats.fixed_point_shift;
+    }
+    hipError_t e = set_range(s0, e0);
+    if (rc != WT_OK) return rc;
+    if (e != hipSuccess) return hip_err(c, e, "filtered deposit");
+    c->dep_stats = tot;
+    c->have_bins = false;   // the bins now hold the last segment only
+    return WT_OK;
+}
+
 int wt_deposit(wt_ctx *c, wt_deposit_stats *out) {
     CHECK_CTX(c);
-    if (!c->have_bins) return set_err(c, WT_ESTATE, "wt_deposit needs wt_bin");
-    hipError_t e = wt::launch_deposit(c);
-    if (e != hipSuccess) return hip_err(c, e, "deposit");
+    if (batch_has_filters(c) && c->ninst() != 1)
+        return set_err(c, WT_EINVAL, "if_filter = 1 needs a single-instance context");
+    const std::vector<int> flt = filter_tubes(c);
+    if (!flt.empty()) {
+        if (!c->have_segments) return set_err(c, WT_ESTATE, "wt_deposit needs wt_build_segments");
+        int rc = deposit_filtered(c, flt);
+        if (rc) return rc;
+    } else {
+        if (!c->have_bins) return set_err(c, WT_ESTATE, "wt_deposit needs wt_bin");
+        hipError_t e = wt::launch_deposit(c);
+        if (e != hipSuccess) return hip_err(c, e, "deposit");
+    }
     if (out) *out = c->dep_stats;
     if (c->dep_stats.overflow)
         return set_err(c, WT_ERANGE, "fixed-point accumulator overflow (2^%d scale); use precision 64", c->fixed_shift);
     c->have_omega = true;
+    c->have_u = false;
+    c->have_spectra = false;
+    return WT_OK;
+}
+
+int wt_filter_spec(wt_ctx *c, int32_t k_c) {
+    CHECK_CTX(c);
+    if (k_c < 0 || k_c > 46340) return set_err(c, WT_EINVAL, "need 0 <= k_c <= 46340");
+    if (!c->have_omega) return set_err(c, WT_ESTATE, "the vorticity is not resident");
+    hipError_t e = wt::launch_filter(c, c->cur, (int64_t)k_c * k_c);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_err(c, e, "filter");
     c->have_u = false;
     c->have_spectra = false;
     return WT_OK;
@@ -677,7 +771,7 @@ int wt_generate(wt_ctx *c, double *etot, double *up, double *enstrophy) {
     int rc = wt_build_segments(c, nullptr);
     if (rc) return rc;
     HIPCALL(c, hipEventRecord(c->ev[1], c->stream), "event");
-    if ((rc = wt_bin(c))) return rc;
+    if (filter_tubes(c).empty() && (rc = wt_bin(c))) return rc;   // a filtered deposit bins per segment
     HIPCALL(c, hipEventRecord(c->ev[2], c->stream), "event");
     if ((rc = wt_deposit(c, nullptr))) return rc;
     HIPCALL(c, hipEventRecord(c->ev[3], c->stream), "event");

@@ -92,7 +92,7 @@ struct Instance {
     int N = 0;
     wt_params params{};
     bool have_params = false, have_tubes = false;
-    std::vector<int32_t> npoint, n_T;
+    std::vector<int32_t> npoint, n_T, if_filter;
     std::vector<double> sigma, gamma, cx, cy, cz;
     int64_t field_off = 0;      // elements, in the omega/u arenas (3*N^3 per instance)
     int64_t spec_off = 0;       // complex elements, in the spectrum arena
@@ -185,12 +185,15 @@ hipError_t launch_segments(wt_ctx *c);       // points -> sections -> sub-points
 hipError_t launch_bins(wt_ctx *c);           // per sub-point boxes -> work items
 hipError_t launch_get_bins(wt_ctx *c, int32_t *d_lohi, double *d_R);
 hipError_t launch_seg_stats(wt_ctx *c, wt_seg_stats *out);
-// deposit.hip
-hipError_t launch_deposit(wt_ctx *c);
+// deposit.hip: accumulate = false writes omega, true adds to it
+hipError_t launch_deposit(wt_ctx *c, bool accumulate = false);
 // fft.hip: mode 0 = Biot-Savart + normalisation of every instance; mode 1 = spectrum of the
 // resident velocity of instance `only`
 hipError_t launch_spectral(wt_ctx *c, int mode, int only);
 hipError_t spectral_reserve(wt_ctx *c);
+// filter_spec (subroutine.f90:1225-1277) on the vorticity of instance `inst`: every mode with
+// kx^2 + ky^2 + kz^2 > kc2 set to zero
+hipError_t launch_filter(wt_ctx *c, int inst, int64_t kc2);
 // Times the candidate tile shapes of the x, y and z passes for every grid size of the context and
 // keeps the fastest in c->fft_tiles; `cached` (7 ints per N, may be empty) is used instead of
 // measuring for the sizes it holds. `report` receives one line per grid size.

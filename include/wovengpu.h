@@ -108,8 +108,9 @@ int wt_set_params(wt_ctx *ctx, const wt_params *p);
 
 /* Replaces the per-tube rows of parameters_input.dat (tube.f90:218-221) and the centerline
  * read + broadcast (tube.f90:341-379): nline tubes, npoint[l] points each, coordinates as
- * three flat FP64 arrays of sum(npoint) entries. if_filter must be all zero (filter_spec is
- * not part of this library). */
+ * three flat FP64 arrays of sum(npoint) entries. if_filter may be NULL (all zero); a nonzero
+ * entry filters the accumulated vorticity after that tube's deposition (filter_spec,
+ * tube.f90:926-936), in single-instance contexts. */
 int wt_set_tubes(wt_ctx *ctx, int32_t nline, const int32_t *npoint, const double *sigma,
                  const double *gamma, const int32_t *n_T, const int32_t *if_filter,
                  const double *cx, const double *cy, const double *cz);
@@ -125,8 +126,15 @@ int wt_get_seg_stats(wt_ctx *ctx, wt_seg_stats *out);
 int wt_bin(wt_ctx *ctx);
 
 /* Replaces tube.f90:800-892: vorticity of every (cell, sub-segment) pair that passes the
- * reference's tests, summed over sub-segments. Leaves omega resident on the device. */
+ * reference's tests, summed over sub-segments. Leaves omega resident on the device.
+ * With if_filter set on tubes of line_s..line_e it also replaces tube.f90:926-936 and
+ * subroutine.f90:1225-1277: the tubes up to each such tube are binned and deposited, then every
+ * Fourier mode of the accumulated vorticity with kx^2 + ky^2 + kz^2 > k_c^2, k_c =
+ * ceiling(4/sigma) of that tube, is set to zero; wt_bin is then not needed beforehand. */
 int wt_deposit(wt_ctx *ctx, wt_deposit_stats *out);
+/* Replaces filter_spec (subroutine.f90:1225-1277) on the resident vorticity of the selected
+ * instance: every Fourier mode with kx^2 + ky^2 + kz^2 > k_c^2 is set to zero. */
+int wt_filter_spec(wt_ctx *ctx, int32_t k_c);
 /* Counters of the last deposition (also run inside wt_generate). */
 int wt_get_deposit_stats(wt_ctx *ctx, wt_deposit_stats *out);
 

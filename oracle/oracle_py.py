@@ -186,6 +186,19 @@ class Geometry:
                                 C.byref(self._g), _dp(om), C.byref(hits)))
         return om.reshape(3, N, N, N), int(hits.value)
 
+    def deposit_filtered(self, sigma_line, Gamma_line, n_T_line, if_filter):
+        """or_deposit_filtered: the tube loop with the per-tube spectral filters."""
+        N = self.params.N
+        om = np.zeros(3 * N ** 3, dtype=np.float64)
+        sig = np.ascontiguousarray(sigma_line, dtype=np.float64)
+        gam = np.ascontiguousarray(Gamma_line, dtype=np.float64)
+        nT = np.ascontiguousarray(n_T_line, dtype=np.int32)
+        flt = np.ascontiguousarray(if_filter, dtype=np.int32)
+        hits = C.c_int64(0)
+        _check(lib().or_deposit_filtered(C.byref(self.params), _dp(sig), _dp(gam), _ip(nT), _ip(flt),
+                                         C.byref(self._g), _dp(om), C.byref(hits)))
+        return om.reshape(3, N, N, N), int(hits.value)
+
 
 def deposit_range_timed(params: Params, geom: "Geometry", sigma_line, Gamma_line, n_T_line, seg_lo, seg_hi):
     """Time or_deposit_range on sub-segments [seg_lo, seg_hi) into a scratch field (seconds, hits)."""
@@ -273,7 +286,15 @@ def sigma(zeta, length, smax, smin, n_T) -> float:
     return float(lib().or_sigma(zeta, length, smax, smin, n_T))
 
 
-def run(params: Params, npoint, cx, cy, cz, sigma_line, Gamma_line, n_T_line):
+def filter_spec(omega: np.ndarray, kc: int) -> np.ndarray:
+    """or_filter_spec on a copy of omega (3, N, N, N)."""
+    N = omega.shape[-1]
+    om = np.ascontiguousarray(omega, dtype=np.float64).copy()
+    _check(lib().or_filter_spec(N, int(kc), _dp(om)))
+    return om
+
+
+def run(params: Params, npoint, cx, cy, cz, sigma_line, Gamma_line, n_T_line, if_filter=None):
     """or_run: the whole path; returns omega, u (3, N, N, N) normalised, and stats."""
     N = params.N
     om = np.zeros(3 * N ** 3)
@@ -282,7 +303,8 @@ def run(params: Params, npoint, cx, cy, cz, sigma_line, Gamma_line, n_T_line):
     npt = np.ascontiguousarray(npoint, dtype=np.int32)
     args = [np.ascontiguousarray(a, dtype=np.float64) for a in (cx, cy, cz, sigma_line, Gamma_line)]
     nT = np.ascontiguousarray(n_T_line, dtype=np.int32)
+    flt = None if if_filter is None else np.ascontiguousarray(if_filter, dtype=np.int32)
     _check(lib().or_run(C.byref(params), len(npt), _ip(npt), _dp(args[0]), _dp(args[1]),
-                        _dp(args[2]), _dp(args[3]), _dp(args[4]), _ip(nT), _dp(om), _dp(u),
-                        _dp(st)))
+                        _dp(args[2]), _dp(args[3]), _dp(args[4]), _ip(nT),
+                        None if flt is None else _ip(flt), _dp(om), _dp(u), _dp(st)))
     return om.reshape(3, N, N, N), u.reshape(3, N, N, N), st

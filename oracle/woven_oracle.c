@@ -619,6 +619,33 @@ int or_fft3d(int N, double *data, int forward) {
 /* wavenumber, subroutine.f90:888-919 (box length 2*pi: integer wavenumbers, Nyquist = -N/2) */
 static inline int wnum(int i, int N) { return ((i + N / 2) % N) - N / 2; }
 
+int or_filter_spec(int N, int kc, double *omega) {
+    const int64_t N3 = (int64_t)N * N * N;
+    const double kc2 = (double)kc * (double)kc; /* k_c**2, compared as real*8 (subroutine.f90:1244) */
+    double *w = (double *)malloc(sizeof(double) * 2 * N3);
+    if (!w) return fail("out of memory");
+    for (int c = 0; c < 3; c++) {
+        double *f = omega + c * N3;
+        for (int64_t i = 0; i < N3; i++) { w[2 * i] = f[i]; w[2 * i + 1] = 0.0; }
+        or_fft3d(N, w, 1);
+        for (int k = 0; k < N; k++)
+            for (int j = 0; j < N; j++)
+                for (int i = 0; i < N; i++) {
+                    double kx = wnum(i, N), ky = wnum(j, N), kz = wnum(k, N);
+                    double k2 = (kx * kx + ky * ky) + kz * kz; /* subroutine.f90:914 */
+                    if (k2 > kc2) {
+                        int64_t id = i + (int64_t)N * j + (int64_t)N * N * k;
+                        w[2 * id] = 0.0;
+                        w[2 * id + 1] = 0.0;
+                    }
+                }
+        or_fft3d(N, w, 0);
+        for (int64_t i = 0; i < N3; i++) f[i] = w[2 * i]; /* dreal, subroutine.f90:1083 */
+    }
+    free(w);
+    return 0;
+}
+
 int or_biot_savart(int N, const double *omega, double *u) {
     const int64_t N3 = (int64_t)N * N * N;
     double *w[3], *uh[3];
@@ -773,10 +800,34 @@ int or_sf_x(int N, const double *ux, int pmin, int pmax, int nsep, const int32_t
     return 0;
 }
 
+int or_deposit_filtered(const or_params *p, const double *sigma_line, const double *Gamma_line,
+                        const int32_t *n_T_line, const int32_t *if_filter, const or_geom *g,
+                        double *omega, int64_t *hits_out) {
+    int64_t hits = 0;
+    int l0 = p->line_s - 1, l1 = p->line_e - 1;
+    if (l0 < 0) l0 = 0;
+    if (l1 > g->nline - 1) l1 = g->nline - 1;
+    for (int l = l0; l <= l1; l++) {
+        int64_t h = 0;
+        int rc = or_deposit_range(p, sigma_line, Gamma_line, n_T_line, g, g->seg_base[l], g->seg_base[l + 1],
+                                  omega, &h);
+        if (rc != 0) return rc;
+        hits += h;
+        if (if_filter && if_filter[l]) {
+            double kcd = ceil(4.0 / sigma_line[l]); /* tube.f90:928 */
+            if (!(kcd < 46341.0)) return fail("k_c**2 exceeds the default integer");
+            rc = or_filter_spec(p->N, (int)kcd, omega);
+            if (rc != 0) return rc;
+        }
+    }
+    if (hits_out) *hits_out = hits;
+    return 0;
+}
+
 int or_run(const or_params *p, int nline, const int32_t *npoint, const double *cx,
            const double *cy, const double *cz, const double *sigma_line,
-           const double *Gamma_line, const int32_t *n_T_line, double *omega, double *u,
-           double *stats) {
+           const double *Gamma_line, const int32_t *n_T_line, const int32_t *if_filter,
+           double *omega, double *u, double *stats) {
     struct timespec t0, t1;
     clock_gettime(CLOCK_MONOTONIC, &t0);
     or_geom g;
@@ -786,7 +837,7 @@ int or_run(const or_params *p, int nline, const int32_t *npoint, const double *c
     const int64_t N3 = (int64_t)p->N * p->N * p->N;
     memset(omega, 0, sizeof(double) * 3 * N3);
     int64_t hits = 0;
-    rc = or_deposit(p, sigma_line, Gamma_line, n_T_line, &g, omega, &hits);
+    rc = or_deposit_filtered(p, sigma_line, Gamma_line, n_T_line, if_filter, &g, omega, &hits);
     or_geom_free(&g);
     if (rc != 0) return rc;
     rc = or_biot_savart(p->N, omega, u);

@@ -114,6 +114,17 @@ int or_fft3d(int N, double *data, int forward);
 /* 1-D DFT of length n (any n), forward sign -1 when forward != 0, unscaled. */
 int or_fft1d(int n, double *data, int forward);
 
+/* filter_spec, subroutine.f90:1225-1277: each component of omega (3*N^3) is transformed, every
+ * mode with k2 > kc^2 set to zero, and transformed back (real part kept). */
+int or_filter_spec(int N, int kc, double *omega);
+
+/* The reference's tube loop with filters (tube.f90:529-936): tubes line_s..line_e deposit in
+ * order; after a tube with if_filter[l] != 0 the accumulated omega is filtered with
+ * kc = ceiling(4/sigma_line[l]) (tube.f90:926-929). Adds to omega like or_deposit. */
+int or_deposit_filtered(const or_params *p, const double *sigma_line, const double *Gamma_line,
+                        const int32_t *n_T_line, const int32_t *if_filter, const or_geom *g,
+                        double *omega, int64_t *hits);
+
 /* tube.f90:985-1023: u = Re(IFFT(i k x w_hat / k^2)), DC zeroed. omega, u: 3*N^3. */
 int or_biot_savart(int N, const double *omega, double *u);
 
@@ -138,13 +149,14 @@ int or_sf_x(int N, const double *ux, int pmin, int pmax, int nsep, const int32_t
 /* subroutine.f90:164-173 (sigma_func). */
 double or_sigma(double zeta, double length, double smax, double smin, int n_T);
 
-/* Whole pipeline in the reference order (tube.f90:392-1043): geometry, deposition,
- * Biot-Savart, normalisation. omega and u are 3*N^3 each (outputs, normalised).
- * stats[0..3] as or_normalize; stats[4] = hits; stats[5] = seconds (wall). */
+/* Whole pipeline in the reference order (tube.f90:392-1043): geometry, deposition (with the
+ * per-tube filters when if_filter is not NULL), Biot-Savart, normalisation. omega and u are
+ * 3*N^3 each (outputs, normalised). stats[0..3] as or_normalize; stats[4] = hits;
+ * stats[5] = seconds (wall). */
 int or_run(const or_params *p, int nline, const int32_t *npoint, const double *cx,
            const double *cy, const double *cz, const double *sigma_line,
-           const double *Gamma_line, const int32_t *n_T_line, double *omega, double *u,
-           double *stats);
+           const double *Gamma_line, const int32_t *n_T_line, const int32_t *if_filter,
+           double *omega, double *u, double *stats);
 
 #ifdef __cplusplus
 }
