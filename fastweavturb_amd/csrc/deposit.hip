@@ -131,7 +131,7 @@ __device__ void hit_math_f64(const Geo &g, const SubSeg &sg, int32_t ip, int32_t
 // (spline wrapped to [-pi, pi), then cminus's single +-2pi correction, which is not a full
 // minimum image: cells of an extended box more than 3pi from the wrapped spline point keep a
 // far offset and receive the negative cut-off term, as in the reference). The Gaussian core runs
-// in FP32. Otherwise the curvature terms carry 1/(1 - kappa rho cos), which amplifies rounding
+// in FP32 with the hardware reciprocal and reciprocal square root. Otherwise the curvature terms carry 1/(1 - kappa rho cos), which amplifies rounding
 // near its zero, so the whole reference formula runs in FP64 (hit_math_f64).
 __device__ void hit_math_f32(const Geo &g, const SubSeg &sg, int32_t ip, int32_t jp, int32_t kp, double *v) {
     const int l = sg.tube;
@@ -159,11 +159,11 @@ __device__ void hit_math_f32(const Geo &g, const SubSeg &sg, int32_t ip, int32_t
     {
 #pragma clang fp contract(fast)
         float rho2 = xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2];
-        float indc = 1.0f / sqrtf(dcf[0] * dcf[0] + dcf[1] * dcf[1] + dcf[2] * dcf[2]);
+        float indc = __builtin_amdgcn_rsqf(dcf[0] * dcf[0] + dcf[1] * dcf[1] + dcf[2] * dcf[2]);
         float sigma = (float)smin;
-        float is2 = 1.0f / (sigma * sigma);
+        float is2 = __builtin_amdgcn_rcpf(sigma * sigma);
         float R = (float)sg.R;
-        float norm = (float)(1.0 / (kPi2 * smin * smin));
+        float norm = 0.15915494309189535f * is2;
         float tp = (__expf(-0.5f * rho2 * is2) - __expf(-0.5f * R * R * is2)) * norm;
         float tg = tp * Gamma * indc;
 #pragma unroll
@@ -173,11 +173,11 @@ __device__ void hit_math_f32(const Geo &g, const SubSeg &sg, int32_t ip, int32_t
 
 // The reference's candidate test (tube.f90:803-818) for one extended cell. rho < R is decided
 // on the squared distance unless it lies within 1e-14 (relative) of R^2, where the rounded
-// square root is compared as in the reference.
+// square root is compared as in the reference. The reference's first test, 0 < k <= nzp for
+// k = mod(k_plus + nz - 1, nz) + 1, holds for every extended index in [1 - N, 2N], and the
+// binning keeps every box in that range, so it is not repeated here.
 __device__ __forceinline__ bool pair_hits_fast(const SubSeg &sg, int32_t ip, int32_t jp, int32_t kp, double R2lo,
                                                double R2hi) {
-    int32_t k = wrap_idx(kp, sg.N);
-    if (!(k > 0 && k <= sg.N)) return false;
     double X0 = mesh(ip, sg), X1 = mesh(jp, sg), X2 = mesh(kp, sg);
     double a0 = X0 - sg.ci0[0], a1 = X1 - sg.ci0[1], a2 = X2 - sg.ci0[2];
     double t0 = (a0 * sg.tci0[0] + a1 * sg.tci0[1]) + a2 * sg.tci0[2];
@@ -188,6 +188,14 @@ __device__ __forceinline__ bool pair_hits_fast(const SubSeg &sg, int32_t ip, int
     if (v >= 0.0 && v < R2lo) return true;
     if (!(v < R2hi)) return false;
     return sqrt(v) < sg.R;
+}
+
+// 0-based wrapped index of an extended index in [1 - N, 2N]
+__device__ __forceinline__ int32_t wrap0(int32_t ip, int32_t N) {
+    int32_t x = ip - 1;
+    x += x < 0 ? N : 0;
+    x -= x >= N ? N : 0;
+    return x;
 }
 
 // Element i of a 3-vector by selects (a runtime index into a private array would live in scratch).
@@ -247,6 +255,10 @@ struct HitQ {
     int16_t ip, jp, kp, owner;                       // extended indices, owning lane
 };
 
+// Widening of a band bound x computed as r * rcp(B) (hardware reciprocal, within 1 ulp): covers the
+// reciprocal's and the product's rounding with a wide factor.
+__device__ __forceinline__ float band_slack(float x) { return 1e-6f * fabsf(x) + 1e-5f; }
+
 // b-range of row ic whose columns can hold candidates: the slab interval reaches a-indices
 // a0..a1 (each plane condition is linear in db) and the column crosses the plane t0 = 0 inside
 // the ellipse of radius Rp (a quadratic in db). A superset, clipped to b0..b1; false if empty.
@@ -262,8 +274,8 @@ __device__ __forceinline__ bool row_band(const ColWalk &w, int32_t b0, int32_t b
         if (w.B0 == 0.0f) {
             if (w.up0 ? (0.0f < r) : (0.0f > r)) return false;
         } else {
-            const float x = r / w.B0;
-            if ((w.B0 > 0.0f) == (bool)w.up0) lo = fmaxf(lo, x); else hi = fminf(hi, x);
+            const float x = r * __builtin_amdgcn_rcpf(w.B0), e = band_slack(x);
+            if ((w.B0 > 0.0f) == (bool)w.up0) lo = fmaxf(lo, x - e); else hi = fminf(hi, x + e);
         }
     }
     if (w.use1) {   // plane 1: up1 needs B1 db <= r, else B1 db >= r
@@ -273,8 +285,8 @@ __device__ __forceinline__ bool row_band(const ColWalk &w, int32_t b0, int32_t b
         if (w.B1 == 0.0f) {
             if (w.up1 ? (0.0f > r) : (0.0f < r)) return false;
         } else {
-            const float x = r / w.B1;
-            if ((w.B1 > 0.0f) != (bool)w.up1) lo = fmaxf(lo, x); else hi = fminf(hi, x);
+            const float x = r * __builtin_amdgcn_rcpf(w.B1), e = band_slack(x);
+            if ((w.B1 > 0.0f) != (bool)w.up1) lo = fmaxf(lo, x - e); else hi = fminf(hi, x + e);
         }
     }
     {   // ellipse: (1 + B0^2) Db^2 + 2 B0 C0 Dc Db + (1 + C0^2) Dc^2 < Rp^2
@@ -283,7 +295,7 @@ __device__ __forceinline__ bool row_band(const ColWalk &w, int32_t b0, int32_t b
         const float cc = __builtin_fmaf(__builtin_fmaf(w.C0, w.C0, 1.0f) * Dc, Dc, -w.Rp2);
         const float D = __builtin_fmaf(ebc, ebc, -eb * cc);
         if (!(D >= 0.0f)) return false;
-        const float sq = sqrtf(D), ie = 1.0f / eb;
+        const float sq = __builtin_amdgcn_sqrtf(D), ie = __builtin_amdgcn_rcpf(eb);
         lo = fmaxf(lo, w.ub + (-ebc - sq) * ie - 1e-3f);
         hi = fminf(hi, w.ub + (-ebc + sq) * ie + 1e-3f);
     }
@@ -370,8 +382,7 @@ __device__ __forceinline__ void stage_hits(Wave<F64> &W, const SegRec &me, int c
         if (FAST) hit_math_f32(W.A->g, sg, h.ip, h.jp, h.kp, v);
         else hit_math_f64(W.A->g, sg, h.ip, h.jp, h.kp, v);
         const int N = W.ri->N;
-        const int cell = (wrap_idx(h.ip, N) - 1 - W.ox) +
-                         kBrickX * ((wrap_idx(h.jp, N) - 1 - W.oy) + kBrickY * (wrap_idx(h.kp, N) - 1 - W.oz));
+        const int cell = (wrap0(h.ip, N) - W.ox) + kBrickX * ((wrap0(h.jp, N) - W.oy) + kBrickY * (wrap0(h.kp, N) - W.oz));
         if (F64) {
 #pragma unroll
             for (int c = 0; c < 3; c++) unsafeAtomicAdd((double *)W.acc + c * kCells + cell, v[c]);
