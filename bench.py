@@ -271,6 +271,7 @@ def c3_bench(densities=(0.5, 1.0, 2.0), tile_cache: str | None = None) -> dict:
         log(f"[c3] density {d}: {t.nline} tubes, {t.npoint_all} points (built in {time.time() - t0:.1f}s)")
         g.set_params(p)
         g.set_tubes(t)
+        g.generate()                       # warm-up: grows the pair list and tube arenas once
         t0 = time.perf_counter()
         etot, up, ens = g.generate()
         t_gen = time.perf_counter() - t0

@@ -46,6 +46,67 @@ struct DepArgs {
     int accumulate;               // 0: the brick's cells are written; 1: added to omega
 };
 
+// subseg_load for a sub-segment that the binning has already accepted (so it starts a deposited
+// sub-segment of a tube in line_s..line_e of the brick's instance `ri`), with a shorter chain of
+// dependent loads: the sub-point's section and tube, and the four sub-points q..q+3 (the last
+// sub-segment of a section takes its second chord from the next section's first two sub-points,
+// which are q+2 and q+3 there), are read at once. Same values as subseg_load.
+__device__ __forceinline__ void subseg_load_pair(const Geo &g, const Run &ri, int64_t q, SubSeg &o) {
+    const int32_t s = g.sub_sec[q], l = g.sub_tube[q];
+    double p[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) p[k] = g.ci[3 * q + k];
+    const int32_t j = (int32_t)(q - g.sp_base[s]);
+    const int32_t nt = g.ntf[s];
+    const double czl = g.czl[s], dtf = g.dtf[s];
+    const double smin = g.sigma[l], len = g.length[l];
+    const int nT = g.n_T[l];
+    o.N = ri.N;
+    o.dx = ri.dx;
+    o.start = ri.start;
+    o.field_off = ri.field_off;
+    o.sec = s;
+    o.tube = l;
+    o.j = j;
+    auto chord_of = [](const double *b, const double *a, double *t, double *nd) {
+        double d[3];
+        cminus(b, a, d);
+        const double n = norm3(d);
+        *nd = n;
+#pragma unroll
+        for (int c = 0; c < 3; c++) t[c] = d[c] / n;
+    };
+    double nd1;
+    chord_of(p + 3, p, o.tci0, &o.ndci0);
+    if (j + 1 < nt) chord_of(p + 6, p + 3, o.tci1, &nd1);
+    else chord_of(p + 9, p + 6, o.tci1, &nd1);
+    o.s0 = czl + dtf * (double)j;
+    o.s1 = czl + dtf * (double)(j + 1);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        o.ci0[c] = p[c];
+        o.ci1[c] = p[3 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {  // mark1 (tube.f90:779-785)
+        if (o.ci1[c] - o.ci0[c] > kPi) o.ci0[c] = o.ci0[c] + kPi2;
+        else if (o.ci1[c] - o.ci0[c] < -kPi) o.ci1[c] = o.ci1[c] + kPi2;
+    }
+    const double smax = ri.sigma_ratio * smin;
+    double sig = sigma_func(o.s0, len, smax, smin, nT);
+    const double R = ri.Rtube_ratio * sig;
+    sig = sigma_func(o.s1, len, smax, smin, nT);
+    const double R2 = ri.Rtube_ratio * sig;
+    o.R = (R2 > R) ? R2 : R;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {  // tube.f90:793-798
+        const double mn = o.ci0[c] < o.ci1[c] ? o.ci0[c] : o.ci1[c];
+        const double mx = o.ci0[c] > o.ci1[c] ? o.ci0[c] : o.ci1[c];
+        o.lo[c] = (int32_t)floor((mn - ri.start - o.R) / ri.dx + 1.0);
+        o.hi[c] = (int32_t)ceil((mx - ri.start + o.R) / ri.dx + 1.0);
+    }
+}
+
 __device__ __forceinline__ unsigned long long lanemask_lt() {
     unsigned lane = __lane_id();
     return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
@@ -531,7 +592,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
             bool have = pi < p1;
             {
                 SubSeg sg;
-                if (have) have = subseg_load(g, A.r, (int64_t)A.b.pairs[pi], sg);
+                if (have) subseg_load_pair(g, ri, (int64_t)A.b.pairs[pi], sg);
                 if (have) {
 #pragma unroll
                     for (int c = 0; c < 3; c++) {

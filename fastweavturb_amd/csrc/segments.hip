@@ -334,7 +334,11 @@ __global__ void k_theta_listed(Geo g, Run r, const int32_t *__restrict__ redo,
 __global__ void k_fill_sub_sec(Geo g) {
     int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= g.nsec_all) return;
-    for (int64_t q = g.sp_base[s]; q < g.sp_base[s + 1]; q++) g.sub_sec[q] = (int32_t)s;
+    const int32_t l = g.sec_tube[s];
+    for (int64_t q = g.sp_base[s]; q < g.sp_base[s + 1]; q++) {
+        g.sub_sec[q] = (int32_t)s;
+        g.sub_tube[q] = l;
+    }
 }
 
 // tube.f90:706-713 sub-points; tube.f90:719-723 seam: the last sub-point of a real section is
@@ -446,6 +450,7 @@ hipError_t launch_segments(wt_ctx *c) {
     if (nsub >= (int64_t)UINT32_MAX) return hipErrorInvalidImage;
     g.nsub_all = nsub;
     WT_HIP(dalloc(c, c->seg_allocs, &g.sub_sec, nsub));
+    WT_HIP(dalloc(c, c->seg_allocs, &g.sub_tube, nsub));
     WT_HIP(dalloc(c, c->seg_allocs, &g.ci, 3 * nsub));
     hipLaunchKernelGGL(k_fill_sub_sec, dim3(grid_for(g.nsec_all, bs)), dim3(bs), 0, st, g);
     kt_begin(c, kSlotSubpoints);

@@ -67,6 +67,7 @@ struct Geo {
     int64_t *sp_base;           // [S+1]
     // per sub-point
     int32_t *sub_sec;           // [NSP]
+    int32_t *sub_tube;          // [NSP] tube of each sub-point
     double *ci;                 // [3*NSP] xyz interleaved
 };
 
