@@ -12,10 +12,11 @@
 // column the two end planes give in closed form the short index range that can pass; only
 // those candidates get the reference's exact FP64 test (the square root of the distance test
 // is taken only when the squared distance is within 1e-14 of R^2), so the set of depositing
-// (cell, sub-segment) pairs is exactly the reference's. Hits are compacted per wavefront
-// through an LDS queue and evaluated 64 at a time with every lane busy; the evaluating lane
-// reads the hit's sub-segment from the owning lane's LDS record, so no sub-segment is
-// re-derived per hit.
+// (cell, sub-segment) pairs is exactly the reference's. Candidates are compacted per wavefront
+// through an LDS queue and tested 64 at a time with every lane busy; the testing lane reads the
+// candidate's sub-segment from the owning lane's registers by cross-lane shuffles, so no
+// sub-segment is re-derived per candidate, and evaluates the contribution in place when the
+// candidate is a hit (about 86% of them are).
 //
 // Precision modes:
 //   double: the reference formulas in the reference order; LDS accumulation in FP64.
@@ -194,17 +195,34 @@ __device__ void hit_math_f64(const Geo &g, const SubSeg &sg, int32_t ip, int32_t
 // far offset and receive the negative cut-off term, as in the reference). The Gaussian core runs
 // in FP32 with the hardware reciprocal and reciprocal square root. Otherwise the curvature terms carry 1/(1 - kappa rho cos), which amplifies rounding
 // near its zero, so the whole reference formula runs in FP64 (hit_math_f64).
-__device__ void hit_math_f32(const Geo &g, const SubSeg &sg, int32_t ip, int32_t jp, int32_t kp, double *v) {
-    const int l = sg.tube;
-    const double LD = g.Len_Disc[l];
+// Per-hit values that live in global memory (the section's spline and the tube's constants):
+// loaded before the candidate test so that their latency overlaps it.
+struct HitConst {
+    double coe[18];
+    double LD, czl, smin;
+    float Gamma;
+};
+
+__device__ __forceinline__ void hit_const_load(const Geo &g, const SubSeg &sg, HitConst &hc) {
     const double *coe = g.coe + 18 * (int64_t)sg.sec;
+#pragma unroll
+    for (int k = 0; k < 18; k++) hc.coe[k] = coe[k];
+    hc.LD = g.Len_Disc[sg.tube];
+    hc.czl = g.czl[sg.sec];
+    hc.smin = g.sigma[sg.tube];
+    hc.Gamma = (float)g.gamma[sg.tube];
+}
+
+__device__ void hit_math_f32(const HitConst &hc, const SubSeg &sg, int32_t ip, int32_t jp, int32_t kp, double *v) {
+    const double LD = hc.LD;
+    const double *coe = hc.coe;
     double X[3] = {mesh(ip, sg), mesh(jp, sg), mesh(kp, sg)};
     double a[3] = {X[0] - sg.ci0[0], X[1] - sg.ci0[1], X[2] - sg.ci0[2]};
     double t0 = (a[0] * sg.tci0[0] + a[1] * sg.tci0[1]) + a[2] * sg.tci0[2];
     double t1 = ((X[0] - sg.ci1[0]) * sg.tci0[0] + (X[1] - sg.ci1[1]) * sg.tci0[1]) +
                 (X[2] - sg.ci1[2]) * sg.tci0[2];
     double zeta = (sg.s1 * t0 - sg.s0 * t1) / sg.ndci0;
-    double z = (zeta - g.czl[sg.sec]) * LD;
+    double z = (zeta - hc.czl) * LD;
     double cz3[3], dc[3], xcd[3];
     spline_pos(z, coe, cz3);
     spline_d1(z, coe, dc);
@@ -215,8 +233,8 @@ __device__ void hit_math_f32(const Geo &g, const SubSeg &sg, int32_t ip, int32_t
         xc[c] = (float)xcd[c];
         dcf[c] = (float)dc[c];
     }
-    const float Gamma = (float)g.gamma[l];
-    const double smin = g.sigma[l];
+    const float Gamma = hc.Gamma;
+    const double smin = hc.smin;
     {
 #pragma clang fp contract(fast)
         float rho2 = xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2];
@@ -413,8 +431,7 @@ __device__ __forceinline__ void fetch_owner(const SegRec &me, int own, const Run
 }
 
 // Per-wave work state of the deposition pipeline: pairs (lanes) -> row segments -> columns ->
-// candidates -> hits; the candidate and hit stages run on 64 entries at a time with every lane
-// busy.
+// candidates (tested and deposited 64 at a time with every lane busy).
 template <bool F64>
 struct Wave {
     using Acc = typename AccT<F64>::type;
@@ -424,46 +441,15 @@ struct Wave {
     ColWalk *walks;     // [64]
     RowSeg *segs;       // [128]
     HitQ *candq;        // [128]
-    HitQ *hitq;         // [128]
     int ox, oy, oz;
     int lane;
-    int segn, candn, hitn;
+    int segn, candn;
     unsigned long long ncols, ncand, nhit, novf;
 };
 
-// Evaluates hits [hitn - cnt, hitn) and adds them to the brick.
-template <bool F64, bool FAST>
-__device__ __forceinline__ void stage_hits(Wave<F64> &W, const SegRec &me, int cnt) {
-    const int lane = W.lane;
-    const HitQ h = W.hitq[W.hitn - cnt + (lane < cnt ? lane : 0)];
-    SubSeg sg;
-    fetch_owner(me, lane < cnt ? h.owner : lane, *W.ri, sg);
-    if (lane < cnt) {
-        double v[3];
-        if (FAST) hit_math_f32(W.A->g, sg, h.ip, h.jp, h.kp, v);
-        else hit_math_f64(W.A->g, sg, h.ip, h.jp, h.kp, v);
-        const int N = W.ri->N;
-        const int cell = (wrap0(h.ip, N) - W.ox) + kBrickX * ((wrap0(h.jp, N) - W.oy) + kBrickY * (wrap0(h.kp, N) - W.oz));
-        if (F64) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) unsafeAtomicAdd((double *)W.acc + c * kCells + cell, v[c]);
-        } else {
-            bool ovf = false;
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                double s = v[c] * W.A->fx_scale;
-                if (!(fabs(s) < W.A->fx_limit)) { ovf = true; s = 0.0; }
-                long long fx = (long long)rint(s);
-                atomicAdd((unsigned long long *)W.acc + c * kCells + cell, (unsigned long long)fx);
-            }
-            W.novf += ovf ? 1ull : 0ull;
-        }
-    }
-    wave_sync_lds();
-    W.hitn -= cnt;
-}
-
-// Tests candidates [candn - cnt, candn) with the reference predicate; hits go to the hit queue.
+// Tests candidates [candn - cnt, candn) with the reference predicate and adds the contribution of
+// each hit to the brick (most candidates are hits, so they are evaluated in place rather than
+// compacted again). In FP32 mode the hit's spline and tube constants are loaded before the test.
 template <bool F64, bool FAST>
 __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me, int cnt) {
     const int lane = W.lane;
@@ -474,17 +460,34 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
     fetch_owner(me, lane < cnt ? h.owner : lane, *W.ri, sg);
     bool hit = false;
     if (lane < cnt) {
+        HitConst hc;
+        if (FAST) hit_const_load(W.A->g, sg, hc);
         const double R2 = sg.R * sg.R;
         hit = pair_hits_fast(sg, h.ip, h.jp, h.kp, R2 * (1.0 - 1e-14), R2 * (1.0 + 1e-14));
+        if (hit) {
+            double v[3];
+            if (FAST) hit_math_f32(hc, sg, h.ip, h.jp, h.kp, v);
+            else hit_math_f64(W.A->g, sg, h.ip, h.jp, h.kp, v);
+            const int N = W.ri->N;
+            const int cell = (wrap0(h.ip, N) - W.ox) + kBrickX * ((wrap0(h.jp, N) - W.oy) + kBrickY * (wrap0(h.kp, N) - W.oz));
+            if (F64) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) unsafeAtomicAdd((double *)W.acc + c * kCells + cell, v[c]);
+            } else {
+                bool ovf = false;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    double s = v[c] * W.A->fx_scale;
+                    if (!(fabs(s) < W.A->fx_limit)) { ovf = true; s = 0.0; }
+                    long long fx = (long long)rint(s);
+                    atomicAdd((unsigned long long *)W.acc + c * kCells + cell, (unsigned long long)fx);
+                }
+                W.novf += ovf ? 1ull : 0ull;
+            }
+        }
     }
     const unsigned long long m = __ballot(hit);
-    if (m) {
-        if (hit) W.hitq[W.hitn + __popcll(m & lanemask_lt())] = h;
-        W.hitn += __popcll(m);
-        W.nhit += lane == 0 ? (unsigned long long)__popcll(m) : 0ull;
-        wave_sync_lds();
-        if (W.hitn >= 64) stage_hits<F64, FAST>(W, me, 64);
-    }
+    W.nhit += lane == 0 ? (unsigned long long)__popcll(m) : 0ull;
 }
 
 // Walks the columns of row segments [segn - cnt, segn), one segment per lane and one column of
@@ -537,7 +540,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     Acc *acc = (Acc *)smem;                                                  // 3 * kCells
     unsigned char *wbase = smem + sizeof(Acc) * 3 * kCells;
-    constexpr size_t kWaveBytes = sizeof(ColWalk) * 64 + sizeof(RowSeg) * 128 + sizeof(HitQ) * 256;
+    constexpr size_t kWaveBytes = sizeof(ColWalk) * 64 + sizeof(RowSeg) * 128 + sizeof(HitQ) * 128;
     unsigned long long *s_misc = (unsigned long long *)(wbase + kWaveBytes * kDepWaves);  // brick, cursor
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -551,10 +554,9 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
         W.segs = (RowSeg *)p;
         p += sizeof(RowSeg) * 128;
         W.candq = (HitQ *)p;
-        W.hitq = W.candq + 128;
     }
     W.lane = lane;
-    W.segn = W.candn = W.hitn = 0;
+    W.segn = W.candn = 0;
     W.ncols = W.ncand = W.nhit = W.novf = 0;
     const Geo &g = A.g;
     for (;;) {
@@ -693,7 +695,6 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
             // the registers of these pairs are replaced by the next chunk: flush every stage
             if (W.segn > 0) stage_columns<F64, FAST>(W, me, W.segn);
             if (W.candn > 0) stage_candidates<F64, FAST>(W, me, W.candn);
-            if (W.hitn > 0) stage_hits<F64, FAST>(W, me, W.hitn);
         }
         __syncthreads();
         // the brick's cells are owned by this workgroup: plain stores (or read-add-store), once
@@ -738,7 +739,7 @@ template <bool F64, bool FAST>
 static hipError_t deposit_t(wt_ctx *c, DepArgs &A) {
     hipStream_t st = c->stream;
     using Acc = typename AccT<F64>::type;
-    const size_t wave_bytes = sizeof(ColWalk) * 64 + sizeof(RowSeg) * 128 + sizeof(HitQ) * 256;
+    const size_t wave_bytes = sizeof(ColWalk) * 64 + sizeof(RowSeg) * 128 + sizeof(HitQ) * 128;
     const size_t lds = sizeof(Acc) * 3 * kCells + wave_bytes * kDepWaves + 2 * sizeof(unsigned long long);
     const void *fn = (const void *)k_deposit<F64, FAST>;
     WT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
