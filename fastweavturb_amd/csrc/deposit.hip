@@ -48,12 +48,13 @@ struct DepArgs {
 };
 
 // subseg_load for a sub-segment that the binning has already accepted (so it starts a deposited
-// sub-segment of a tube in line_s..line_e of the brick's instance `ri`), with a shorter chain of
-// dependent loads: the sub-point's section and tube, and the four sub-points q..q+3 (the last
-// sub-segment of a section takes its second chord from the next section's first two sub-points,
-// which are q+2 and q+3 there), are read at once. Same values as subseg_load.
-__device__ __forceinline__ void subseg_load_pair(const Geo &g, const Run &ri, int64_t q, SubSeg &o) {
-    const int32_t s = g.sub_sec[q], l = g.sub_tube[q];
+// sub-segment of a tube in line_s..line_e of the brick's instance `ri`), given its section s and
+// tube l (sub_sec[q], sub_tube[q]), with a shorter chain of dependent loads: the four sub-points
+// q..q+3 (the last sub-segment of a section takes its second chord from the next section's first
+// two sub-points, which are q+2 and q+3 there) and the section's and tube's values are read at
+// once. Same values as subseg_load.
+__device__ __forceinline__ void subseg_load_pair(const Geo &g, const Run &ri, int64_t q, int32_t s, int32_t l,
+                                                 SubSeg &o) {
     double p[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) p[k] = g.ci[3 * q + k];
@@ -581,12 +582,31 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
         for (int i = threadIdx.x; i < 3 * kCells; i += kDepThreads) acc[i] = (Acc)0;
         __syncthreads();
         const int64_t p0 = A.b.brick_off[bi], p1 = A.b.brick_off[bi + 1];
+        // chunks of 64 pairs are claimed one step ahead, and the next chunk's pair index and image
+        // read while the current one is processed
+        unsigned long long next = 0;
+        if (lane == 0) next = atomicAdd(&s_misc[1], 64ull);
+        next = __shfl(next, 0, 64);
+        uint32_t q_next = 0;
+        int32_t code_next = 0, s_next = 0, l_next = 0;
+        if (p0 + (int64_t)next + lane < p1) {
+            q_next = A.b.pairs[p0 + (int64_t)next + lane];
+            code_next = A.b.pair_img[p0 + (int64_t)next + lane];
+            s_next = g.sub_sec[q_next];
+            l_next = g.sub_tube[q_next];
+        }
         for (;;) {
-            unsigned long long chunk = 0;
-            if (lane == 0) chunk = atomicAdd(&s_misc[1], 64ull);
-            chunk = __shfl(chunk, 0, 64);
+            const unsigned long long chunk = next;
             if (p0 + (int64_t)chunk >= p1) break;
             const int64_t pi = p0 + (int64_t)chunk + lane;
+            const uint32_t q_cur = q_next;
+            const int32_t code = code_next, s_cur = s_next, l_cur = l_next;
+            if (lane == 0) next = atomicAdd(&s_misc[1], 64ull);
+            next = __shfl(next, 0, 64);
+            if (p0 + (int64_t)next + lane < p1) {
+                q_next = A.b.pairs[p0 + (int64_t)next + lane];
+                code_next = A.b.pair_img[p0 + (int64_t)next + lane];
+            }
             // ---- stage 0 (lane = pair): the sub-segment to registers, its column walk to LDS ----
             SegRec me;
             ColWalk w;
@@ -594,7 +614,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
             bool have = pi < p1;
             {
                 SubSeg sg;
-                if (have) subseg_load_pair(g, ri, (int64_t)A.b.pairs[pi], sg);
+                if (have) subseg_load_pair(g, ri, (int64_t)q_cur, s_cur, l_cur, sg);
                 if (have) {
 #pragma unroll
                     for (int c = 0; c < 3; c++) {
@@ -610,7 +630,6 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                     me.sec = sg.sec;
                     me.tube = sg.tube;
                     // the pair's cells on each axis: the box within the brick's periodic image
-                    const int32_t code = A.b.pair_img[pi];
                     const int32_t img[3] = {code % 3, (code / 3) % 3, code / 9};
                     const int32_t org[3] = {ox, oy, oz}, ext[3] = {ex, ey, ez};
                     int32_t clo[3], chi[3];
@@ -681,6 +700,10 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                 }
             }
             wave_sync_lds();
+            if (p0 + (int64_t)next + lane < p1) {   // q_next has arrived by now
+                s_next = g.sub_sec[q_next];
+                l_next = g.sub_tube[q_next];
+            }
             // ---- stage 1 (lane = pair): one row of the pair per step, non-empty bands queued ----
             for (int32_t ic = c0; __ballot(have && ic <= c1); ic++) {
                 int32_t bl = 0, bh = -1;
