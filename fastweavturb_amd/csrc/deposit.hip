@@ -344,44 +344,46 @@ __device__ __forceinline__ float band_slack(float x) { return 1e-6f * fabsf(x) +
 // the ellipse of radius Rp (a quadratic in db). A superset, clipped to b0..b1; false if empty.
 __device__ __forceinline__ bool row_band(const ColWalk &w, int32_t b0, int32_t b1, int32_t ic, int32_t &bl,
                                          int32_t &bh) {
+    // branch-free: every condition is evaluated and the flags select
     const float dcn = (float)(ic - w.lo_c);
     const float A0 = (float)(w.a0 - w.lo_a), A1 = (float)(w.a1 - w.lo_a);
     float lo = -1e9f, hi = 1e9f;
+    bool ok = true;
     {   // plane 0: up0 needs B0 db >= r, else B0 db <= r
         const float q0 = __builtin_fmaf(-w.C0, dcn, w.P0);
         const float m2 = 2.0f * w.m0;
         const float r = w.up0 ? q0 - m2 - A1 : q0 + m2 - A0;
-        if (w.B0 == 0.0f) {
-            if (w.up0 ? (0.0f < r) : (0.0f > r)) return false;
-        } else {
-            const float x = r * __builtin_amdgcn_rcpf(w.B0), e = band_slack(x);
-            if ((w.B0 > 0.0f) == (bool)w.up0) lo = fmaxf(lo, x - e); else hi = fminf(hi, x + e);
-        }
+        const bool z = w.B0 == 0.0f;
+        ok = ok && !(z && (w.up0 ? (0.0f < r) : (0.0f > r)));
+        const float x = r * __builtin_amdgcn_rcpf(z ? 1.0f : w.B0), e = band_slack(x);
+        const bool lower = (w.B0 > 0.0f) == (bool)w.up0;
+        lo = (!z && lower) ? fmaxf(lo, x - e) : lo;
+        hi = (!z && !lower) ? fminf(hi, x + e) : hi;
     }
-    if (w.use1) {   // plane 1: up1 needs B1 db <= r, else B1 db >= r
+    {   // plane 1 (use1): up1 needs B1 db <= r, else B1 db >= r
         const float q1 = __builtin_fmaf(-w.C1, dcn, w.P1);
         const float m2 = 2.0f * w.m1;
         const float r = w.up1 ? q1 + m2 - A0 : q1 - m2 - A1;
-        if (w.B1 == 0.0f) {
-            if (w.up1 ? (0.0f > r) : (0.0f < r)) return false;
-        } else {
-            const float x = r * __builtin_amdgcn_rcpf(w.B1), e = band_slack(x);
-            if ((w.B1 > 0.0f) != (bool)w.up1) lo = fmaxf(lo, x - e); else hi = fminf(hi, x + e);
-        }
+        const bool z = w.B1 == 0.0f, use = w.use1;
+        ok = ok && !(use && z && (w.up1 ? (0.0f > r) : (0.0f < r)));
+        const float x = r * __builtin_amdgcn_rcpf(z ? 1.0f : w.B1), e = band_slack(x);
+        const bool lower = (w.B1 > 0.0f) != (bool)w.up1;
+        lo = (use && !z && lower) ? fmaxf(lo, x - e) : lo;
+        hi = (use && !z && !lower) ? fminf(hi, x + e) : hi;
     }
     {   // ellipse: (1 + B0^2) Db^2 + 2 B0 C0 Dc Db + (1 + C0^2) Dc^2 < Rp^2
         const float Dc = dcn - w.uc;
         const float eb = __builtin_fmaf(w.B0, w.B0, 1.0f), ebc = w.B0 * w.C0 * Dc;
         const float cc = __builtin_fmaf(__builtin_fmaf(w.C0, w.C0, 1.0f) * Dc, Dc, -w.Rp2);
         const float D = __builtin_fmaf(ebc, ebc, -eb * cc);
-        if (!(D >= 0.0f)) return false;
-        const float sq = __builtin_amdgcn_sqrtf(D), ie = __builtin_amdgcn_rcpf(eb);
+        ok = ok && D >= 0.0f;
+        const float sq = __builtin_amdgcn_sqrtf(fmaxf(D, 0.0f)), ie = __builtin_amdgcn_rcpf(eb);
         lo = fmaxf(lo, w.ub + (-ebc - sq) * ie - 1e-3f);
         hi = fminf(hi, w.ub + (-ebc + sq) * ie + 1e-3f);
     }
     lo = fmaxf(lo, -1e9f);
     hi = fminf(hi, 1e9f);
-    if (!(lo <= hi)) return false;
+    if (!ok || !(lo <= hi)) return false;
     const int32_t l = w.lo_b + (int32_t)ceilf(lo), h = w.lo_b + (int32_t)floorf(hi);
     bl = l > b0 ? l : b0;
     bh = h < b1 ? h : b1;
@@ -392,14 +394,15 @@ __device__ __forceinline__ bool row_band(const ColWalk &w, int32_t b0, int32_t b
 __device__ __forceinline__ int32_t column_range(const ColWalk &w, int32_t ib, int32_t ic, int32_t *ia0) {
     const float db = (float)(ib - w.lo_b), dcn = (float)(ic - w.lo_c);
     float flo = (float)(w.a0 - w.lo_a), fhi = (float)(w.a1 - w.lo_a);
+    // branch-free: both bounds of each plane are formed and the flags select (P1 = B1 = C1 = 0
+    // when use1 is clear)
     const float p0 = __builtin_fmaf(-w.B0, db, __builtin_fmaf(-w.C0, dcn, w.P0));
-    if (w.up0) flo = fmaxf(flo, ceilf(p0 - w.m0));
-    else fhi = fminf(fhi, floorf(p0 + w.m0));
-    if (w.use1) {
-        const float p1 = __builtin_fmaf(-w.B1, db, __builtin_fmaf(-w.C1, dcn, w.P1));
-        if (w.up1) fhi = fminf(fhi, floorf(p1 + w.m1));
-        else flo = fmaxf(flo, ceilf(p1 - w.m1));
-    }
+    const float p1 = __builtin_fmaf(-w.B1, db, __builtin_fmaf(-w.C1, dcn, w.P1));
+    const bool up0 = w.up0, lo1 = w.use1 && !w.up1, hi1 = w.use1 && w.up1;
+    flo = fmaxf(flo, up0 ? ceilf(p0 - w.m0) : flo);
+    fhi = fminf(fhi, up0 ? fhi : floorf(p0 + w.m0));
+    flo = fmaxf(flo, lo1 ? ceilf(p1 - w.m1) : flo);
+    fhi = fminf(fhi, hi1 ? floorf(p1 + w.m1) : fhi);
     *ia0 = w.lo_a + (int32_t)flo;
     return flo <= fhi ? (int32_t)(fhi - flo) + 1 : 0;
 }
@@ -445,7 +448,7 @@ struct Wave {
     int ox, oy, oz;
     int lane;
     int segn, candn;
-    unsigned long long ncols, ncand, nhit, novf;
+    unsigned long long ncols, ncand, nhit, novf;   // wave-uniform counts
 };
 
 // Tests candidates [candn - cnt, candn) with the reference predicate and adds the contribution of
@@ -459,7 +462,7 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
     W.candn -= cnt;
     SubSeg sg;
     fetch_owner(me, lane < cnt ? h.owner : lane, *W.ri, sg);
-    bool hit = false;
+    bool hit = false, ovf = false;
     if (lane < cnt) {
         HitConst hc;
         if (FAST) hit_const_load(W.A->g, sg, hc);
@@ -475,7 +478,6 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
 #pragma unroll
                 for (int c = 0; c < 3; c++) unsafeAtomicAdd((double *)W.acc + c * kCells + cell, v[c]);
             } else {
-                bool ovf = false;
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
                     double s = v[c] * W.A->fx_scale;
@@ -483,12 +485,11 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
                     long long fx = (long long)rint(s);
                     atomicAdd((unsigned long long *)W.acc + c * kCells + cell, (unsigned long long)fx);
                 }
-                W.novf += ovf ? 1ull : 0ull;
             }
         }
     }
-    const unsigned long long m = __ballot(hit);
-    W.nhit += lane == 0 ? (unsigned long long)__popcll(m) : 0ull;
+    W.nhit += (unsigned long long)__popcll(__ballot(hit));
+    W.novf += (unsigned long long)__popcll(__ballot(ovf));
 }
 
 // Walks the columns of row segments [segn - cnt, segn), one segment per lane and one column of
@@ -509,12 +510,11 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
     const int ax = lane < cnt ? (int)w.ax : 0;
     const int bx = ax == 2 ? 0 : ax + 1;
     W.segn -= cnt;
-    for (int32_t col = 0; __ballot(col < len); col++, ib++) {
+    unsigned long long mc;
+    for (int32_t col = 0; (mc = __ballot(col < len)) != 0ull; col++, ib++) {
+        W.ncols += (unsigned long long)__popcll(mc);
         int32_t nc = 0, ia0 = 0;
-        if (col < len) {
-            nc = column_range(w, ib, ic, &ia0);
-            W.ncols++;
-        }
+        if (col < len) nc = column_range(w, ib, ic, &ia0);
         for (int u = 0; __ballot(u < nc); u++) {
             const bool c = u < nc;
             const unsigned long long m = __ballot(c);
@@ -526,9 +526,9 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
                 q.kp = (int16_t)(ax == 2 ? ia : (bx == 2 ? ib : ic));
                 q.owner = (int16_t)own;
                 W.candq[W.candn + __popcll(m & lanemask_lt())] = q;
-                W.ncand++;
             }
             W.candn += __popcll(m);
+            W.ncand += (unsigned long long)__popcll(m);
             wave_sync_lds();
             if (W.candn >= 64) stage_candidates<F64, FAST>(W, me, 64);
         }
@@ -742,19 +742,12 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
         }
         __syncthreads();
     }
-    // counters: one atomic per wave
-    unsigned long long cols = W.ncols, cands = W.ncand, ov = W.novf;
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        cols += __shfl_xor(cols, o, 64);
-        cands += __shfl_xor(cands, o, 64);
-        ov += __shfl_xor(ov, o, 64);
-    }
+    // counters (wave-uniform): one atomic per wave
     if (lane == 0) {
-        atomicAdd(A.counters + 0, cols);
-        atomicAdd(A.counters + 1, cands);
+        atomicAdd(A.counters + 0, W.ncols);
+        atomicAdd(A.counters + 1, W.ncand);
         atomicAdd(A.counters + 2, W.nhit);
-        if (ov) atomicAdd(A.counters + 3, ov);
+        if (W.novf) atomicAdd(A.counters + 3, W.novf);
     }
 }
 
