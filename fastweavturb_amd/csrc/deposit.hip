@@ -571,6 +571,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
         const int4 bt = A.b.brick_tab[bi];
         const Run ri = g.runs[bt.x];
         W.ri = &ri;
+        const double inv_dx = 1.0 / ri.dx;   // cells per unit length
         const int N = ri.N;
         const int ox = bt.y * kBrickX, oy = bt.z * kBrickY, oz = bt.w * kBrickZ;
         const int ex = N - ox < kBrickX ? N - ox : kBrickX;
@@ -651,12 +652,11 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                     b1 = pick3(chi, bx);
                     c0 = pick3(clo, cx);
                     c1 = pick3(chi, cx);
-                    const double idx = 1.0 / sg.dx;
-                    const double t0a = pick3(sg.tci0, ax);
-                    const double ua = (pick3(sg.ci0, ax) - sg.start) * idx + 1.0 - w.lo_a;
-                    const double ub = (pick3(sg.ci0, bx) - sg.start) * idx + 1.0 - w.lo_b;
-                    const double uc = (pick3(sg.ci0, cx) - sg.start) * idx + 1.0 - w.lo_c;
-                    const double B0 = pick3(sg.tci0, bx) / t0a, C0 = pick3(sg.tci0, cx) / t0a;
+                    const double t0a = pick3(sg.tci0, ax), it0a = 1.0 / t0a;
+                    const double ua = (pick3(sg.ci0, ax) - sg.start) * inv_dx + 1.0 - w.lo_a;
+                    const double ub = (pick3(sg.ci0, bx) - sg.start) * inv_dx + 1.0 - w.lo_b;
+                    const double uc = (pick3(sg.ci0, cx) - sg.start) * inv_dx + 1.0 - w.lo_c;
+                    const double B0 = pick3(sg.tci0, bx) * it0a, C0 = pick3(sg.tci0, cx) * it0a;
                     const double Wb = pick3(sg.hi, bx) - pick3(sg.lo, bx), Wc = pick3(sg.hi, cx) - pick3(sg.lo, cx);
                     w.B0 = (float)B0;
                     w.C0 = (float)C0;
@@ -676,15 +676,16 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                     const double cd = (sg.tci0[0] * sg.tci1[0] + sg.tci0[1] * sg.tci1[1]) + sg.tci0[2] * sg.tci1[2];
                     double Rp = 1e30;
                     if (cd > 0.2) {
-                        const double tand = sqrt(fmax(1.0 - cd * cd, 0.0)) / cd;
-                        Rp = (sg.R + (sg.ndci0 + sg.R * tand) * sqrt(fmax(1.0 - t0a * t0a, 0.0)) / fabs(t0a)) * idx;
+                        const double tand = sqrt(fmax(1.0 - cd * cd, 0.0)) * (1.0 / cd);
+                        Rp = (sg.R + (sg.ndci0 + sg.R * tand) * sqrt(fmax(1.0 - t0a * t0a, 0.0)) * fabs(it0a)) * inv_dx;
                     }
                     w.Rp2 = (float)fmin(Rp * Rp * (1.0 + 1e-5) + 1e-3, 1e30);
                     if (w.use1) {
-                        const double va = (pick3(sg.ci1, ax) - sg.start) * idx + 1.0 - w.lo_a;
-                        const double vb = (pick3(sg.ci1, bx) - sg.start) * idx + 1.0 - w.lo_b;
-                        const double vc = (pick3(sg.ci1, cx) - sg.start) * idx + 1.0 - w.lo_c;
-                        const double B1 = pick3(sg.tci1, bx) / t1a, C1 = pick3(sg.tci1, cx) / t1a;
+                        const double va = (pick3(sg.ci1, ax) - sg.start) * inv_dx + 1.0 - w.lo_a;
+                        const double vb = (pick3(sg.ci1, bx) - sg.start) * inv_dx + 1.0 - w.lo_b;
+                        const double vc = (pick3(sg.ci1, cx) - sg.start) * inv_dx + 1.0 - w.lo_c;
+                        const double it1a = 1.0 / t1a;
+                        const double B1 = pick3(sg.tci1, bx) * it1a, C1 = pick3(sg.tci1, cx) * it1a;
                         w.B1 = (float)B1;
                         w.C1 = (float)C1;
                         w.P1 = (float)((va + B1 * vb) + C1 * vc);
