@@ -190,17 +190,20 @@ __device__ void hit_math_f64(const Geo &g, const SubSeg &sg, int32_t ip, int32_t
 // Constant core radius and no twist (sigma_ratio = 1, eta = 0; every benchmark workload): the
 // reference's contribution reduces exactly to tp*Gamma*T (t5..t8 and the straight-branch extras
 // vanish). The host selects this path only when every instance has sigma_ratio = 1 and eta = 0.
-// The spline point, its tangent and the offset xc are the reference's FP64 values
-// (spline wrapped to [-pi, pi), then cminus's single +-2pi correction, which is not a full
-// minimum image: cells of an extended box more than 3pi from the wrapped spline point keep a
-// far offset and receive the negative cut-off term, as in the reference). The Gaussian core runs
-// in FP32 with the hardware reciprocal and reciprocal square root. Otherwise the curvature terms carry 1/(1 - kappa rho cos), which amplifies rounding
-// near its zero, so the whole reference formula runs in FP64 (hit_math_f64).
+// The spline point, its tangent and the offset xc are FP64 values: the spline parameter is the
+// reference's zeta rewritten as an affine function of t0 (equal up to rounding), the spline
+// wrapped to [-pi, pi), then cminus's single +-2pi correction, which is not a full minimum
+// image: cells of an extended box more than 3pi from the wrapped spline point keep a far offset
+// and receive the negative cut-off term, as in the reference. The Gaussian core runs in FP32 with
+// the hardware reciprocal and reciprocal square root. Otherwise the curvature terms carry
+// 1/(1 - kappa rho cos), which amplifies rounding near its zero, so the whole reference formula
+// runs in FP64 (hit_math_f64).
+
 // Per-hit values that live in global memory (the section's spline and the tube's constants):
 // loaded before the candidate test so that their latency overlaps it.
 struct HitConst {
     double coe[18];
-    double LD, czl, smin;
+    double smin;
     float Gamma;
 };
 
@@ -208,22 +211,17 @@ __device__ __forceinline__ void hit_const_load(const Geo &g, const SubSeg &sg, H
     const double *coe = g.coe + 18 * (int64_t)sg.sec;
 #pragma unroll
     for (int k = 0; k < 18; k++) hc.coe[k] = coe[k];
-    hc.LD = g.Len_Disc[sg.tube];
-    hc.czl = g.czl[sg.sec];
     hc.smin = g.sigma[sg.tube];
     hc.Gamma = (float)g.gamma[sg.tube];
 }
 
+// sg.s0, sg.s1 hold zA, zB (SegRec).
 __device__ void hit_math_f32(const HitConst &hc, const SubSeg &sg, int32_t ip, int32_t jp, int32_t kp, double *v) {
-    const double LD = hc.LD;
     const double *coe = hc.coe;
     double X[3] = {mesh(ip, sg), mesh(jp, sg), mesh(kp, sg)};
     double a[3] = {X[0] - sg.ci0[0], X[1] - sg.ci0[1], X[2] - sg.ci0[2]};
     double t0 = (a[0] * sg.tci0[0] + a[1] * sg.tci0[1]) + a[2] * sg.tci0[2];
-    double t1 = ((X[0] - sg.ci1[0]) * sg.tci0[0] + (X[1] - sg.ci1[1]) * sg.tci0[1]) +
-                (X[2] - sg.ci1[2]) * sg.tci0[2];
-    double zeta = (sg.s1 * t0 - sg.s0 * t1) / sg.ndci0;
-    double z = (zeta - hc.czl) * LD;
+    double z = sg.s0 + sg.s1 * t0;
     double cz3[3], dc[3], xcd[3];
     spline_pos(z, coe, cz3);
     spline_d1(z, coe, dc);
@@ -298,6 +296,8 @@ constexpr int kCells = kBrickX * kBrickY * kBrickZ;
 
 // The sub-segment of the lane's pair, as the candidate test and the hit evaluation need it. It
 // stays in the owning lane's registers; other lanes read it with cross-lane shuffles.
+// FP32 mode keeps, in place of s0 and s1, the constants of z(t0) = zA + zB t0, the section's
+// spline parameter as a function of t0 = (X - ci0) . tci0 (ndci0 is then unused).
 struct SegRec {
     double ci0[3], ci1[3], tci0[3], tci1[3];
     double s0, s1, ndci0, R;
@@ -414,6 +414,7 @@ __device__ __forceinline__ void wave_sync_lds() {
 }
 
 // The sub-segment of lane `own` (every lane of the wave must call this).
+template <bool FAST>
 __device__ __forceinline__ void fetch_owner(const SegRec &me, int own, const Run &ri, SubSeg &sg) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -424,7 +425,7 @@ __device__ __forceinline__ void fetch_owner(const SegRec &me, int own, const Run
     }
     sg.s0 = __shfl(me.s0, own, 64);
     sg.s1 = __shfl(me.s1, own, 64);
-    sg.ndci0 = __shfl(me.ndci0, own, 64);
+    if (!FAST) sg.ndci0 = __shfl(me.ndci0, own, 64);
     sg.R = __shfl(me.R, own, 64);
     sg.sec = __shfl(me.sec, own, 64);
     sg.tube = __shfl(me.tube, own, 64);
@@ -461,7 +462,7 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
     wave_sync_lds();
     W.candn -= cnt;
     SubSeg sg;
-    fetch_owner(me, lane < cnt ? h.owner : lane, *W.ri, sg);
+    fetch_owner<FAST>(me, lane < cnt ? h.owner : lane, *W.ri, sg);
     bool hit = false, ovf = false;
     if (lane < cnt) {
         HitConst hc;
@@ -627,6 +628,15 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                     me.s0 = sg.s0;
                     me.s1 = sg.s1;
                     me.ndci0 = sg.ndci0;
+                    if (FAST) {
+                        // zeta = (s1 t0 - s0 t1) / ndci0 with t1 = t0 - L, L = (ci1 - ci0) . tci0, so
+                        // z = (zeta - czl) LD = zA + zB t0
+                        const double L = ((sg.ci1[0] - sg.ci0[0]) * sg.tci0[0] + (sg.ci1[1] - sg.ci0[1]) * sg.tci0[1]) +
+                                         (sg.ci1[2] - sg.ci0[2]) * sg.tci0[2];
+                        const double LD = g.Len_Disc[sg.tube], ind = 1.0 / sg.ndci0;
+                        me.s0 = (sg.s0 * L * ind - g.czl[sg.sec]) * LD;
+                        me.s1 = (sg.s1 - sg.s0) * LD * ind;
+                    }
                     me.R = sg.R;
                     me.sec = sg.sec;
                     me.tube = sg.tube;
