@@ -449,7 +449,7 @@ struct Wave {
     int ox, oy, oz;
     int lane;
     int segn, candn;
-    unsigned long long ncols, ncand, nhit, novf;   // wave-uniform counts
+    uint32_t ncols, ncand, nhit, novf;   // wave-uniform counts (scalar registers; below 2^32 per wave)
 };
 
 // Tests candidates [candn - cnt, candn) with the reference predicate and adds the contribution of
@@ -460,7 +460,7 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
     const int lane = W.lane;
     const HitQ h = W.candq[W.candn - cnt + (lane < cnt ? lane : 0)];
     wave_sync_lds();
-    W.candn -= cnt;
+    W.candn = __builtin_amdgcn_readfirstlane(W.candn - cnt);
     SubSeg sg;
     fetch_owner<FAST>(me, lane < cnt ? h.owner : lane, *W.ri, sg);
     bool hit = false, ovf = false;
@@ -489,8 +489,8 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
             }
         }
     }
-    W.nhit += (unsigned long long)__popcll(__ballot(hit));
-    W.novf += (unsigned long long)__popcll(__ballot(ovf));
+    W.nhit = __builtin_amdgcn_readfirstlane(W.nhit + (uint32_t)__popcll(__ballot(hit)));
+    W.novf = __builtin_amdgcn_readfirstlane(W.novf + (uint32_t)__popcll(__ballot(ovf)));
 }
 
 // Walks the columns of row segments [segn - cnt, segn), one segment per lane and one column of
@@ -510,10 +510,10 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
     }
     const int ax = lane < cnt ? (int)w.ax : 0;
     const int bx = ax == 2 ? 0 : ax + 1;
-    W.segn -= cnt;
+    W.segn = __builtin_amdgcn_readfirstlane(W.segn - cnt);
     unsigned long long mc;
     for (int32_t col = 0; (mc = __ballot(col < len)) != 0ull; col++, ib++) {
-        W.ncols += (unsigned long long)__popcll(mc);
+        W.ncols = __builtin_amdgcn_readfirstlane(W.ncols + (uint32_t)__popcll(mc));
         int32_t nc = 0, ia0 = 0;
         if (col < len) nc = column_range(w, ib, ic, &ia0);
         for (int u = 0; __ballot(u < nc); u++) {
@@ -528,8 +528,8 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
                 q.owner = (int16_t)own;
                 W.candq[W.candn + __popcll(m & lanemask_lt())] = q;
             }
-            W.candn += __popcll(m);
-            W.ncand += (unsigned long long)__popcll(m);
+            W.candn = __builtin_amdgcn_readfirstlane(W.candn + __popcll(m));
+            W.ncand = __builtin_amdgcn_readfirstlane(W.ncand + (uint32_t)__popcll(m));
             wave_sync_lds();
             if (W.candn >= 64) stage_candidates<F64, FAST>(W, me, 64);
         }
@@ -722,7 +722,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                 const unsigned long long m = __ballot(sgok);
                 if (sgok) W.segs[W.segn + __popcll(m & lanemask_lt())] =
                     RowSeg{(int16_t)ic, (int16_t)bl, (uint8_t)(bh - bl + 1), (uint8_t)lane, 0};
-                W.segn += __popcll(m);
+                W.segn = __builtin_amdgcn_readfirstlane(W.segn + __popcll(m));
                 wave_sync_lds();
                 if (W.segn >= 64) stage_columns<F64, FAST>(W, me, 64);
             }
@@ -755,10 +755,10 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
     }
     // counters (wave-uniform): one atomic per wave
     if (lane == 0) {
-        atomicAdd(A.counters + 0, W.ncols);
-        atomicAdd(A.counters + 1, W.ncand);
-        atomicAdd(A.counters + 2, W.nhit);
-        if (W.novf) atomicAdd(A.counters + 3, W.novf);
+        atomicAdd(A.counters + 0, (unsigned long long)W.ncols);
+        atomicAdd(A.counters + 1, (unsigned long long)W.ncand);
+        atomicAdd(A.counters + 2, (unsigned long long)W.nhit);
+        if (W.novf) atomicAdd(A.counters + 3, (unsigned long long)W.novf);
     }
 }
 
