@@ -80,13 +80,20 @@ __device__ __forceinline__ AxisImages axis_images(int32_t lo, int32_t hi, int32_
     return r;
 }
 
-// Pass 0 (FILL = false): counts pairs per brick and flags boxes outside [1-N, 2N].
+// A sub-segment with more pairs than this (a box of many bricks: the widest cores at high N) is
+// expanded by a whole wave in k_pairs_big instead of one lane, so that no lane runs thousands of
+// steps while the rest of its wave waits.
+constexpr int32_t kBigPairs = 256;
+
+// Pass 0 (FILL = false): counts pairs per brick, flags boxes outside [1-N, 2N] and lists the
+// sub-segments with more than kBigPairs pairs in big[0 .. *nbig).
 // Pass 1 (FILL = true): writes every pair (sub-point index, image code kx + 3ky + 9kz) into its
 // brick's list.
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restrict__ cnt,
                                                const int64_t *__restrict__ off, uint32_t *__restrict__ pairs,
-                                               uint8_t *__restrict__ pair_img, int32_t *__restrict__ err) {
+                                               uint8_t *__restrict__ pair_img, int32_t *__restrict__ err,
+                                               uint32_t *__restrict__ big, uint32_t *__restrict__ nbig) {
     int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     SubSeg sg;
     bool ok = q < g.nsub_all && subseg_load(g, r0, q, sg);
@@ -110,6 +117,10 @@ __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restri
             nx = ab[0].count();
             ny = ab[1].count();
             n = nx * ny * ab[2].count();
+            if (n > kBigPairs) {
+                if (!FILL) big[atomicAdd(nbig, 1u)] = (uint32_t)q;
+                n = 0;
+            }
         }
     }
     // wave-uniform loop over the lanes' brick lists
@@ -135,6 +146,40 @@ __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restri
         if (FILL && act) {
             pairs[off[key] + slot] = (uint32_t)q;
             pair_img[off[key] + slot] = (uint8_t)code;
+        }
+    }
+}
+
+// The sub-segments listed by k_pairs, one wave each (grid-stride over the list): lane t takes
+// pairs t, t + 64, ... of the box; each increments its brick's counter (pairs of one box are
+// distinct bricks or images) and, in the fill pass, writes its pair.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_pairs_big(Geo g, Run r0, uint32_t *__restrict__ cnt,
+                                                   const int64_t *__restrict__ off, uint32_t *__restrict__ pairs,
+                                                   uint8_t *__restrict__ pair_img, const uint32_t *__restrict__ big,
+                                                   const uint32_t *__restrict__ nbig) {
+    const uint32_t nb = *nbig;
+    const int lane = threadIdx.x & 63;
+    for (uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < nb; w += gridDim.x * (blockDim.x >> 6)) {
+        const int64_t q = big[w];
+        SubSeg sg;
+        if (!subseg_load(g, r0, q, sg)) continue;   // listed sub-segments always load
+        const Run &r = g.runs[g.tube_inst[sg.tube]];
+        AxisImages ab[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) ab[c] = axis_images(sg.lo[c], sg.hi[c], sg.N, brick_dim(c));
+        const int32_t nx = ab[0].count(), ny = ab[1].count(), n = nx * ny * ab[2].count();
+        for (int32_t t = lane; t < n; t += 64) {
+            int32_t bxi, byi, bzi, kx, ky, kz;
+            ab[0].at(t % nx, bxi, kx);
+            ab[1].at((t / nx) % ny, byi, ky);
+            ab[2].at(t / (nx * ny), bzi, kz);
+            const int64_t key = r.brick0 + ((int64_t)bzi * r.nb[1] + byi) * r.nb[0] + bxi;
+            const uint32_t slot = atomicAdd(cnt + key, 1u);
+            if (FILL) {
+                pairs[off[key] + slot] = (uint32_t)q;
+                pair_img[off[key] + slot] = (uint8_t)(kx + 3 * ky + 9 * kz);
+            }
         }
     }
 }
@@ -172,18 +217,26 @@ hipError_t launch_bins(wt_ctx *c) {
                 for (int x = 0; x < r.nb[0]; x++)
                     tab[(size_t)(r.brick0 + ((int64_t)z * r.nb[1] + y) * r.nb[0] + x)] = make_int4(i, x, y, z);
     }
-    uint32_t *cnt = nullptr;
+    uint32_t *cnt = nullptr, *big = nullptr, *nbig = nullptr;
     int32_t *err = nullptr;
     WT_HIP(balloc(c, &b.brick_tab, nbricks));
     WT_HIP(balloc(c, &b.brick_off, nbricks + 1));
     WT_HIP(balloc(c, &cnt, nbricks));
     WT_HIP(balloc(c, &err, 1));
+    WT_HIP(balloc(c, &big, g.nsub_all));
+    WT_HIP(balloc(c, &nbig, 1));
+    WT_HIP(hipMemsetAsync(nbig, 0, sizeof(uint32_t), st));
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    const unsigned big_grid = (unsigned)(4 * cus);
     WT_HIP(hipMemcpyAsync(b.brick_tab, tab.data(), sizeof(int4) * nbricks, hipMemcpyHostToDevice, st));
     WT_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * nbricks, st));
     WT_HIP(hipMemsetAsync(err, 0, sizeof(int32_t), st));
     kt_begin(c, kSlotBinCount);
     hipLaunchKernelGGL(k_pairs<false>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, cnt,
-                       (const int64_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr, err);
+                       (const int64_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr, err, big, nbig);
+    hipLaunchKernelGGL(k_pairs_big<false>, dim3(big_grid), dim3(256), 0, st, g, c->run, cnt, (const int64_t *)nullptr,
+                       (uint32_t *)nullptr, (uint8_t *)nullptr, (const uint32_t *)big, (const uint32_t *)nbig);
     kt_end(c, kSlotBinCount);
     WT_HIP(hipGetLastError());
     void *tmp = nullptr;
@@ -211,7 +264,9 @@ hipError_t launch_bins(wt_ctx *c) {
     WT_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * nbricks, st));
     kt_begin(c, kSlotBinFill);
     hipLaunchKernelGGL(k_pairs<true>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, cnt,
-                       (const int64_t *)b.brick_off, b.pairs, b.pair_img, err);
+                       (const int64_t *)b.brick_off, b.pairs, b.pair_img, err, big, nbig);
+    hipLaunchKernelGGL(k_pairs_big<true>, dim3(big_grid), dim3(256), 0, st, g, c->run, cnt,
+                       (const int64_t *)b.brick_off, b.pairs, b.pair_img, (const uint32_t *)big, (const uint32_t *)nbig);
     kt_end(c, kSlotBinFill);
     return hipGetLastError();
 }

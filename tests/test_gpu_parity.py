@@ -107,6 +107,28 @@ def test_deposit(oracle, name, precision):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
+def test_deposit_wide_cores(oracle, precision):
+    """Boxes of 7 bricks per axis (more than 256 (brick, image) pairs per sub-segment) are binned by
+    a wave per sub-segment; the depositing pairs and the field must still be the reference's."""
+    p, t = fbb.config_small(N=128, seed=17, ntubes=(1, 1), npts=(24, 20), radii=(0.9, 0.5), ntf_pre=100)
+    t.sigma = np.array([0.8, 0.3])
+    geo = oracle.Geometry(oracle_params(oracle, p, t), t.npoint, t.cx, t.cy, t.cz)
+    om_ref, hits = geo.deposit(t.sigma, t.gamma, t.n_T)
+    g = gpu_ctx(p, t, precision)
+    g.build_segments()
+    g.bin()
+    box, R = g.bins()
+    valid = R >= 0.0
+    widest = int((box[valid, 3:] - box[valid, :3]).max())
+    assert widest > 6 * 16, widest
+    d = g.deposit()
+    assert d["hits"] == hits
+    assert d["pairs"] > 256 * 10
+    err = rel_l2(g.get_fields("omega"), om_ref)
+    assert err <= (1e-12 if precision == 64 else 1e-5), err
+
+
+@pytest.mark.parametrize("precision", [64, 32])
 def test_biot_savart_from_oracle_vorticity(oracle, precision):
     p, t = _case("small32")
     geo = oracle.Geometry(oracle_params(oracle, p, t), t.npoint, t.cx, t.cy, t.cz)
