@@ -214,9 +214,12 @@ __global__ void k_theta(Geo g, Run r, int32_t *__restrict__ err) {
 // need not be reproduced bit for bit: the reference's sum of ntf_pre+1 samples of
 // |c' x c''| / |c'|^2 is re-evaluated in FP32 at the same sample points, from polynomial
 // coefficients in t = z/zM on [0, 1] (c' x c'' has degree 7, |c'|^2 degree 8; coefficients
-// formed in FP64 and rounded once), with a running bound on the FP32 error of every sample
-// (Horner with FMA: |err| <= gamma * sum (1+n)|a_n| for t in [0, 1], plus the rounding of the
-// coefficients, of t and of the sqrt/reciprocal). A section is decided here when the interval
+// formed in FP64). The samples are taken in symmetric pairs t = 1/2 +- s: each polynomial is
+// shifted to p(1/2 + s) = E(s^2) + s O(s^2) in FP64 (Taylor shift) and its even and odd parts
+// rounded once to FP32, so one evaluation of E and O in s^2 gives both samples. The error of every
+// sample is bounded by gamma * sum (1+k)|d_k| 2^-k over the shifted coefficients d_k (|s| <= 1/2;
+// this is never more than sum (1+n)|a_n| over the unshifted ones), plus the rounding of the
+// coefficients, of s and of the sqrt/reciprocal. A section is decided here when the interval
 // [x - dx, x + dx] around x = ntf_times*theta/pi2 contains no integer; every other section is
 // listed and re-evaluated with the reference's exact FP64 sum (k_theta_listed), so ntf and dtf
 // always equal the reference's.
@@ -253,12 +256,17 @@ __global__ __launch_bounds__(128) void k_theta_fast(Geo g, Run, int32_t *__restr
             zp *= zM;
         }
     }
-    float wf[3][8], gf[9];
+    // Taylor shift by 1/2 in FP64, then even/odd parts rounded to FP32
+    auto shift_half = [](double *a, int n) {
+        for (int i = 0; i < n; i++)
+            for (int k = n - 1; k >= i; k--) a[k] += 0.5 * a[k + 1];
+    };
+    float wE[3][4], wO[3][4], gE[5], gO[4];
     double Wb2 = 0.0, Gb = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-        double wb = 0.0;
+        double wd[8];
 #pragma unroll
         for (int n = 0; n < 8; n++) {
             double v = 0.0;
@@ -267,31 +275,46 @@ __global__ __launch_bounds__(128) void k_theta_fast(Geo g, Run, int32_t *__restr
                 int mm = n - m;
                 if (mm >= 0 && mm < 4) v += b[c1][m] * e[c2][mm] - b[c2][m] * e[c1][mm];
             }
-            wf[c][n] = (float)v;
-            wb += (double)(1 + n) * fabs((double)wf[c][n]);
+            wd[n] = v;
+        }
+        shift_half(wd, 7);
+        double wb = 0.0, sc2 = 1.0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float f = (float)wd[k];
+            if (k & 1) wO[c][k >> 1] = f; else wE[c][k >> 1] = f;
+            wb += (double)(1 + k) * fabs((double)f) * sc2;
+            sc2 *= 0.5;
         }
         Wb2 += wb * wb;
     }
+    {
+        double gd[9];
 #pragma unroll
-    for (int n = 0; n < 9; n++) {
-        double v = 0.0;
+        for (int n = 0; n < 9; n++) {
+            double v = 0.0;
 #pragma unroll
-        for (int c = 0; c < 3; c++)
+            for (int c = 0; c < 3; c++)
 #pragma unroll
-            for (int m = 0; m < 5; m++) {
-                int mm = n - m;
-                if (mm >= 0 && mm < 5) v += b[c][m] * b[c][mm];
-            }
-        gf[n] = (float)v;
-        Gb += (double)(1 + n) * fabs((double)gf[n]);
+                for (int m = 0; m < 5; m++) {
+                    int mm = n - m;
+                    if (mm >= 0 && mm < 5) v += b[c][m] * b[c][mm];
+                }
+            gd[n] = v;
+        }
+        shift_half(gd, 8);
+        double sc2 = 1.0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const float f = (float)gd[k];
+            if (k & 1) gO[k >> 1] = f; else gE[k >> 1] = f;
+            Gb += (double)(1 + k) * fabs((double)f) * sc2;
+            sc2 *= 0.5;
+        }
     }
-    const float invM = 1.0f / (float)M;
     double acc = 0.0;           // sum of f
     float sg = 0.0f, sfg = 0.0f, gmin = 3.0e38f;
-    for (int j = 0; j <= M; j++) {
-        float t = (float)j * invM;
-        float wx = horner8(wf[0], 7, t), wy = horner8(wf[1], 7, t), wz = horner8(wf[2], 7, t);
-        float gg = horner8(gf, 8, t);
+    auto sample = [&](float wx, float wy, float wz, float gg) {
         float n2 = __builtin_fmaf(wx, wx, __builtin_fmaf(wy, wy, wz * wz));
         float rg = __builtin_amdgcn_rcpf(gg);
         float f = __builtin_amdgcn_sqrtf(n2) * rg;
@@ -299,6 +322,24 @@ __global__ __launch_bounds__(128) void k_theta_fast(Geo g, Run, int32_t *__restr
         sg += rg;
         sfg = __builtin_fmaf(f, rg, sfg);
         gmin = fminf(gmin, gg);
+    };
+    // samples j = 0..M at t = j/M: pairs t = 1/2 +- s with s = (2i + o)/(2M), plus t = 1/2 for even M
+    const bool odd = (M & 1) != 0;
+    const int npair = odd ? (M + 1) / 2 : M / 2, o = odd ? 1 : 2;
+    const float h = 1.0f / (float)(2 * M);
+    if (!odd) sample(wE[0][0], wE[1][0], wE[2][0], gE[0]);
+    for (int i = 0; i < npair; i++) {
+        const float sv = (float)(2 * i + o) * h, v = sv * sv;
+        float wp[3], wm[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float E = horner8(wE[c], 3, v), Os = sv * horner8(wO[c], 3, v);
+            wp[c] = E + Os;
+            wm[c] = E - Os;
+        }
+        const float gEv = horner8(gE, 4, v), gOs = sv * horner8(gO, 3, v);
+        sample(wp[0], wp[1], wp[2], gEv + gOs);
+        sample(wm[0], wm[1], wm[2], gEv - gOs);
     }
     // error bound of the FP32 sum against the reference's FP64 sum (both times dz)
     const double u = 5.9604644775390625e-08;             // 2^-24
