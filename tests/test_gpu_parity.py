@@ -68,6 +68,21 @@ def test_segments_bitexact(oracle, name):
     assert st["dtf_min"] == geo.dtf.min()
 
 
+@pytest.mark.parametrize("ntf_pre", [1, 2, 7, 101, 1000])
+def test_turning_angle_sample_counts(oracle, ntf_pre):
+    """The FP32 turning angle takes the ntf_pre + 1 samples in symmetric pairs (odd and even
+    ntf_pre pair them differently); ntf and dtf must stay the reference's, and most sections must
+    be decided without the exact re-evaluation."""
+    p, t = fbb.config_small(N=32, seed=23, ntubes=(2, 3), npts=(80, 60), radii=(0.5, 0.3), ntf_pre=ntf_pre)
+    geo = oracle.Geometry(oracle_params(oracle, p, t), t.npoint, t.cx, t.cy, t.cz)
+    g = gpu_ctx(p, t, 64)
+    st = g.build_segments()
+    ntf, dtf = g.sections()
+    np.testing.assert_array_equal(ntf, geo.ntf)
+    np.testing.assert_array_equal(dtf.view(np.int64), geo.dtf.view(np.int64))
+    assert st["theta_exact_sections"] < 0.2 * st["nsec_all"], st
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_bins_bitexact(oracle, name):
     p, t = _case(name)
