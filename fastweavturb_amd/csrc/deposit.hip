@@ -191,7 +191,8 @@ __device__ void hit_math_f64(const Geo &g, const SubSeg &sg, int32_t ip, int32_t
 // reference's contribution reduces exactly to tp*Gamma*T (t5..t8 and the straight-branch extras
 // vanish). The host selects this path only when every instance has sigma_ratio = 1 and eta = 0.
 // The spline point, its tangent and the offset xc are FP64 values: the spline parameter is the
-// reference's zeta rewritten as an affine function of t0 (equal up to rounding), the spline
+// reference's zeta rewritten as an affine function of t0 (equal up to rounding), point and
+// tangent evaluated together by Horner's scheme with FMA, the spline
 // wrapped to [-pi, pi), then cminus's single +-2pi correction, which is not a full minimum
 // image: cells of an extended box more than 3pi from the wrapped spline point keep a far offset
 // and receive the negative cut-off term, as in the reference. The Gaussian core runs in FP32 with
@@ -222,9 +223,25 @@ __device__ void hit_math_f32(const HitConst &hc, const SubSeg &sg, int32_t ip, i
     double a[3] = {X[0] - sg.ci0[0], X[1] - sg.ci0[1], X[2] - sg.ci0[2]};
     double t0 = (a[0] * sg.tci0[0] + a[1] * sg.tci0[1]) + a[2] * sg.tci0[2];
     double z = sg.s0 + sg.s1 * t0;
+    // spline point and tangent together (Horner with derivative, FP64 FMA), point wrapped to
+    // [-pi, pi) as in coo_curve_spline_5th
     double cz3[3], dc[3], xcd[3];
-    spline_pos(z, coe, cz3);
-    spline_d1(z, coe, dc);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double *a = coe + 6 * c;
+        double pv = a[5], dv = 0.0;
+#pragma unroll
+        for (int k = 4; k >= 0; k--) {
+            dv = __builtin_fma(dv, z, pv);
+            pv = __builtin_fma(pv, z, a[k]);
+        }
+        cz3[c] = pv;
+        dc[c] = dv;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) if (cz3[c] >= kPi) cz3[c] = cz3[c] - kPi2;
+#pragma unroll
+    for (int c = 0; c < 3; c++) if (cz3[c] < -kPi) cz3[c] = cz3[c] + kPi2;
     cminus(X, cz3, xcd);
     float xc[3], dcf[3];
 #pragma unroll
