@@ -46,7 +46,7 @@ def make_workload(name: str):
 
 def algorithmic_cost(slot: str, N: int, seg: dict, dep: dict, p):
     """(amount, unit, bound, peak) of one launch of a kernel: bytes for memory-bound kernels,
-    FLOPs for arithmetic-bound ones (per-unit figures documented in DESIGN.md)."""
+    FLOPs for arithmetic-bound ones: per-unit costs times the units one launch processes."""
     H = N // 2 + 1
     half = 3 * N * N * H * 8          # three FP32 half spectra, bytes
     real = 3 * N ** 3 * 4             # three FP32 fields, bytes
@@ -354,8 +354,8 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU decomposition is described in DESIGN.md and not built.", file=sys.stderr)
+        print(f"bench.py: the woven-field path runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the z-slab decomposition over several GPUs is not built.", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
