@@ -86,14 +86,16 @@ __device__ __forceinline__ AxisImages axis_images(int32_t lo, int32_t hi, int32_
 constexpr int32_t kBigPairs = 256;
 
 // Pass 0 (FILL = false): counts pairs per brick, flags boxes outside [1-N, 2N] and lists the
-// sub-segments with more than kBigPairs pairs in big[0 .. *nbig).
+// sub-segments with more than kBigPairs pairs in big[0 .. *nbig) (entries beyond big_cap are not
+// stored: the host then grows the list and repeats the pass).
 // Pass 1 (FILL = true): writes every pair (sub-point index, image code kx + 3ky + 9kz) into its
 // brick's list.
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restrict__ cnt,
                                                const int64_t *__restrict__ off, uint32_t *__restrict__ pairs,
                                                uint8_t *__restrict__ pair_img, int32_t *__restrict__ err,
-                                               uint32_t *__restrict__ big, uint32_t *__restrict__ nbig) {
+                                               uint32_t *__restrict__ big, uint32_t *__restrict__ nbig,
+                                               uint32_t big_cap) {
     int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     SubSeg sg;
     bool ok = q < g.nsub_all && subseg_load(g, r0, q, sg);
@@ -118,7 +120,10 @@ __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restri
             ny = ab[1].count();
             n = nx * ny * ab[2].count();
             if (n > kBigPairs) {
-                if (!FILL) big[atomicAdd(nbig, 1u)] = (uint32_t)q;
+                if (!FILL) {
+                    const uint32_t k = atomicAdd(nbig, 1u);
+                    if (k < big_cap) big[k] = (uint32_t)q;
+                }
                 n = 0;
             }
         }
@@ -157,8 +162,8 @@ template <bool FILL>
 __global__ __launch_bounds__(256) void k_pairs_big(Geo g, Run r0, uint32_t *__restrict__ cnt,
                                                    const int64_t *__restrict__ off, uint32_t *__restrict__ pairs,
                                                    uint8_t *__restrict__ pair_img, const uint32_t *__restrict__ big,
-                                                   const uint32_t *__restrict__ nbig) {
-    const uint32_t nb = *nbig;
+                                                   const uint32_t *__restrict__ nbig, uint32_t big_cap) {
+    const uint32_t nb = *nbig < big_cap ? *nbig : big_cap;
     const int lane = threadIdx.x & 63;
     for (uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < nb; w += gridDim.x * (blockDim.x >> 6)) {
         const int64_t q = big[w];
@@ -223,32 +228,45 @@ hipError_t launch_bins(wt_ctx *c) {
     WT_HIP(balloc(c, &b.brick_off, nbricks + 1));
     WT_HIP(balloc(c, &cnt, nbricks));
     WT_HIP(balloc(c, &err, 1));
-    WT_HIP(balloc(c, &big, g.nsub_all));
+    // list of wide boxes: sized for one sub-segment in 64 (at least 1024), grown if the count pass
+    // finds more
+    uint32_t big_cap = (uint32_t)std::min<int64_t>(std::max<int64_t>(g.nsub_all / 64, 1024), g.nsub_all + 1);
+    WT_HIP(balloc(c, &big, big_cap));
     WT_HIP(balloc(c, &nbig, 1));
-    WT_HIP(hipMemsetAsync(nbig, 0, sizeof(uint32_t), st));
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
     const unsigned big_grid = (unsigned)(4 * cus);
     WT_HIP(hipMemcpyAsync(b.brick_tab, tab.data(), sizeof(int4) * nbricks, hipMemcpyHostToDevice, st));
-    WT_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * nbricks, st));
-    WT_HIP(hipMemsetAsync(err, 0, sizeof(int32_t), st));
-    kt_begin(c, kSlotBinCount);
-    hipLaunchKernelGGL(k_pairs<false>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, cnt,
-                       (const int64_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr, err, big, nbig);
-    hipLaunchKernelGGL(k_pairs_big<false>, dim3(big_grid), dim3(256), 0, st, g, c->run, cnt, (const int64_t *)nullptr,
-                       (uint32_t *)nullptr, (uint8_t *)nullptr, (const uint32_t *)big, (const uint32_t *)nbig);
-    kt_end(c, kSlotBinCount);
-    WT_HIP(hipGetLastError());
-    void *tmp = nullptr;
-    size_t tmp_bytes = 0;
-    WT_HIP(exclusive_scan_i32_to_i64((const int32_t *)cnt, 0, b.brick_off, nbricks, st, &tmp, &tmp_bytes));
     int64_t npairs = 0;
-    int32_t herr = 0;
-    WT_HIP(hipMemcpyAsync(&npairs, b.brick_off + nbricks, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    WT_HIP(hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    WT_HIP(hipStreamSynchronize(st));
-    if (tmp) (void)hipFree(tmp);
-    if (herr) return hipErrorInvalidValue;
+    for (int attempt = 0;; attempt++) {
+        WT_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * nbricks, st));
+        WT_HIP(hipMemsetAsync(err, 0, sizeof(int32_t), st));
+        WT_HIP(hipMemsetAsync(nbig, 0, sizeof(uint32_t), st));
+        kt_begin(c, kSlotBinCount);
+        hipLaunchKernelGGL(k_pairs<false>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, cnt,
+                           (const int64_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr, err, big, nbig, big_cap);
+        hipLaunchKernelGGL(k_pairs_big<false>, dim3(big_grid), dim3(256), 0, st, g, c->run, cnt,
+                           (const int64_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr, (const uint32_t *)big,
+                           (const uint32_t *)nbig, big_cap);
+        kt_end(c, kSlotBinCount);
+        WT_HIP(hipGetLastError());
+        void *tmp = nullptr;
+        size_t tmp_bytes = 0;
+        WT_HIP(exclusive_scan_i32_to_i64((const int32_t *)cnt, 0, b.brick_off, nbricks, st, &tmp, &tmp_bytes));
+        int32_t herr = 0;
+        uint32_t hbig = 0;
+        WT_HIP(hipMemcpyAsync(&npairs, b.brick_off + nbricks, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        WT_HIP(hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        WT_HIP(hipMemcpyAsync(&hbig, nbig, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        WT_HIP(hipStreamSynchronize(st));
+        if (tmp) (void)hipFree(tmp);
+        if (herr) return hipErrorInvalidValue;
+        if (hbig <= big_cap) break;
+        if (attempt > 0) return hipErrorUnknown;   // the list was sized from the first count
+        // too many wide boxes for the list: grow it and count again
+        big_cap = hbig;
+        WT_HIP(balloc(c, &big, big_cap));
+    }
     b.npairs = npairs;
     // the pair list is kept between calls and grown when needed
     if ((size_t)npairs > c->pairs_cap) {
@@ -264,9 +282,10 @@ hipError_t launch_bins(wt_ctx *c) {
     WT_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * nbricks, st));
     kt_begin(c, kSlotBinFill);
     hipLaunchKernelGGL(k_pairs<true>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, cnt,
-                       (const int64_t *)b.brick_off, b.pairs, b.pair_img, err, big, nbig);
+                       (const int64_t *)b.brick_off, b.pairs, b.pair_img, err, big, nbig, big_cap);
     hipLaunchKernelGGL(k_pairs_big<true>, dim3(big_grid), dim3(256), 0, st, g, c->run, cnt,
-                       (const int64_t *)b.brick_off, b.pairs, b.pair_img, (const uint32_t *)big, (const uint32_t *)nbig);
+                       (const int64_t *)b.brick_off, b.pairs, b.pair_img, (const uint32_t *)big, (const uint32_t *)nbig,
+                       big_cap);
     kt_end(c, kSlotBinFill);
     return hipGetLastError();
 }

@@ -143,6 +143,43 @@ def test_deposit_wide_cores(oracle, precision):
     assert err <= (1e-12 if precision == 64 else 1e-5), err
 
 
+def _expected_pairs(lohi, N, bd=16):
+    """(brick, image) pairs of each box, counted on the host the way the binning defines them."""
+    total = 0
+    for lo, hi in zip(lohi[:, :3], lohi[:, 3:]):
+        n = 1
+        for d in range(3):
+            cnt = 0
+            for k in (-1, 0, 1):
+                a, b = max(lo[d], 1 + k * N), min(hi[d], N + k * N)
+                if a <= b:
+                    cnt += (b - k * N - 1) // bd - (a - k * N - 1) // bd + 1
+            n *= cnt
+        total += n
+    return total
+
+
+def test_wide_box_list_growth():
+    """More wide boxes (over 256 pairs each) than the list's first size: the list grows and the pair
+    lists stay complete (pair count equal to a host recount of the boxes; FP32 and FP64 agree)."""
+    p, t = fbb.config_small(N=128, seed=29, ntubes=(1,), npts=(400,), radii=(0.9,), ntf_pre=100)
+    t.sigma = np.array([0.8])
+    out = {}
+    for precision in (64, 32):
+        g = gpu_ctx(p, t, precision)
+        g.build_segments()
+        g.bin()
+        lohi, R = g.bins()
+        valid = R >= 0.0
+        d = g.deposit()
+        assert d["pairs"] == _expected_pairs(lohi[valid], 128)
+        out[precision] = (d["hits"], g.get_fields("omega"))
+        g.close()
+    assert int(valid.sum()) > 1024
+    assert out[64][0] == out[32][0]
+    assert rel_l2(out[32][1], out[64][1]) < 1e-5
+
+
 @pytest.mark.parametrize("precision", [64, 32])
 def test_biot_savart_from_oracle_vorticity(oracle, precision):
     p, t = _case("small32")
