@@ -331,7 +331,7 @@ struct SegRec {
 // the worst-case rounding of each plane position over the box; the reference's FP64 tests decide.
 struct ColWalk {
     float P0, B0, C0, P1, B1, C1;
-    float ub, uc, Rp2, m0, m1;
+    float m0, m1;
     int16_t lo_a, lo_b, lo_c, a0, a1;   // a0..a1: the pair's a-range in the brick image
     int8_t ax, up0, up1, use1;
 };
@@ -352,58 +352,99 @@ struct HitQ {
     int16_t ip, jp, kp, owner;                       // extended indices, owning lane
 };
 
-// Widening of a band bound x computed as r * rcp(B) (hardware reciprocal, within 1 ulp): covers the
-// reciprocal's and the product's rounding with a wide factor.
-__device__ __forceinline__ float band_slack(float x) { return 1e-6f * fabsf(x) + 1e-5f; }
+// Per-pair constants of the row stage (registers, lane = pair). Each end-plane condition is a
+// bound on db that is linear in dc; the ellipse's centre is linear in Dc = dc - uc and its squared
+// half-width quadratic. The constants are formed in FP64 from the single-precision walk and
+// rounded once; a row then costs two plane lines, one square root and the clipping.
+struct RowBand {
+    float al0, be0, al1, be1;      // plane bounds: db >= / <= al + be dc (lower where lo0 / lo1)
+    float cm, cp, g, k0, k1, uc;   // ellipse: db in [cm + g Dc - hw, cp + g Dc + hw], hw^2 = k0 - k1 Dc^2
+    float b0r, b1r;                // the pair's b-range relative to lo_b
+    int32_t lo_b, lo_c;
+    bool lo0, lo1;
+};
+
+// One plane's condition B db >= Q - C dc (s = +1) or B db <= Q - C dc (s = -1) as a line
+// al + be dc bounding db, widened by 1e-6 of its largest magnitude over rows |dc| <= dmax plus
+// 1e-5 (covers the rounding of al, be and of the fma that evaluates it; Q carries twice the
+// column stage's margin). With B = 0 it is a condition on dc alone, applied to the row range c0..c1.
+__device__ __forceinline__ void plane_line(double Q, double B, double C, double s, double dmax, int32_t lo_c,
+                                           float &al, float &be, bool &lower, int32_t &c0, int32_t &c1) {
+    if (B != 0.0) {
+        const double iB = 1.0 / B, a = Q * iB, b = -C * iB;
+        const double e = 1e-6 * (fabs(a) + fabs(b) * dmax) + 1e-5;
+        lower = (B > 0.0) == (s > 0.0);
+        al = (float)(lower ? a - e : a + e);
+        be = (float)b;
+    } else {   // s C dc >= s Q
+        lower = true;
+        al = -1e9f;
+        be = 0.0f;
+        const double Cs = s * C, Qs = s * Q;
+        if (Cs == 0.0) {
+            if (Qs > 0.0) c1 = c0 - 1;
+        } else {
+            const double x = fmin(fmax(Qs / Cs, -1e6), 1e6), e = 1e-6 * fabs(x) + 1e-5;
+            if (Cs > 0.0) c0 = max(c0, lo_c + (int32_t)ceil(x - e));
+            else c1 = min(c1, lo_c + (int32_t)floor(x + e));
+        }
+    }
+}
+
+// Row constants of a pair whose walk is w, b-range b0..b1 and rows c0..c1 (narrowed here where an
+// end plane is parallel to the b-axis). (ub, uc): the chord's crossing of the plane t0 = 0 relative
+// to (lo_b, lo_c); Rp2: the squared ellipse radius, already widened by 1e-5 relative (which covers
+// the rounding of the half-width).
+__device__ __forceinline__ RowBand row_setup(const ColWalk &w, double ub, double uc, double Rp2, int32_t b0,
+                                             int32_t b1, int32_t &c0, int32_t &c1) {
+    RowBand rb;
+    rb.lo_b = w.lo_b;
+    rb.lo_c = w.lo_c;
+    rb.b0r = (float)(b0 - w.lo_b);
+    rb.b1r = (float)(b1 - w.lo_b);
+    const double dmax = fmax(fabs((double)(c0 - w.lo_c)), fabs((double)(c1 - w.lo_c)));
+    const double A0 = w.a0 - w.lo_a, A1 = w.a1 - w.lo_a;
+    {   // plane 0: up0 needs B0 db >= P0 - C0 dc - 2 m0 - A1, else B0 db <= P0 - C0 dc + 2 m0 - A0
+        const double m2 = 2.0 * (double)w.m0;
+        const double Q = w.up0 ? (double)w.P0 - m2 - A1 : (double)w.P0 + m2 - A0;
+        plane_line(Q, w.B0, w.C0, w.up0 ? 1.0 : -1.0, dmax, w.lo_c, rb.al0, rb.be0, rb.lo0, c0, c1);
+    }
+    rb.lo1 = true;
+    rb.al1 = -1e9f;
+    rb.be1 = 0.0f;
+    if (w.use1) {   // plane 1: up1 needs B1 db <= P1 - C1 dc + 2 m1 - A0, else B1 db >= P1 - C1 dc - 2 m1 - A1
+        const double m2 = 2.0 * (double)w.m1;
+        const double Q = w.up1 ? (double)w.P1 + m2 - A0 : (double)w.P1 - m2 - A1;
+        plane_line(Q, w.B1, w.C1, w.up1 ? -1.0 : 1.0, dmax, w.lo_c, rb.al1, rb.be1, rb.lo1, c0, c1);
+    }
+    // ellipse (1 + B0^2) Db^2 + 2 B0 C0 Db Dc + (1 + C0^2) Dc^2 < Rp2, Db = db - ub: for each Dc,
+    // Db = -B0 C0 Dc / eb +- sqrt(Rp2 / eb - (1 + B0^2 + C0^2) Dc^2 / eb^2) with eb = 1 + B0^2
+    const double B0 = w.B0, C0 = w.C0, eb = 1.0 + B0 * B0, ieb = 1.0 / eb;
+    rb.k0 = (float)(Rp2 * ieb);
+    rb.k1 = (float)((eb + C0 * C0) * ieb * ieb);
+    rb.g = (float)(-B0 * C0 * ieb);
+    rb.cm = (float)(ub - 1e-3);
+    rb.cp = (float)(ub + 1e-3);
+    rb.uc = (float)uc;
+    return rb;
+}
 
 // b-range of row ic whose columns can hold candidates: the slab interval reaches a-indices
-// a0..a1 (each plane condition is linear in db) and the column crosses the plane t0 = 0 inside
-// the ellipse of radius Rp (a quadratic in db). A superset, clipped to b0..b1; false if empty.
-__device__ __forceinline__ bool row_band(const ColWalk &w, int32_t b0, int32_t b1, int32_t ic, int32_t &bl,
-                                         int32_t &bh) {
-    // branch-free: every condition is evaluated and the flags select
-    const float dcn = (float)(ic - w.lo_c);
-    const float A0 = (float)(w.a0 - w.lo_a), A1 = (float)(w.a1 - w.lo_a);
-    float lo = -1e9f, hi = 1e9f;
-    bool ok = true;
-    {   // plane 0: up0 needs B0 db >= r, else B0 db <= r
-        const float q0 = __builtin_fmaf(-w.C0, dcn, w.P0);
-        const float m2 = 2.0f * w.m0;
-        const float r = w.up0 ? q0 - m2 - A1 : q0 + m2 - A0;
-        const bool z = w.B0 == 0.0f;
-        ok = ok && !(z && (w.up0 ? (0.0f < r) : (0.0f > r)));
-        const float x = r * __builtin_amdgcn_rcpf(z ? 1.0f : w.B0), e = band_slack(x);
-        const bool lower = (w.B0 > 0.0f) == (bool)w.up0;
-        lo = (!z && lower) ? fmaxf(lo, x - e) : lo;
-        hi = (!z && !lower) ? fminf(hi, x + e) : hi;
-    }
-    {   // plane 1 (use1): up1 needs B1 db <= r, else B1 db >= r
-        const float q1 = __builtin_fmaf(-w.C1, dcn, w.P1);
-        const float m2 = 2.0f * w.m1;
-        const float r = w.up1 ? q1 + m2 - A0 : q1 - m2 - A1;
-        const bool z = w.B1 == 0.0f, use = w.use1;
-        ok = ok && !(use && z && (w.up1 ? (0.0f > r) : (0.0f < r)));
-        const float x = r * __builtin_amdgcn_rcpf(z ? 1.0f : w.B1), e = band_slack(x);
-        const bool lower = (w.B1 > 0.0f) != (bool)w.up1;
-        lo = (use && !z && lower) ? fmaxf(lo, x - e) : lo;
-        hi = (use && !z && !lower) ? fminf(hi, x + e) : hi;
-    }
-    {   // ellipse: (1 + B0^2) Db^2 + 2 B0 C0 Dc Db + (1 + C0^2) Dc^2 < Rp^2
-        const float Dc = dcn - w.uc;
-        const float eb = __builtin_fmaf(w.B0, w.B0, 1.0f), ebc = w.B0 * w.C0 * Dc;
-        const float cc = __builtin_fmaf(__builtin_fmaf(w.C0, w.C0, 1.0f) * Dc, Dc, -w.Rp2);
-        const float D = __builtin_fmaf(ebc, ebc, -eb * cc);
-        ok = ok && D >= 0.0f;
-        const float sq = __builtin_amdgcn_sqrtf(fmaxf(D, 0.0f)), ie = __builtin_amdgcn_rcpf(eb);
-        lo = fmaxf(lo, w.ub + (-ebc - sq) * ie - 1e-3f);
-        hi = fminf(hi, w.ub + (-ebc + sq) * ie + 1e-3f);
-    }
-    lo = fmaxf(lo, -1e9f);
-    hi = fminf(hi, 1e9f);
-    if (!ok || !(lo <= hi)) return false;
-    const int32_t l = w.lo_b + (int32_t)ceilf(lo), h = w.lo_b + (int32_t)floorf(hi);
-    bl = l > b0 ? l : b0;
-    bh = h < b1 ? h : b1;
+// a0..a1 (each plane condition is linear in dc) and the column crosses the plane t0 = 0 inside
+// the ellipse of radius Rp. A superset, clipped to b0..b1; false if empty.
+__device__ __forceinline__ bool row_band(const RowBand &rb, int32_t ic, int32_t &bl, int32_t &bh) {
+    const float dcn = (float)(ic - rb.lo_c);
+    const float x0 = __builtin_fmaf(rb.be0, dcn, rb.al0), x1 = __builtin_fmaf(rb.be1, dcn, rb.al1);
+    const float Dc = dcn - rb.uc;
+    const float h2 = __builtin_fmaf(-rb.k1 * Dc, Dc, rb.k0);
+    const float hw = __builtin_amdgcn_sqrtf(fmaxf(h2, 0.0f));
+    const float lo = fmaxf(fmaxf(rb.lo0 ? x0 : rb.b0r, rb.lo1 ? x1 : rb.b0r),
+                           fmaxf(__builtin_fmaf(rb.g, Dc, rb.cm) - hw, rb.b0r));
+    const float hi = fminf(fminf(rb.lo0 ? rb.b1r : x0, rb.lo1 ? rb.b1r : x1),
+                           fminf(__builtin_fmaf(rb.g, Dc, rb.cp) + hw, rb.b1r));
+    if (!(h2 >= 0.0f) || !(lo <= hi)) return false;
+    bl = rb.lo_b + (int32_t)ceilf(lo);
+    bh = rb.lo_b + (int32_t)floorf(hi);
     return bl <= bh;
 }
 
@@ -510,8 +551,14 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
     W.novf = __builtin_amdgcn_readfirstlane(W.novf + (uint32_t)__popcll(__ballot(ovf)));
 }
 
+// Columns walked per row segment and pass: a longer segment's remainder is queued again, so that a
+// pass is not held for its longest segment while the other lanes idle.
+constexpr int32_t kColCap = 8;
+
 // Walks the columns of row segments [segn - cnt, segn), one segment per lane and one column of
-// it per step; candidates go to the candidate queue.
+// it per step, at most kColCap columns each (the remainders go back to the segment queue, which
+// has room: at most 63 segments stay queued plus at most 64 remainders); candidates go to the
+// candidate queue.
 template <bool F64, bool FAST>
 __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, int cnt) {
     const int lane = W.lane;
@@ -528,11 +575,12 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
     const int ax = lane < cnt ? (int)w.ax : 0;
     const int bx = ax == 2 ? 0 : ax + 1;
     W.segn = __builtin_amdgcn_readfirstlane(W.segn - cnt);
+    const int32_t bl0 = ib, steps = len < kColCap ? len : kColCap;
     unsigned long long mc;
-    for (int32_t col = 0; (mc = __ballot(col < len)) != 0ull; col++, ib++) {
+    for (int32_t col = 0; (mc = __ballot(col < steps)) != 0ull; col++, ib++) {
         W.ncols = __builtin_amdgcn_readfirstlane(W.ncols + (uint32_t)__popcll(mc));
         int32_t nc = 0, ia0 = 0;
-        if (col < len) nc = column_range(w, ib, ic, &ia0);
+        if (col < steps) nc = column_range(w, ib, ic, &ia0);
         for (int u = 0; __ballot(u < nc); u++) {
             const bool c = u < nc;
             const unsigned long long m = __ballot(c);
@@ -551,6 +599,12 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
             if (W.candn >= 64) stage_candidates<F64, FAST>(W, me, 64);
         }
     }
+    const bool more = len > kColCap;
+    const unsigned long long mr = __ballot(more);
+    if (more) W.segs[W.segn + __popcll(mr & lanemask_lt())] =
+        RowSeg{(int16_t)ic, (int16_t)(bl0 + kColCap), (uint8_t)(len - kColCap), (uint8_t)own, 0};
+    W.segn = __builtin_amdgcn_readfirstlane(W.segn + __popcll(mr));
+    wave_sync_lds();
 }
 
 template <bool F64, bool FAST>
@@ -628,11 +682,12 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
             }
             // ---- stage 0 (lane = pair): the sub-segment to registers, its column walk to LDS ----
             SegRec me;
-            ColWalk w;
+            RowBand rb;
             int32_t b0 = 0, b1 = -1, c0 = 0, c1 = -1;
             bool have = pi < p1;
             {
                 SubSeg sg;
+                ColWalk w;
                 if (have) subseg_load_pair(g, ri, (int64_t)q_cur, s_cur, l_cur, sg);
                 if (have) {
 #pragma unroll
@@ -690,8 +745,6 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                     w.P0 = (float)((ua + B0 * ub) + C0 * uc);
                     w.m0 = plane_margin((ua + B0 * ub) + C0 * uc, B0, C0, Wb, Wc);
                     w.m1 = 0.0f;
-                    w.ub = (float)ub;
-                    w.uc = (float)uc;
                     w.up0 = t0a > 0.0;
                     const double t1a = pick3(sg.tci1, ax);
                     w.use1 = fabs(t1a) >= 0.05;
@@ -706,7 +759,6 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                         const double tand = sqrt(fmax(1.0 - cd * cd, 0.0)) * (1.0 / cd);
                         Rp = (sg.R + (sg.ndci0 + sg.R * tand) * sqrt(fmax(1.0 - t0a * t0a, 0.0)) * fabs(it0a)) * inv_dx;
                     }
-                    w.Rp2 = (float)fmin(Rp * Rp * (1.0 + 1e-5) + 1e-3, 1e30);
                     if (w.use1) {
                         const double va = (pick3(sg.ci1, ax) - sg.start) * inv_dx + 1.0 - w.lo_a;
                         const double vb = (pick3(sg.ci1, bx) - sg.start) * inv_dx + 1.0 - w.lo_b;
@@ -720,6 +772,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                         w.up1 = t1a > 0.0;
                     }
                     W.walks[lane] = w;
+                    rb = row_setup(w, ub, uc, fmin(Rp * Rp * (1.0 + 1e-5) + 1e-3, 1e30), b0, b1, c0, c1);
                 } else {
 #pragma unroll
                     for (int c = 0; c < 3; c++) me.ci0[c] = me.ci1[c] = me.tci0[c] = me.tci1[c] = 0.0;
@@ -735,16 +788,16 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
             // ---- stage 1 (lane = pair): one row of the pair per step, non-empty bands queued ----
             for (int32_t ic = c0; __ballot(have && ic <= c1); ic++) {
                 int32_t bl = 0, bh = -1;
-                const bool sgok = have && ic <= c1 && row_band(w, b0, b1, ic, bl, bh);
+                const bool sgok = have && ic <= c1 && row_band(rb, ic, bl, bh);
                 const unsigned long long m = __ballot(sgok);
                 if (sgok) W.segs[W.segn + __popcll(m & lanemask_lt())] =
                     RowSeg{(int16_t)ic, (int16_t)bl, (uint8_t)(bh - bl + 1), (uint8_t)lane, 0};
                 W.segn = __builtin_amdgcn_readfirstlane(W.segn + __popcll(m));
                 wave_sync_lds();
-                if (W.segn >= 64) stage_columns<F64, FAST>(W, me, 64);
+                while (W.segn >= 64) stage_columns<F64, FAST>(W, me, 64);
             }
             // the registers of these pairs are replaced by the next chunk: flush every stage
-            if (W.segn > 0) stage_columns<F64, FAST>(W, me, W.segn);
+            while (W.segn > 0) stage_columns<F64, FAST>(W, me, W.segn);
             if (W.candn > 0) stage_candidates<F64, FAST>(W, me, W.candn);
         }
         __syncthreads();

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from fastweavturb_amd import fbb
-from fastweavturb_amd.woven_io import read_inputs
+from fastweavturb_amd.woven_io import Tubes, read_inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -157,6 +157,39 @@ def _expected_pairs(lohi, N, bd=16):
             n *= cnt
         total += n
     return total
+
+
+def _planar_tubes(n=48):
+    """Circles in the x-y and y-z planes and a straight tube along x: many chords have exact zero
+    components, so end planes parallel to a column axis and chords along an axis both occur."""
+    th = 2 * np.pi * np.arange(n) / n
+    cx = np.concatenate([3.0 + 1.2 * np.cos(th), np.full(n, 4.5), 2 * np.pi * np.arange(n) / n])
+    cy = np.concatenate([2.5 + 1.2 * np.sin(th), 3.0 + 1.0 * np.cos(th), np.full(n, 5.0)])
+    cz = np.concatenate([np.full(n, 1.75), 3.5 + 1.4 * np.sin(th), np.full(n, 0.6)])
+    return Tubes(np.full(3, n, np.int32), cx, cy, cz, np.array([0.3, 0.25, 0.2]), np.array([1.0, -0.7, 0.5]),
+                 np.ones(3, np.int32), np.zeros(3, np.int32))
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_deposit_planar_tubes(oracle, precision):
+    p, _ = fbb.config_small(N=64)
+    p.line_s, p.line_e = 1, 3
+    t = _planar_tubes()
+    geo = oracle.Geometry(oracle_params(oracle, p, t), t.npoint, t.cx, t.cy, t.cz)
+    tci = geo.tci[np.linalg.norm(geo.tci, axis=1) > 0]
+    ax = np.argmax(np.abs(tci), 1)
+    r = np.arange(len(tci))
+    B = tci[r, (ax + 1) % 3]
+    C = tci[r, (ax + 2) % 3]
+    assert ((B == 0) & (C != 0)).sum() > 50 and ((B == 0) & (C == 0)).sum() > 50
+    om_ref, hits = geo.deposit(t.sigma, t.gamma, t.n_T)
+    g = gpu_ctx(p, t, precision)
+    g.build_segments()
+    g.bin()
+    d = g.deposit()
+    assert d["hits"] == hits
+    err = rel_l2(g.get_fields("omega"), om_ref)
+    assert err <= (1e-12 if precision == 64 else 1e-5), err
 
 
 def test_wide_box_list_growth():
