@@ -448,20 +448,44 @@ __device__ __forceinline__ bool row_band(const RowBand &rb, int32_t ic, int32_t 
     return bl <= bh;
 }
 
-// Candidate a-range of column (ib, ic): returns the count, first index in *ia0.
-__device__ __forceinline__ int32_t column_range(const ColWalk &w, int32_t ib, int32_t ic, int32_t *ia0) {
-    const float db = (float)(ib - w.lo_b), dcn = (float)(ic - w.lo_c);
-    float flo = (float)(w.a0 - w.lo_a), fhi = (float)(w.a1 - w.lo_a);
-    // branch-free: both bounds of each plane are formed and the flags select (P1 = B1 = C1 = 0
-    // when use1 is clear)
-    const float p0 = __builtin_fmaf(-w.B0, db, __builtin_fmaf(-w.C0, dcn, w.P0));
-    const float p1 = __builtin_fmaf(-w.B1, db, __builtin_fmaf(-w.C1, dcn, w.P1));
+// The a-range bounds of one row segment's columns: on row dc the plane positions are linear in
+// db, so each plane gives one line q - B db that bounds a from below or from above, with the
+// margin already applied (q = (P - C dc) -/+ m, rounded twice before the fma that evaluates it:
+// within 1.8e-7 of |P| + |B| Wb + |C| Wc, under the margin's 3e-7). Unused slots hold B = 0 and
+// q = -/+1e9.
+struct ColLines {
+    float qL0, BL0, qL1, BL1;   // lower bounds
+    float qU0, BU0, qU1, BU1;   // upper bounds
+    float A0, A1;               // the pair's a-range, relative to lo_a
+    int32_t lo_a, lo_b;
+};
+
+__device__ __forceinline__ ColLines col_lines(const ColWalk &w, int32_t ic) {
+    ColLines L;
+    const float dcn = (float)(ic - w.lo_c);
+    const float p0 = __builtin_fmaf(-w.C0, dcn, w.P0), p1 = __builtin_fmaf(-w.C1, dcn, w.P1);
     const bool up0 = w.up0, lo1 = w.use1 && !w.up1, hi1 = w.use1 && w.up1;
-    flo = fmaxf(flo, up0 ? ceilf(p0 - w.m0) : flo);
-    fhi = fminf(fhi, up0 ? fhi : floorf(p0 + w.m0));
-    flo = fmaxf(flo, lo1 ? ceilf(p1 - w.m1) : flo);
-    fhi = fminf(fhi, hi1 ? floorf(p1 + w.m1) : fhi);
-    *ia0 = w.lo_a + (int32_t)flo;
+    L.qL0 = up0 ? p0 - w.m0 : -1e9f;
+    L.BL0 = up0 ? w.B0 : 0.0f;
+    L.qU0 = up0 ? 1e9f : p0 + w.m0;
+    L.BU0 = up0 ? 0.0f : w.B0;
+    L.qL1 = lo1 ? p1 - w.m1 : -1e9f;
+    L.BL1 = lo1 ? w.B1 : 0.0f;
+    L.qU1 = hi1 ? p1 + w.m1 : 1e9f;
+    L.BU1 = hi1 ? w.B1 : 0.0f;
+    L.A0 = (float)(w.a0 - w.lo_a);
+    L.A1 = (float)(w.a1 - w.lo_a);
+    L.lo_a = w.lo_a;
+    L.lo_b = w.lo_b;
+    return L;
+}
+
+// Candidate a-range of column ib of a row segment: returns the count, first index in *ia0.
+__device__ __forceinline__ int32_t column_range(const ColLines &L, int32_t ib, int32_t *ia0) {
+    const float db = (float)(ib - L.lo_b);
+    const float flo = ceilf(fmaxf(fmaxf(L.A0, __builtin_fmaf(-L.BL0, db, L.qL0)), __builtin_fmaf(-L.BL1, db, L.qL1)));
+    const float fhi = floorf(fminf(fminf(L.A1, __builtin_fmaf(-L.BU0, db, L.qU0)), __builtin_fmaf(-L.BU1, db, L.qU1)));
+    *ia0 = L.lo_a + (int32_t)flo;
     return flo <= fhi ? (int32_t)(fhi - flo) + 1 : 0;
 }
 
@@ -562,17 +586,18 @@ constexpr int32_t kColCap = 8;
 template <bool F64, bool FAST>
 __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, int cnt) {
     const int lane = W.lane;
-    int32_t len = 0, ib = 0, ic = 0, own = 0;
-    ColWalk w;
+    int32_t len = 0, ib = 0, ic = 0, own = 0, ax = 0;
+    ColLines L;
     if (lane < cnt) {
         const RowSeg r = W.segs[W.segn - cnt + lane];
         len = r.len;
         ib = r.bl;
         ic = r.c;
         own = r.pair;
-        w = W.walks[own];
+        const ColWalk w = W.walks[own];
+        ax = w.ax;
+        L = col_lines(w, ic);
     }
-    const int ax = lane < cnt ? (int)w.ax : 0;
     const int bx = ax == 2 ? 0 : ax + 1;
     W.segn = __builtin_amdgcn_readfirstlane(W.segn - cnt);
     const int32_t bl0 = ib, steps = len < kColCap ? len : kColCap;
@@ -580,7 +605,7 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
     for (int32_t col = 0; (mc = __ballot(col < steps)) != 0ull; col++, ib++) {
         W.ncols = __builtin_amdgcn_readfirstlane(W.ncols + (uint32_t)__popcll(mc));
         int32_t nc = 0, ia0 = 0;
-        if (col < steps) nc = column_range(w, ib, ic, &ia0);
+        if (col < steps) nc = column_range(L, ib, &ia0);
         for (int u = 0; __ballot(u < nc); u++) {
             const bool c = u < nc;
             const unsigned long long m = __ballot(c);
