@@ -109,9 +109,9 @@ __device__ __forceinline__ void subseg_load_pair(const Geo &g, const Run &ri, in
     }
 }
 
-__device__ __forceinline__ unsigned long long lanemask_lt() {
-    unsigned lane = __lane_id();
-    return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+// Number of lanes below this one whose bit is set in the wave mask m.
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 
@@ -543,6 +543,7 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
     const HitQ h = W.candq[W.candn - cnt + (lane < cnt ? lane : 0)];
     wave_sync_lds();
     W.candn = __builtin_amdgcn_readfirstlane(W.candn - cnt);
+    W.ncand += (uint32_t)cnt;
     SubSeg sg;
     fetch_owner<FAST>(me, lane < cnt ? h.owner : lane, *W.ri, sg);
     bool hit = false, ovf = false;
@@ -606,27 +607,25 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
         W.ncols = __builtin_amdgcn_readfirstlane(W.ncols + (uint32_t)__popcll(mc));
         int32_t nc = 0, ia0 = 0;
         if (col < steps) nc = column_range(L, ib, &ia0);
-        for (int u = 0; __ballot(u < nc); u++) {
-            const bool c = u < nc;
-            const unsigned long long m = __ballot(c);
-            if (c) {
+        unsigned long long m;
+        for (int u = 0; (m = __ballot(u < nc)) != 0ull; u++) {
+            if (u < nc) {
                 const int32_t ia = ia0 + u;
                 HitQ q;
                 q.ip = (int16_t)(ax == 0 ? ia : (bx == 0 ? ib : ic));
                 q.jp = (int16_t)(ax == 1 ? ia : (bx == 1 ? ib : ic));
                 q.kp = (int16_t)(ax == 2 ? ia : (bx == 2 ? ib : ic));
                 q.owner = (int16_t)own;
-                W.candq[W.candn + __popcll(m & lanemask_lt())] = q;
+                W.candq[W.candn + lanes_below(m)] = q;
             }
             W.candn = __builtin_amdgcn_readfirstlane(W.candn + __popcll(m));
-            W.ncand = __builtin_amdgcn_readfirstlane(W.ncand + (uint32_t)__popcll(m));
             wave_sync_lds();
             if (W.candn >= 64) stage_candidates<F64, FAST>(W, me, 64);
         }
     }
     const bool more = len > kColCap;
     const unsigned long long mr = __ballot(more);
-    if (more) W.segs[W.segn + __popcll(mr & lanemask_lt())] =
+    if (more) W.segs[W.segn + lanes_below(mr)] =
         RowSeg{(int16_t)ic, (int16_t)(bl0 + kColCap), (uint8_t)(len - kColCap), (uint8_t)own, 0};
     W.segn = __builtin_amdgcn_readfirstlane(W.segn + __popcll(mr));
     wave_sync_lds();
@@ -815,7 +814,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                 int32_t bl = 0, bh = -1;
                 const bool sgok = have && ic <= c1 && row_band(rb, ic, bl, bh);
                 const unsigned long long m = __ballot(sgok);
-                if (sgok) W.segs[W.segn + __popcll(m & lanemask_lt())] =
+                if (sgok) W.segs[W.segn + lanes_below(m)] =
                     RowSeg{(int16_t)ic, (int16_t)bl, (uint8_t)(bh - bl + 1), (uint8_t)lane, 0};
                 W.segn = __builtin_amdgcn_readfirstlane(W.segn + __popcll(m));
                 wave_sync_lds();
