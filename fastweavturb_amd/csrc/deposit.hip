@@ -352,6 +352,14 @@ struct HitQ {
     int16_t ip, jp, kp, owner;                       // extended indices, owning lane
 };
 
+// 1/x from the hardware reciprocal refined by one Newton step (within a few ulp, never exactly
+// rounded): only for quantities that feed single-precision bounds carrying their own margins.
+// x must be nonzero and finite.
+__device__ __forceinline__ double rcp_nr(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+}
+
 // Per-pair constants of the row stage (registers, lane = pair). Each end-plane condition is a
 // bound on db that is linear in dc; the ellipse's centre is linear in Dc = dc - uc and its squared
 // half-width quadratic. The constants are formed in FP64 from the single-precision walk and
@@ -371,7 +379,7 @@ struct RowBand {
 __device__ __forceinline__ void plane_line(double Q, double B, double C, double s, double dmax, int32_t lo_c,
                                            float &al, float &be, bool &lower, int32_t &c0, int32_t &c1) {
     if (B != 0.0) {
-        const double iB = 1.0 / B, a = Q * iB, b = -C * iB;
+        const double iB = rcp_nr(B), a = Q * iB, b = -C * iB;
         const double e = 1e-6 * (fabs(a) + fabs(b) * dmax) + 1e-5;
         lower = (B > 0.0) == (s > 0.0);
         al = (float)(lower ? a - e : a + e);
@@ -419,7 +427,7 @@ __device__ __forceinline__ RowBand row_setup(const ColWalk &w, double ub, double
     }
     // ellipse (1 + B0^2) Db^2 + 2 B0 C0 Db Dc + (1 + C0^2) Dc^2 < Rp2, Db = db - ub: for each Dc,
     // Db = -B0 C0 Dc / eb +- sqrt(Rp2 / eb - (1 + B0^2 + C0^2) Dc^2 / eb^2) with eb = 1 + B0^2
-    const double B0 = w.B0, C0 = w.C0, eb = 1.0 + B0 * B0, ieb = 1.0 / eb;
+    const double B0 = w.B0, C0 = w.C0, eb = 1.0 + B0 * B0, ieb = rcp_nr(eb);
     rb.k0 = (float)(Rp2 * ieb);
     rb.k1 = (float)((eb + C0 * C0) * ieb * ieb);
     rb.g = (float)(-B0 * C0 * ieb);
@@ -758,7 +766,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                     b1 = pick3(chi, bx);
                     c0 = pick3(clo, cx);
                     c1 = pick3(chi, cx);
-                    const double t0a = pick3(sg.tci0, ax), it0a = 1.0 / t0a;
+                    const double t0a = pick3(sg.tci0, ax), it0a = rcp_nr(t0a);
                     const double ua = (pick3(sg.ci0, ax) - sg.start) * inv_dx + 1.0 - w.lo_a;
                     const double ub = (pick3(sg.ci0, bx) - sg.start) * inv_dx + 1.0 - w.lo_b;
                     const double uc = (pick3(sg.ci0, cx) - sg.start) * inv_dx + 1.0 - w.lo_c;
@@ -780,14 +788,14 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                     const double cd = (sg.tci0[0] * sg.tci1[0] + sg.tci0[1] * sg.tci1[1]) + sg.tci0[2] * sg.tci1[2];
                     double Rp = 1e30;
                     if (cd > 0.2) {
-                        const double tand = sqrt(fmax(1.0 - cd * cd, 0.0)) * (1.0 / cd);
+                        const double tand = sqrt(fmax(1.0 - cd * cd, 0.0)) * rcp_nr(cd);
                         Rp = (sg.R + (sg.ndci0 + sg.R * tand) * sqrt(fmax(1.0 - t0a * t0a, 0.0)) * fabs(it0a)) * inv_dx;
                     }
                     if (w.use1) {
                         const double va = (pick3(sg.ci1, ax) - sg.start) * inv_dx + 1.0 - w.lo_a;
                         const double vb = (pick3(sg.ci1, bx) - sg.start) * inv_dx + 1.0 - w.lo_b;
                         const double vc = (pick3(sg.ci1, cx) - sg.start) * inv_dx + 1.0 - w.lo_c;
-                        const double it1a = 1.0 / t1a;
+                        const double it1a = rcp_nr(t1a);
                         const double B1 = pick3(sg.tci1, bx) * it1a, C1 = pick3(sg.tci1, cx) * it1a;
                         w.B1 = (float)B1;
                         w.C1 = (float)C1;
