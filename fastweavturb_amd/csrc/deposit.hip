@@ -539,7 +539,8 @@ struct Wave {
     int ox, oy, oz;
     int lane;
     int segn, candn;
-    uint32_t ncols, ncand, nhit, novf;   // wave-uniform counts (scalar registers; below 2^32 per wave)
+    uint32_t ncand, nhit, novf;   // wave-uniform counts (scalar registers; below 2^32 per wave)
+    uint32_t ncols;               // columns walked by this lane
 };
 
 // Tests candidates [candn - cnt, candn) with the reference predicate and adds the contribution of
@@ -610,9 +611,8 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
     const int bx = ax == 2 ? 0 : ax + 1;
     W.segn = __builtin_amdgcn_readfirstlane(W.segn - cnt);
     const int32_t bl0 = ib, steps = len < kColCap ? len : kColCap;
-    unsigned long long mc;
-    for (int32_t col = 0; (mc = __ballot(col < steps)) != 0ull; col++, ib++) {
-        W.ncols = __builtin_amdgcn_readfirstlane(W.ncols + (uint32_t)__popcll(mc));
+    W.ncols += (uint32_t)steps;
+    for (int32_t col = 0; __ballot(col < steps) != 0ull; col++, ib++) {
         int32_t nc = 0, ia0 = 0;
         if (col < steps) nc = column_range(L, ib, &ia0);
         unsigned long long m;
@@ -855,9 +855,12 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
         }
         __syncthreads();
     }
-    // counters (wave-uniform): one atomic per wave
+    // counters: one atomic per wave (the column counts summed over the lanes first)
+    unsigned long long ncols = W.ncols;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ncols += __shfl_xor(ncols, o, 64);
     if (lane == 0) {
-        atomicAdd(A.counters + 0, (unsigned long long)W.ncols);
+        atomicAdd(A.counters + 0, ncols);
         atomicAdd(A.counters + 1, (unsigned long long)W.ncand);
         atomicAdd(A.counters + 2, (unsigned long long)W.nhit);
         if (W.novf) atomicAdd(A.counters + 3, (unsigned long long)W.novf);
