@@ -326,7 +326,8 @@ struct SegRec {
 // (relative to lo_a) P0 - B0 db - C0 dc and the plane t1 = 0 at P1 - B1 db - C1 dc (db, dc
 // relative to lo_b, lo_c). Every cell that can pass the reference's tests lies on a column
 // whose crossing with the plane t0 = 0 is within Rp of ci0 (cells): with (ub, uc) the (b, c)
-// position of ci0, (B0 Db + C0 Dc)^2 + Db^2 + Dc^2 < Rp^2 for Db = db - ub, Dc = dc - uc.
+// position of ci0, (B0 Db + C0 Dc)^2 + Db^2 + Dc^2 < Rp^2 for Db = db - ub, Dc = dc - uc (the row
+// stage's ellipse, whose constants stay in the pair's registers: RowBand).
 // Single precision: these only select candidates, with margins m0, m1 of at least three times
 // the worst-case rounding of each plane position over the box; the reference's FP64 tests decide.
 struct ColWalk {
