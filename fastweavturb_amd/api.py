@@ -317,21 +317,21 @@ class WovenGPU:
 
     # --- intermediate arrays (parity inspection) -------------------------------------------
     def sections(self):
-        S = self.seg_stats["nsec_all"]
+        S = self.segment_stats()["nsec_all"]
         ntf = np.zeros(S, dtype=np.int32)
         dtf = np.zeros(S)
         self._check(self._L.wt_get_sections(self._h, _ip(ntf), _dp(dtf)))
         return ntf, dtf
 
     def subpoints(self):
-        n = self.seg_stats["nsub_all"]
+        n = self.segment_stats()["nsub_all"]
         ci = np.zeros((n, 3))
         s = np.zeros(n)
         self._check(self._L.wt_get_subpoints(self._h, _dp(ci), _dp(s)))
         return ci, s
 
     def bins(self):
-        n = self.seg_stats["nsub_all"]
+        n = self.segment_stats()["nsub_all"]
         lohi = np.zeros((n, 6), dtype=np.int32)
         R = np.zeros(n)
         self._check(self._L.wt_get_bins(self._h, _ip(lohi), _dp(R)))

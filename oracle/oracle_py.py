@@ -186,6 +186,19 @@ class Geometry:
                                 C.byref(self._g), _dp(om), C.byref(hits)))
         return om.reshape(3, N, N, N), int(hits.value)
 
+    def deposit_planes(self, sigma_line, Gamma_line, n_T_line, kfirst, klast):
+        """or_deposit_planes: the vorticity on planes z = kfirst..klast (1-based) only, as
+        (3, klast - kfirst + 1, N, N) [c, z, y, x], and the deposits on those planes."""
+        N = self.params.N
+        om = np.zeros(3 * N ** 3, dtype=np.float64)
+        sig = np.ascontiguousarray(sigma_line, dtype=np.float64)
+        gam = np.ascontiguousarray(Gamma_line, dtype=np.float64)
+        nT = np.ascontiguousarray(n_T_line, dtype=np.int32)
+        hits = C.c_int64(0)
+        _check(lib().or_deposit_planes(C.byref(self.params), _dp(sig), _dp(gam), _ip(nT), C.byref(self._g),
+                                       int(kfirst), int(klast), _dp(om), C.byref(hits)))
+        return om.reshape(3, N, N, N)[:, kfirst - 1:klast].copy(), int(hits.value)
+
     def deposit_filtered(self, sigma_line, Gamma_line, n_T_line, if_filter):
         """or_deposit_filtered: the tube loop with the per-tube spectral filters."""
         N = self.params.N

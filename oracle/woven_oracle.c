@@ -4,8 +4,17 @@
  * product library never links it.
  *
  * A plain-C restatement of the reference Fortran (YYgroup/FastWeavTurb tube.f90 and
- * subroutine.f90), single-threaded, compiled with -ffp-contract=off so that every FP64
- * operation is one IEEE rounding, in the reference's evaluation order.
+ * subroutine.f90), compiled with -ffp-contract=off so that every FP64 operation is one IEEE
+ * rounding, in the reference's evaluation order.
+ *
+ * Threads (OpenMP; OMP_NUM_THREADS=1 gives the serial program) never change a result bit:
+ *  - geometry runs one tube per thread (tubes are independent, tube.f90:528-751);
+ *  - the deposition splits the grid into z-slabs as the reference's MPI ranks do
+ *    (tube.f90:803-804): every thread walks every sub-segment but only the planes k of its own
+ *    slab, so each cell receives its contributions in the serial order (sub-segment, then kp,
+ *    jp, ip);
+ *  - each 1-D line of a 3-D FFT is an independent transform;
+ *  - sums over the field (Etot, enstrophy, spectra, structure functions) stay serial.
  *
  * Conventions fixed here (the reference leaves them to its compiler):
  *  - real-valued integral exponents are multiplication chains: x**2 = x*x,
@@ -24,6 +33,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define PI 3.141592653589793115997963468544185161590576171875  /* 4*atan(1d0) */
 #define PI2 6.28318530717958623199592693708837032318115234375  /* 8*atan(1d0) */
@@ -221,7 +233,9 @@ int or_segments(const or_params *p, const double *cx, const double *cy, const do
     const int ntf_pre = p->ntf_pre;
     if (ntf_pre < 1 || p->ntf_times < 1) return fail("ntf_pre and ntf_times must be positive");
 
-    /* pass 1: coefficients, turning angle, ntf, dtf */
+    /* pass 1: coefficients, turning angle, ntf, dtf (one tube per thread) */
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
     for (int l = 0; l < nline; l++) {
         int64_t b = g->pt_base[l];
         int n_sec = (int)(g->pt_base[l + 1] - b);
@@ -273,12 +287,13 @@ int or_segments(const or_params *p, const double *cx, const double *cy, const do
             }
             g->theta[s] = theta;
             double q = ceil((double)p->ntf_times * theta / PI2);
-            if (!(q >= 0.0 && q < 1.0e8)) { free(pl); return fail("turning angle is not finite"); }
+            if (!(q >= 0.0 && q < 1.0e8)) { bad = 1; q = 0.0; }
             g->ntf[s] = (int32_t)q + 1;
             g->dtf[s] = (czepl[i + 1] - czepl[i]) / (1.0 * g->ntf[s]);
         }
         free(pl);
     }
+    if (bad) return fail("turning angle is not finite");
     /* sub-point layout */
     g->sp_base[0] = 0;
     for (int64_t s = 0; s < S; s++) g->sp_base[s + 1] = g->sp_base[s] + g->ntf[s] + 1;
@@ -289,6 +304,7 @@ int or_segments(const or_params *p, const double *cx, const double *cy, const do
     g->tci = (double *)calloc(3 * NSP, sizeof(double));
     g->ndci = (double *)calloc(NSP, sizeof(double));
     /* tube.f90:706-713: sub-points of every section (ghost included) */
+#pragma omp parallel for schedule(dynamic, 16)
     for (int l = 0; l < nline; l++) {
         double LD = g->Len_Disc[l];
         for (int64_t s = g->sec_base[l]; s < g->sec_base[l + 1]; s++) {
@@ -418,113 +434,157 @@ int or_pair_hits(const or_params *p, const or_geom *g, const or_bin_rec *b, int6
     return rho < b->R;
 }
 
-int or_deposit_range(const or_params *p, const double *sigma_line, const double *Gamma_line,
-                     const int32_t *n_T_line, const or_geom *g, int64_t seg_lo, int64_t seg_hi,
-                     double *omega, int64_t *hits_out) {
+/* tube.f90:800-892 for one sub-segment, restricted to the grid planes k in [klo, khi] (1-based,
+ * wrapped); returns the number of deposits. */
+static int64_t deposit_seg(const or_params *p, const or_geom *g, int l, int64_t seg, double Gamma, double smin,
+                           double smax, int n_T, double *omega, int32_t klo, int32_t khi) {
     const int N = p->N;
     const double dx = PI2 / N, start = -PI;
     const int64_t N3 = (int64_t)N * N * N;
     double *wx = omega, *wy = omega + N3, *wz = omega + 2 * N3;
-    int64_t hits = 0;
     const double eta = p->eta_t;
+    const double LD = g->Len_Disc[l], len = g->length[l];
+    int64_t hits = 0;
+    or_bin_rec b;
+    bin_one(p, smin, n_T, g, l, seg, &b);
+    int64_t sec = g->seg_sec[seg];
+    int64_t q = g->sp_base[sec] + g->seg_j[seg];
+    const double *coe = g->coe + 18 * sec;
+    double s0 = g->s[q], s1 = g->s[q + 1];
+    double sfirst = g->s[g->sp_base[sec]];
+    const double *tci0 = g->tci + 3 * q, *tci1 = g->tci + 3 * (q + 1);
+    double ndci0 = g->ndci[q];
+    double R = b.R;
+    double dzeta = p->dzeta_sup < g->dtf[sec] ? p->dzeta_sup : g->dtf[sec];
+    for (int32_t kp = b.lo[2]; kp <= b.hi[2]; kp++) {
+        int k = wrap_idx(kp, N);
+        if (!(k > 0 && k <= N)) continue;
+        if (k < klo || k > khi) continue;   /* another thread's slab */
+        for (int32_t jp = b.lo[1]; jp <= b.hi[1]; jp++)
+            for (int32_t ip = b.lo[0]; ip <= b.hi[0]; ip++) {
+                double X[3] = {mesh(ip, dx, start), mesh(jp, dx, start), mesh(kp, dx, start)};
+                double a0 = X[0] - b.ci0[0], a1 = X[1] - b.ci0[1], a2 = X[2] - b.ci0[2];
+                double t0 = (a0 * tci0[0] + a1 * tci0[1]) + a2 * tci0[2];
+                if (!(t0 >= 0.0)) continue;
+                double t1 = ((X[0] - b.ci1[0]) * tci1[0] + (X[1] - b.ci1[1]) * tci1[1]) +
+                            (X[2] - b.ci1[2]) * tci1[2];
+                if (!(t1 < 0.0)) continue;
+                double rho = sqrt(((p2(a0) + p2(a1)) + p2(a2)) - p2(t0));
+                if (!(rho < R)) continue;
+                /* tube.f90:819-892 */
+                t1 = ((X[0] - b.ci1[0]) * tci0[0] + (X[1] - b.ci1[1]) * tci0[1]) +
+                     (X[2] - b.ci1[2]) * tci0[2];
+                double zeta = (s1 * t0 - s0 * t1) / ndci0;
+                double zsec = zeta - sfirst;
+                double cz3[3], dc[3], ddc[3], cr[3], xc[3], T[3];
+                spline_pos(zsec * LD, coe, cz3);
+                spline_d1(zsec * LD, coe, dc);
+                double ndc = norm3(dc);
+                for (int c = 0; c < 3; c++) T[c] = dc[c] / ndc;
+                spline_d2(zsec * LD, coe, ddc);
+                cross3(dc, ddc, cr);
+                double ncr = norm3(cr);
+                cminus(X, cz3, xc);
+                double nxc = norm3(xc);
+                rho = nxc;
+                double sigma = or_sigma(zeta, len, smax, smin, n_T);
+                double tp = exp(-(p2(rho - 0.0) / 2.0 / p2(sigma))) / (PI2 * p2(sigma));
+                double ftd = -exp(-(p2(R) / 2.0 / p2(sigma))) / (PI2 * p2(sigma));
+                tp = tp + ftd;
+                /* dsigmadzeta (subroutine.f90:748-761) with the tube's n_T (T1) */
+                double sl = or_sigma(zeta - dzeta, len, smax, smin, n_T);
+                double sr = or_sigma(zeta + dzeta, len, smax, smin, n_T);
+                double dsds = (sr - sl) / (2.0 * dzeta);
+                dsds = dsds / ndc / LD;
+                int i = wrap_idx(ip, N), j = wrap_idx(jp, N);
+                int64_t idx = (int64_t)(i - 1) + (int64_t)N * (j - 1) + (int64_t)N * N * (k - 1);
+                double tg = tp * Gamma;
+                double v[3];
+                if (ncr < 1.0e-8) { /* tube.f90:854-867 */
+                    double er[3], eth[3];
+                    for (int c = 0; c < 3; c++) er[c] = xc[c] / (nxc + 1.0e-15);
+                    cross3(er, T, eth);
+                    for (int c = 0; c < 3; c++)
+                        v[c] = tg * ((T[c] + eta * rho * eth[c]) + (rho * dsds / sigma) * er[c]);
+                } else { /* tube.f90:868-892 */
+                    double B[3], Nv[3];
+                    for (int c = 0; c < 3; c++) B[c] = cr[c] / ncr;
+                    cross3(B, T, Nv);
+                    double kap = ncr / p3(ndc);
+                    double costh = dot3(xc, Nv) / (nxc + 1.0e-15);
+                    double sinth = dot3(xc, B) / (nxc + 1.0e-15);
+                    double D = 1.0 - kap * rho * costh;
+                    double t5 = -(sinth * rho / D);
+                    double t6 = costh * rho / D;
+                    double t7 = costh * rho * dsds / sigma / D;
+                    double t8 = sinth * rho * dsds / sigma / D;
+                    for (int c = 0; c < 3; c++)
+                        v[c] = tg * ((T[c] + eta * (t5 * Nv[c] + t6 * B[c])) + (t7 * Nv[c] + t8 * B[c]));
+                }
+                wx[idx] += v[0];
+                wy[idx] += v[1];
+                wz[idx] += v[2];
+                hits++;
+            }
+    }
+    return hits;
+}
+
+static int deposit_planes(const or_params *p, const double *sigma_line, const double *Gamma_line,
+                          const int32_t *n_T_line, const or_geom *g, int64_t seg_lo, int64_t seg_hi,
+                          int32_t kfirst, int32_t klast, double *omega, int64_t *hits_out) {
+    const int N = p->N;
     int l0 = p->line_s - 1, l1 = p->line_e - 1;
     if (l0 < 0) l0 = 0;
     if (l1 > g->nline - 1) l1 = g->nline - 1;
-    for (int l = l0; l <= l1; l++) {
-        double Gamma = Gamma_line[l];
-        double smin = sigma_line[l], smax = p->sigma_ratio * smin;
-        int n_T = n_T_line[l];
-        double LD = g->Len_Disc[l], len = g->length[l];
+    /* Fortran MOD of a negative index would address outside the field */
+    int bad = 0;
+    for (int l = l0; l <= l1 && !bad; l++) {
         int64_t sa = g->seg_base[l] > seg_lo ? g->seg_base[l] : seg_lo;
         int64_t sb = g->seg_base[l + 1] < seg_hi ? g->seg_base[l + 1] : seg_hi;
+#pragma omp parallel for reduction(| : bad)
         for (int64_t seg = sa; seg < sb; seg++) {
             or_bin_rec b;
-            bin_one(p, smin, n_T, g, l, seg, &b);
-            for (int c = 0; c < 3; c++) /* Fortran MOD of a negative index would address outside the field */
-                if (b.lo[c] < 1 - N || b.hi[c] > 2 * N)
-                    return fail("a tube's cut-off radius spans more than the periodic box");
-            int64_t sec = g->seg_sec[seg];
-            int64_t q = g->sp_base[sec] + g->seg_j[seg];
-            const double *coe = g->coe + 18 * sec;
-            double s0 = g->s[q], s1 = g->s[q + 1];
-            double sfirst = g->s[g->sp_base[sec]];
-            const double *tci0 = g->tci + 3 * q, *tci1 = g->tci + 3 * (q + 1);
-            double ndci0 = g->ndci[q];
-            double R = b.R;
-            double dzeta = p->dzeta_sup < g->dtf[sec] ? p->dzeta_sup : g->dtf[sec];
-            for (int32_t kp = b.lo[2]; kp <= b.hi[2]; kp++)
-                for (int32_t jp = b.lo[1]; jp <= b.hi[1]; jp++)
-                    for (int32_t ip = b.lo[0]; ip <= b.hi[0]; ip++) {
-                        int k = wrap_idx(kp, N);
-                        if (!(k > 0 && k <= N)) continue;
-                        double X[3] = {mesh(ip, dx, start), mesh(jp, dx, start), mesh(kp, dx, start)};
-                        double a0 = X[0] - b.ci0[0], a1 = X[1] - b.ci0[1], a2 = X[2] - b.ci0[2];
-                        double t0 = (a0 * tci0[0] + a1 * tci0[1]) + a2 * tci0[2];
-                        if (!(t0 >= 0.0)) continue;
-                        double t1 = ((X[0] - b.ci1[0]) * tci1[0] + (X[1] - b.ci1[1]) * tci1[1]) +
-                                    (X[2] - b.ci1[2]) * tci1[2];
-                        if (!(t1 < 0.0)) continue;
-                        double rho = sqrt(((p2(a0) + p2(a1)) + p2(a2)) - p2(t0));
-                        if (!(rho < R)) continue;
-                        /* tube.f90:819-892 */
-                        t1 = ((X[0] - b.ci1[0]) * tci0[0] + (X[1] - b.ci1[1]) * tci0[1]) +
-                             (X[2] - b.ci1[2]) * tci0[2];
-                        double zeta = (s1 * t0 - s0 * t1) / ndci0;
-                        double zsec = zeta - sfirst;
-                        double cz3[3], dc[3], ddc[3], cr[3], xc[3], T[3];
-                        spline_pos(zsec * LD, coe, cz3);
-                        spline_d1(zsec * LD, coe, dc);
-                        double ndc = norm3(dc);
-                        for (int c = 0; c < 3; c++) T[c] = dc[c] / ndc;
-                        spline_d2(zsec * LD, coe, ddc);
-                        cross3(dc, ddc, cr);
-                        double ncr = norm3(cr);
-                        cminus(X, cz3, xc);
-                        double nxc = norm3(xc);
-                        rho = nxc;
-                        double sigma = or_sigma(zeta, len, smax, smin, n_T);
-                        double tp = exp(-(p2(rho - 0.0) / 2.0 / p2(sigma))) / (PI2 * p2(sigma));
-                        double ftd = -exp(-(p2(R) / 2.0 / p2(sigma))) / (PI2 * p2(sigma));
-                        tp = tp + ftd;
-                        /* dsigmadzeta (subroutine.f90:748-761) with the tube's n_T (T1) */
-                        double sl = or_sigma(zeta - dzeta, len, smax, smin, n_T);
-                        double sr = or_sigma(zeta + dzeta, len, smax, smin, n_T);
-                        double dsds = (sr - sl) / (2.0 * dzeta);
-                        dsds = dsds / ndc / LD;
-                        int i = wrap_idx(ip, N), j = wrap_idx(jp, N);
-                        int64_t idx = (int64_t)(i - 1) + (int64_t)N * (j - 1) + (int64_t)N * N * (k - 1);
-                        double tg = tp * Gamma;
-                        double v[3];
-                        if (ncr < 1.0e-8) { /* tube.f90:854-867 */
-                            double er[3], eth[3];
-                            for (int c = 0; c < 3; c++) er[c] = xc[c] / (nxc + 1.0e-15);
-                            cross3(er, T, eth);
-                            for (int c = 0; c < 3; c++)
-                                v[c] = tg * ((T[c] + eta * rho * eth[c]) + (rho * dsds / sigma) * er[c]);
-                        } else { /* tube.f90:868-892 */
-                            double B[3], Nv[3];
-                            for (int c = 0; c < 3; c++) B[c] = cr[c] / ncr;
-                            cross3(B, T, Nv);
-                            double kap = ncr / p3(ndc);
-                            double costh = dot3(xc, Nv) / (nxc + 1.0e-15);
-                            double sinth = dot3(xc, B) / (nxc + 1.0e-15);
-                            double D = 1.0 - kap * rho * costh;
-                            double t5 = -(sinth * rho / D);
-                            double t6 = costh * rho / D;
-                            double t7 = costh * rho * dsds / sigma / D;
-                            double t8 = sinth * rho * dsds / sigma / D;
-                            for (int c = 0; c < 3; c++)
-                                v[c] = tg * ((T[c] + eta * (t5 * Nv[c] + t6 * B[c])) + (t7 * Nv[c] + t8 * B[c]));
-                        }
-                        wx[idx] += v[0];
-                        wy[idx] += v[1];
-                        wz[idx] += v[2];
-                        hits++;
-                    }
+            bin_one(p, sigma_line[l], n_T_line[l], g, l, seg, &b);
+            for (int c = 0; c < 3; c++)
+                if (b.lo[c] < 1 - N || b.hi[c] > 2 * N) bad = 1;
+        }
+    }
+    if (bad) return fail("a tube's cut-off radius spans more than the periodic box");
+    int64_t hits = 0;
+#pragma omp parallel reduction(+ : hits)
+    {
+        int t = 0, T = 1;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+        T = omp_get_num_threads();
+#endif
+        const int64_t K = (int64_t)klast - kfirst + 1;
+        const int32_t klo = kfirst + (int32_t)(K * t / T), khi = kfirst + (int32_t)(K * (t + 1) / T) - 1;
+        for (int l = l0; l <= l1; l++) {
+            const double smin = sigma_line[l], smax = p->sigma_ratio * smin;
+            int64_t sa = g->seg_base[l] > seg_lo ? g->seg_base[l] : seg_lo;
+            int64_t sb = g->seg_base[l + 1] < seg_hi ? g->seg_base[l + 1] : seg_hi;
+            if (klo <= khi)
+                for (int64_t seg = sa; seg < sb; seg++)
+                    hits += deposit_seg(p, g, l, seg, Gamma_line[l], smin, smax, n_T_line[l], omega, klo, khi);
         }
     }
     if (hits_out) *hits_out = hits;
     return 0;
+}
+
+int or_deposit_range(const or_params *p, const double *sigma_line, const double *Gamma_line,
+                     const int32_t *n_T_line, const or_geom *g, int64_t seg_lo, int64_t seg_hi,
+                     double *omega, int64_t *hits_out) {
+    return deposit_planes(p, sigma_line, Gamma_line, n_T_line, g, seg_lo, seg_hi, 1, p->N, omega, hits_out);
+}
+
+int or_deposit_planes(const or_params *p, const double *sigma_line, const double *Gamma_line,
+                      const int32_t *n_T_line, const or_geom *g, int32_t kfirst, int32_t klast,
+                      double *omega, int64_t *hits_out) {
+    if (kfirst < 1 || klast > p->N || kfirst > klast) return fail("plane range outside 1..N");
+    return deposit_planes(p, sigma_line, Gamma_line, n_T_line, g, 0, g->nseg_all, kfirst, klast, omega, hits_out);
 }
 
 int or_deposit(const or_params *p, const double *sigma_line, const double *Gamma_line,
@@ -591,12 +651,15 @@ int or_fft1d(int n, double *data, int forward) {
 }
 
 int or_fft3d(int N, double *data, int forward) {
-    double *line = (double *)malloc(sizeof(double) * 2 * N);
     const int64_t sx = 1, sy = N, sz = (int64_t)N * N;
     const int64_t strides[3] = {sx, sy, sz};
     for (int ax = 0; ax < 3; ax++) {
         int64_t st = strides[ax];
         int64_t o1 = strides[(ax + 1) % 3], o2 = strides[(ax + 2) % 3];
+#pragma omp parallel
+        {
+        double *line = (double *)malloc(sizeof(double) * 2 * N);
+#pragma omp for schedule(static)
         for (int a = 0; a < N; a++)
             for (int b = 0; b < N; b++) {
                 int64_t base = a * o1 + b * o2;
@@ -611,8 +674,9 @@ int or_fft3d(int N, double *data, int forward) {
                     data[2 * (base + i * st) + 1] = line[2 * i + 1];
                 }
             }
+        free(line);
+        }
     }
-    free(line);
     return 0;
 }
 

@@ -102,6 +102,14 @@ int or_deposit_range(const or_params *p, const double *sigma_line, const double 
                      const int32_t *n_T_line, const or_geom *g, int64_t seg_lo, int64_t seg_hi,
                      double *omega, int64_t *hits);
 
+/* or_deposit restricted to the grid planes k = kfirst..klast (1-based, wrapped): those planes
+ * receive exactly what or_deposit gives them, the others are left unchanged; hits counts the
+ * deposits on those planes. The cost scales with the planes' share of the boxes, so a slab of a
+ * large grid can be checked in seconds. */
+int or_deposit_planes(const or_params *p, const double *sigma_line, const double *Gamma_line,
+                      const int32_t *n_T_line, const or_geom *g, int32_t kfirst, int32_t klast,
+                      double *omega, int64_t *hits);
+
 /* Brute-force candidate test of one (cell, sub-segment) pair: returns 1 when the reference's
  * loop would deposit on extended cell (ip, jp, kp) from sub-segment seg (tube.f90:800-818). */
 int or_pair_hits(const or_params *p, const or_geom *g, const or_bin_rec *b, int64_t seg,

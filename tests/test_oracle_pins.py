@@ -302,3 +302,18 @@ def test_pipeline_normalises_to_unit_rms(oracle):
                            t.gamma, t.n_T)
     assert 0.5 * np.mean(np.sum(u ** 2, axis=0)) == pytest.approx(1.5, rel=1e-12)
     assert st[4] > 0
+
+
+def test_deposit_planes_equals_the_slice_of_the_full_deposit(oracle):
+    """or_deposit_planes gives the planes k0..k1 bitwise what or_deposit gives them (the z-slab
+    split of the threads keeps each cell's summation order)."""
+    p, t = fbb.config_small(N=32, seed=11, sigma_ratio=4.0, ntubes=(2, 3), npts=(40, 30), radii=(0.25, 0.12))
+    P = params(oracle, 32, t.nline, sigma_ratio=4.0, eta_t=0.7, ntf_pre=100)
+    geo = oracle.Geometry(P, t.npoint, t.cx, t.cy, t.cz)
+    full, hits = geo.deposit(t.sigma, t.gamma, t.n_T)
+    total = 0
+    for k0, k1 in ((1, 4), (5, 17), (18, 32)):
+        part, h = geo.deposit_planes(t.sigma, t.gamma, t.n_T, k0, k1)
+        np.testing.assert_array_equal(part, full[:, k0 - 1:k1])
+        total += h
+    assert total == hits
