@@ -61,76 +61,135 @@ def algorithmic_cost(slot: str, N: int, seg: dict, dep: dict, p):
         # t = 1/2 +- s: per pair s^2, the even and odd parts of the 4 shifted polynomials (25 FMA),
         # 4 products s*O and 8 sums (63 FLOP); per sample |w|^2 (1 mul + 2 FMA), sqrt, reciprocal,
         # 1 mul, the FP64 running sum and 3 bound updates (12 FLOP): 43.5 FLOP per sample
-        return seg["nsec_all"] * (p.ntf_pre + 1) * 43.5, "TFLOP/s", "mfma", PEAK_FP32_TFLOPS
+        return seg["nsec_all"] * (p.ntf_pre + 1) * 43.5, "TFLOP/s", "valu", PEAK_FP32_TFLOPS
     if slot == "deposit":
         # FP64 on the vector ALU (its peak equals the published FP64 peak): 16 FLOP per grid column
         # walked (two plane positions and the interval), 33 per candidate cell (the reference's
         # three tests), 150 per hit (zeta, spline point and tangent, offset, Gaussian core,
         # fixed-point conversion)
-        return dep["columns"] * 16 + dep["candidates"] * 33 + dep["hits"] * 150, "TFLOP/s", "mfma", PEAK_FP64_TFLOPS
+        return dep["columns"] * 16 + dep["candidates"] * 33 + dep["hits"] * 150, "TFLOP/s", "valu", PEAK_FP64_TFLOPS
     return None
 
 
-def cpu_baseline(p, t, budget_s: float):
-    """Time the FP64 oracle (single-threaded C restatement of the reference) on a bounded
-    sample of the same workload and extrapolate to the whole field."""
+def host_cpu() -> dict:
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def tube_subsegments(p, t, ntf) -> np.ndarray:
+    """Sub-segments of every tube: the sum of ntf over its real sections (its npoint sections;
+    the periodic ghost section deposits nothing)."""
+    sec_base = np.concatenate([[0], np.cumsum(t.npoint.astype(np.int64) + 1)])
+    tot = np.add.reduceat(ntf.astype(np.int64), sec_base[:-1])
+    return tot - ntf[sec_base[1:] - 1]
+
+
+def cpu_baseline(p, t, nseg_tube, budget_s: float):
+    """The reference's CPU path for the same workload, on every host core: geometry and deposition
+    by the C restatement of tube.f90 (threaded: geometry one tube per thread, deposition split into
+    z-slabs of planes the way the reference's MPI ranks split it, tube.f90:803-804), the 3-D FFTs of
+    the Biot-Savart stage by scipy.fft (pocketfft, FP64 c2c like the reference's FFTW plans) on
+    every core. Geometry and deposition run on a sample of each level's tubes (and of the grid
+    planes when a level's boxes are large) sized to the time budget and are scaled to the level by
+    its sub-segment (deposition) and point (geometry) counts; the FFTs run at the full size. Core
+    seconds are measured (process CPU time) and converted to wall time assuming the workload's
+    thousands of tubes and N z-planes keep every core busy."""
+    import scipy.fft as sfft
     from oracle import oracle_py as O
     from fastweavturb_amd.woven_io import Tubes
-    rng = np.random.default_rng(0)
+    cores = O.threads()
     N = p.N
-    # level boundaries by sigma
-    sig = t.sigma
-    levels = sorted(set(np.round(sig, 12)), reverse=True)
-    pt_base = np.concatenate([[0], np.cumsum(t.npoint)])
-    t_geo = 0.0
-    t_dep = 0.0
-    notes = []
-    per_level_budget = budget_s * 0.8 / max(1, len(levels))
+    dx = 2 * np.pi / N
+    rng = np.random.default_rng(0)
+    P = O.Params(N, N, 1, 1, p.sigma_ratio, p.eta_t, p.ntf_times, p.ntf_pre, p.Rtube_ratio, p.dzeta_sup)
+    sig = np.round(t.sigma, 12)
+    levels = sorted(set(sig.tolist()), reverse=True)
+    pt_base = np.concatenate([[0], np.cumsum(t.npoint.astype(np.int64))])
+    # predicted cost per tube: box cells walked (6 ns per cell and core) and turning-angle samples
+    # (25 ns per sample and core)
+    box = (2 * p.Rtube_ratio * t.sigma / dx + 3) ** 3
+    dep_s = nseg_tube * box * 6e-9 / cores
+    cost_s = dep_s + t.npoint * (p.ntf_pre + 1) * 25e-9 / cores
+    dep_budget = 0.7 * budget_s
+    full = cost_s.sum() <= dep_budget
+    geo_core = dep_core = 0.0
+    parts = []
     for lv in levels:
-        idx = np.flatnonzero(np.round(sig, 12) == lv)
-        tube = int(rng.choice(idx))
-        a, b = pt_base[tube], pt_base[tube + 1]
-        sub = Tubes(t.npoint[tube:tube + 1].copy(), t.cx[a:b].copy(), t.cy[a:b].copy(), t.cz[a:b].copy(),
-                    t.sigma[tube:tube + 1].copy(), t.gamma[tube:tube + 1].copy(), t.n_T[tube:tube + 1].copy(),
-                    t.if_filter[tube:tube + 1].copy())
-        P = O.Params(N, N, 1, 1, p.sigma_ratio, p.eta_t, p.ntf_times, p.ntf_pre, p.Rtube_ratio, p.dzeta_sup)
-        t0 = time.perf_counter()
+        idx = np.flatnonzero(sig == lv)
+        order = idx if full else rng.permutation(idx)
+        lvl_cost = float(cost_s[idx].sum())
+        share = dep_budget / len(levels)
+        if full:
+            take, frac = order, 1.0
+        else:
+            take, acc = [], 0.0
+            for i in order:
+                take.append(int(i))
+                acc += cost_s[i]
+                if acc >= share:
+                    break
+            take = np.array(take)
+            g_s, d_s = float((cost_s - dep_s)[take].sum()), float(dep_s[take].sum())
+            frac = min(1.0, max((share - g_s) / max(d_s, 1e-12), 0.02))
+        # grid planes: every 64/nplane-th block of 8 planes when only a fraction is needed
+        nblk = N // 8
+        step = max(1, int(np.ceil(1.0 / max(frac, 1e-6) - 1e-9)))
+        blocks = list(range(0, nblk, step))
+        planes = N if step == 1 else 8 * len(blocks)
+        sel = np.concatenate([np.arange(pt_base[i], pt_base[i + 1]) for i in take])
+        sub = Tubes(t.npoint[take].copy(), t.cx[sel].copy(), t.cy[sel].copy(), t.cz[sel].copy(), t.sigma[take].copy(),
+                    t.gamma[take].copy(), t.n_T[take].copy(), t.if_filter[take].copy())
+        P.line_e = len(take)
+        c0 = time.process_time()
         geo = O.Geometry(P, sub.npoint, sub.cx, sub.cy, sub.cz)
-        tg = time.perf_counter() - t0
-        level_points = int(t.npoint[idx].sum())
-        t_geo += tg * level_points / int(sub.npoint[0])
-        # deposit a growing chunk of sub-segments until the per-level budget is used
-        nseg = geo.nseg
-        chunk = 1
-        spent, done, secs = 0.0, 0, 0.0
-        while done < nseg and spent < per_level_budget:
-            hi = min(nseg, done + chunk)
-            dt, _ = O.deposit_range_timed(P, geo, sub.sigma, sub.gamma, sub.n_T, done, hi)
-            secs += dt
-            spent += dt
-            done = hi
-            chunk = min(chunk * 2, 1 << 20)
-        seg_per_point = nseg / int(sub.npoint[0])
-        t_dep += secs * (seg_per_point * level_points) / done
-        notes.append(f"r={lv * np.sqrt(2):.4g}: tube {tube}, {done}/{nseg} sub-segments")
+        cg = time.process_time() - c0
+        c0 = time.process_time()
+        if step == 1:
+            geo.deposit_planes(sub.sigma, sub.gamma, sub.n_T, 1, N)
+        else:
+            for bk in blocks:
+                geo.deposit_planes(sub.sigma, sub.gamma, sub.n_T, 8 * bk + 1, 8 * bk + 8)
+        cd = time.process_time() - c0
         geo.close()
-    # spectral part: 6 3-D c2c transforms of N^3 = 18 N^2 lines of length N, timed on a sample
-    line = np.random.default_rng(1).standard_normal(N) + 0j
-    nl = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < min(2.0, budget_s * 0.1):
-        O.fft1d(line, True)
-        nl += 1
-    t_line = (time.perf_counter() - t0) / nl
-    t_fft = 18 * N * N * t_line
+        seg_s = int(nseg_tube[take].sum())
+        geo_scale = float(t.npoint[idx].sum()) / float(sub.npoint.sum())
+        dep_scale = float(nseg_tube[idx].sum()) / seg_s * (N / planes)
+        geo_core += cg * geo_scale
+        dep_core += cd * dep_scale
+        parts.append(f"r={lv * np.sqrt(2):.4g}: {len(take)}/{len(idx)} tubes, {planes}/{N} planes")
+        log(f"[cpu] level r={lv * np.sqrt(2):.4g}: {len(take)}/{len(idx)} tubes, {planes}/{N} planes, "
+            f"geometry {cg:.1f} core-s (x{geo_scale:.3g}), deposition {cd:.1f} core-s (x{dep_scale:.3g}), "
+            f"predicted level {lvl_cost:.1f} s")
+    # spectral stage at full size: 3 forward and 3 inverse c2c transforms of N^3 FP64 fields
+    x = np.empty((N, N, N), dtype=np.complex128)
+    x.real = rng.standard_normal((N, N, N))
+    x.imag = 0.0
+    w0 = time.perf_counter()
+    for _ in range(3):
+        y = sfft.fftn(x, workers=cores)
+        x = sfft.ifftn(y, workers=cores)
+    t_fft = time.perf_counter() - w0
+    del x, y
+    t_geo, t_dep = geo_core / cores, dep_core / cores
     total = t_geo + t_dep + t_fft
-    sample = ("FP64 oracle, 1 thread: per level one random tube's geometry (turning angle, "
-              "splines, sub-points) and a leading run of its sub-segments deposited, scaled by "
-              "points/sub-segments to the whole level; 3-D FFTs from timed 1-D lines; "
-              + "; ".join(notes))
-    return {"value": N ** 3 / total / 1e6, "unit": "Mpts/s", "cores": 1, "kind": "port",
-            "sample": sample, "seconds_estimated": total,
-            "seconds_parts": {"geometry": t_geo, "deposition": t_dep, "fft": t_fft}}
+    sample = ("port on %d cores: tube.f90 geometry and deposition restated in C (threaded per tube / per "
+              "z-slab), %s; Biot-Savart FFTs by scipy.fft at the full %d^3 (6 c2c, measured); core seconds "
+              "scaled by sub-segment and point counts, divided by the cores. %s" % (
+                  cores, "every tube and plane measured" if full else "a seeded sample of each level",
+                  N, "; ".join(parts)))
+    return {"value": N ** 3 / total / 1e6, "unit": "Mpts/s", "cores": cores, "kind": "port",
+            "cpu": host_cpu(), "sample": sample, "seconds_estimated": total,
+            "seconds_parts": {"geometry": t_geo, "deposition": t_dep, "fft": t_fft},
+            "whole_workload_measured": bool(full),
+            "note": "the reference Fortran (tube.f90) needs FFTW3 and MPI, absent here; its C restatement is timed"}
 
 
 def time_to_target(n_targets: int, n_sweeps: int):
@@ -176,6 +235,10 @@ def time_to_target(n_targets: int, n_sweeps: int):
            "sweep_mean_runs": float(np.mean([r["sweep_runs"] for r in both])) if both else None,
            "speedup": (float(np.mean([r["sweep_s"] for r in both]) / np.mean([r["surrogate_s"] for r in both]))
                        if both else None),
+           # the network's training on the GPU, shared by the targets of this run
+           "speedup_incl_training": (float(np.mean([r["sweep_s"] for r in both]) /
+                                           (np.mean([r["surrogate_s"] for r in both]) + t_train / len(rows)))
+                                     if both else None),
            "training": {"pairs": int(len(X)), "train_s": t_train, "epochs": 300},
            "per_target": rows}
     return out
@@ -338,16 +401,18 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=["c2", "c1", "small", "c4", "c3"])
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-budget", type=float, default=30.0,
+                    help="seconds of host time the CPU baseline's sample may take")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ttt-targets", type=int, default=4,
                     help="targets of the time-to-target-spectrum measurement (0 skips it)")
-    ap.add_argument("--ttt-sweeps", type=int, default=1, help="targets also searched by the plain sweep")
+    ap.add_argument("--ttt-sweeps", type=int, default=-1,
+                    help="targets also searched by the plain sweep (-1: every target)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default FFT tile shapes")
     ap.add_argument("--tile-cache", default=os.path.join(ROOT, "fastweavturb_amd", "data", "fft_tiles.json"),
                     help="autotune cache (JSON): shapes recorded there are reused, new ones appended")
-    ap.add_argument("--fft-compare", action="store_true",
-                    help="also time rocFFT's transforms of the same Biot-Savart stage")
+    ap.add_argument("--no-fft-compare", action="store_true",
+                    help="skip timing rocFFT's transforms of the same Biot-Savart stage")
     ap.add_argument("--rocfft-only", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB -> 1 GiB device copy kernel (calibrates the "
@@ -370,11 +435,12 @@ def main():
         print(json.dumps(ensemble_bench(args.steps, args.warmup, tile_cache=args.tile_cache)), flush=True)
         return
     fftc = None
-    if args.fft_compare:
+    if not args.no_fft_compare:
         # torch carries its own HIP runtime: it runs in a child process started before this
         # process touches the GPU
         import subprocess
-        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--rocfft-only", "512"],
+        nfft = {"c2": 512, "c1": 128, "small": 64}[args.config]
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--rocfft-only", str(nfft)],
                            capture_output=True, text=True, timeout=600)
         if r.returncode != 0:
             log(f"[bench] rocFFT comparison failed: {r.stderr[-2000:]}")
@@ -407,6 +473,9 @@ def main():
     kernels = {SLOT_NAMES[i]: float(times[4 + i]) for i in range(len(SLOT_NAMES)) if times[4 + i] > 0}
     seg = g.segment_stats()
     dep = g.deposit_stats()
+    ntf, _ = g.sections()
+    nseg_tube = tube_subsegments(p, t, ntf)
+    del ntf
     log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step; stages {np.round(stage_ms / args.steps, 2).tolist()}")
     log(f"[bench] kernels ms {json.dumps({k: round(v, 3) for k, v in kernels.items()})}")
     log(f"[bench] geometry {json.dumps(seg)}; deposit {json.dumps(dep)}")
@@ -422,6 +491,13 @@ def main():
         roof = {"kernel": dom, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
                 "frac": achieved / peak, "traffic": measured_traffic(wname, dom), "ms": kernels[dom],
                 "resource": resource}
+    # the path's own bound: every byte it must move once at 8 TB/s (omega written, the 6 FFT passes
+    # of the half spectra, centerline points read, sub-segment records written and read once)
+    b_w = p.N ** 3 * (12 + 144) + 24 * int(t.npoint_all) + 128 * int(seg["nseg_all"])
+    t_roof = b_w / (PEAK_HBM_GBS * 1e9)
+    headline = {"bytes": b_w, "t_roof_ms": t_roof * 1e3, "t_step_ms": ms_per_step,
+                "frac": t_roof / (ms_per_step * 1e-3),
+                "model": "B_W = N^3 (12 + 144) + 24 P + 128 S_sub bytes at 8 TB/s against the whole step"}
     if fftc is not None:
         spectral = [k for k in ("fft_x", "fft_y", "fft_z_biot_savart", "fft_y_inv", "fft_x_inv", "spectral_reduce")
                     if k in kernels]
@@ -429,13 +505,13 @@ def main():
         fftc["ours_kernels_ms"] = float(sum(kernels[k] for k in spectral))
         log(f"[bench] fft compare {json.dumps(fftc)}")
     ttt = None
+    g.close()
     if args.ttt_targets > 0 and args.config == "c2":
-        g.close()
-        ttt = time_to_target(args.ttt_targets, args.ttt_sweeps)
+        ttt = time_to_target(args.ttt_targets, args.ttt_targets if args.ttt_sweeps < 0 else args.ttt_sweeps)
     cpu = None
     if not args.no_cpu_baseline:
         t0 = time.time()
-        cpu = cpu_baseline(p, t, args.cpu_budget)
+        cpu = cpu_baseline(p, t, nseg_tube, args.cpu_budget)
         log(f"[bench] cpu baseline {cpu['value']:.4g} Mpts/s (sampled in {time.time() - t0:.1f}s)")
     out = {
         "metric": METRIC, "value": value, "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps,
@@ -444,7 +520,8 @@ def main():
         "config": {"workload": wname, "N": p.N, "tubes": t.nline, "points": int(t.npoint_all),
                    "sub_segments": int(seg["nseg_all"]), "levels": 6 if args.config == "c2" else None},
         "stages_ms": dict(zip(["segments", "bin", "deposit", "biot_savart"], (stage_ms / args.steps).tolist())),
-        "kernels_ms": kernels, "roofline": roof, "cpu_baseline": cpu, "time_to_target": ttt,
+        "kernels_ms": kernels, "roofline": roof, "headline_roofline": headline, "cpu_baseline": cpu,
+        "time_to_target": ttt,
         "fft_vs_rocfft": fftc,
     }
     print(json.dumps(out), flush=True)

@@ -39,9 +39,8 @@ struct DepArgs {
     Run r;                        // instance 0 (only used for the Run argument of subseg_load)
     Bins b;
     void *omega;                  // double* (FP64 mode) or float* (FP32 mode)
-    double fx_scale;              // 2^shift (FP32 mode)
-    double fx_limit;              // |value * scale| bound before overflow
-    double inv_scale;             // 2^-shift
+    double fx_limit;              // |value * scale| bound before overflow (FP32 mode; the scale
+                                  // is the brick's instance's Run::fx_scale)
     unsigned long long *counters; // [0] columns [1] candidates [2] hits [3] overflow
     unsigned int *next;           // brick queue cursor
     int accumulate;               // 0: the brick's cells are written; 1: added to omega
@@ -574,7 +573,7 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
             } else {
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    double s = v[c] * W.A->fx_scale;
+                    double s = v[c] * W.ri->fx_scale;
                     if (!(fabs(s) < W.A->fx_limit)) { ovf = true; s = 0.0; }
                     long long fx = (long long)rint(s);
                     atomicAdd((unsigned long long *)W.acc + c * kCells + cell, (unsigned long long)fx);
@@ -848,7 +847,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                         *o = A.accumulate ? *o + v : v;
                     } else {
                         float *o = (float *)A.omega + idx + c * N3;
-                        const double v = (double)(long long)acc[c * kCells + i] * A.inv_scale;
+                        const double v = (double)(long long)acc[c * kCells + i] * ri.fx_inv;
                         *o = (float)(A.accumulate ? (double)*o + v : v);
                     }
                 }
@@ -904,8 +903,6 @@ hipError_t launch_deposit(wt_ctx *c, bool accumulate) {
     A.next = (unsigned int *)(cnt + 4);
     A.accumulate = accumulate ? 1 : 0;
     if (!f64) {
-        A.fx_scale = ldexp(1.0, c->fixed_shift);
-        A.inv_scale = ldexp(1.0, -c->fixed_shift);
         A.fx_limit = ldexp(1.0, 62) / 64.0;
     }
     bool fast = !f64;   // FP32 core only for constant core radius and no twist in every instance

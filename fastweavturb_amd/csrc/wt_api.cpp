@@ -317,7 +317,7 @@ static int upload_tubes(wt_ctx *c) {
     xs.reserve(P);
     ys.reserve(P);
     zs.reserve(P);
-    double bound = 0.0;
+    std::vector<double> bound(ni, 0.0);
     for (int i = 0; i < ni; i++) {
         const wt::Instance &I = c->inst[i];
         for (int l = 0; l < I.ntube; l++) {
@@ -326,7 +326,7 @@ static int upload_tubes(wt_ctx *c) {
             sig.push_back(I.sigma[l]);
             gam.push_back(I.gamma[l]);
             tinst.push_back(i);
-            bound += std::fabs(I.gamma[l]) / (wt::kPi2 * I.sigma[l] * I.sigma[l]);
+            bound[i] += std::fabs(I.gamma[l]) / (wt::kPi2 * I.sigma[l] * I.sigma[l]);
         }
         xs.insert(xs.end(), I.cx.begin(), I.cx.end());
         ys.insert(ys.end(), I.cy.begin(), I.cy.end());
@@ -366,8 +366,17 @@ static int upload_tubes(wt_ctx *c) {
         for (int a = 0; a < 3; a++) r.nb[a] = (I.N + wt::brick_dim(a) - 1) / wt::brick_dim(a);
         r.brick0 = brick0;
         brick0 += (int64_t)r.nb[0] * r.nb[1] * r.nb[2];
+        // fixed-point scale of the FP32 accumulation: |omega| <= sum Gamma/(2 pi sigma^2) over the
+        // instance's tubes times a generous factor for the curvature and twist terms
+        const double B = bound[i] * 64.0 + 1.0;
+        const int shift = (int)std::floor(60.0 - std::log2(B));
+        r.fx_shift = shift < 8 ? 8 : (shift > 44 ? 44 : shift);
+        r.fx_scale = std::ldexp(1.0, r.fx_shift);
+        r.fx_inv = std::ldexp(1.0, -r.fx_shift);
     }
     c->run = c->runs_host[0];
+    c->fixed_shift = c->runs_host[0].fx_shift;
+    for (const wt::Run &r : c->runs_host) c->fixed_shift = std::min(c->fixed_shift, r.fx_shift);
     free_list(c->bin_allocs);
     free_list(c->seg_allocs);
     free_list(c->geo_allocs);
@@ -433,11 +442,6 @@ static int upload_tubes(wt_ctx *c) {
     c->geo = g;
     c->pt_base = pt;
     c->sec_base = sec;
-    // fixed-point scale for the FP32 accumulation: |omega| <= sum Gamma/(2 pi sigma^2) times a
-    // generous factor for the curvature and twist terms
-    double B = bound * 64.0 + 1.0;
-    int shift = (int)std::floor(60.0 - std::log2(B));
-    c->fixed_shift = shift < 8 ? 8 : (shift > 44 ? 44 : shift);
     c->tubes_dirty = false;
     return WT_OK;
 }

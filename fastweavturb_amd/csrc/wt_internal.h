@@ -31,6 +31,8 @@ struct Run {
     int64_t field_off;          // element offset of the instance's 3*N^3 field block
     int64_t brick0;             // first global brick of the instance
     int32_t nb[3];              // bricks per axis, ceil(N / kBrickDim[axis])
+    int32_t fx_shift;           // FP32 mode: the instance's accumulator scale 2^fx_shift, from its own
+    double fx_scale, fx_inv;    // tubes only, so a batched instance rounds like a single run
 };
 
 // The deposition accumulates one brick of kBrickX * kBrickY * kBrickZ cells at a time in LDS.

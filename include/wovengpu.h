@@ -69,7 +69,8 @@ typedef struct {
     int64_t columns;      /* grid columns scanned                                */
     int64_t candidates;   /* cells tested with the reference predicate          */
     int64_t hits;         /* (cell, sub-segment) pairs that deposited            */
-    int32_t fixed_point_shift; /* FP32 mode: accumulator scale 2^shift; 0 in FP64 */
+    int32_t fixed_point_shift; /* FP32 mode: accumulator scale 2^shift (the smallest over the
+                                  instances of a batch, each scaled from its own tubes); 0 in FP64 */
     int32_t overflow;     /* nonzero if a fixed-point contribution overflowed    */
     int64_t pairs;        /* (sub-segment, brick) work pairs of the binning      */
 } wt_deposit_stats;

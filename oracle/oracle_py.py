@@ -73,6 +73,11 @@ def lib():
     return _lib
 
 
+def threads() -> int:
+    """Threads of the oracle's parallel loops (geometry per tube, deposition per z-slab, FFT lines)."""
+    return int(lib().or_threads())
+
+
 def _check(rc):
     if rc < 0:
         raise RuntimeError("oracle: " + lib().or_last_error().decode())

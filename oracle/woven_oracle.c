@@ -42,6 +42,14 @@
 
 static char g_err[512];
 const char *or_last_error(void) { return g_err; }
+
+int or_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 static int fail(const char *msg) {
     snprintf(g_err, sizeof g_err, "%s", msg);
     return -1;

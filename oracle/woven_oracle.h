@@ -78,6 +78,8 @@ typedef struct {
 } or_bin_rec;
 
 const char *or_last_error(void);
+/* Threads the oracle's parallel loops use (OMP_NUM_THREADS, else every core). */
+int or_threads(void);
 
 /* tube.f90:392-515 */
 int or_preprocess(const or_params *p, int nline, const int32_t *npoint, const double *cx,
