@@ -90,6 +90,7 @@ def load_library():
     L.wt_set_field.argtypes = [vp, i32, vp, i32]
     L.wt_get_sections.argtypes = [vp, ip, dp]
     L.wt_get_subpoints.argtypes = [vp, dp, dp]
+    L.wt_get_tube_lengths.argtypes = [vp, dp]
     L.wt_get_bins.argtypes = [vp, ip, dp]
     L.wt_stage_times.argtypes = [vp, dp, i32]
     L.wt_create_batch.argtypes = [C.POINTER(vp), i32, ip, i32, i32]
@@ -147,6 +148,7 @@ class WovenGPU:
         self.precision = precision
         self.dtype = np.float64 if precision == 64 else np.float32
         self.nline = 0
+        self._ntubes = {}
         self.instance = 0
 
     def select(self, inst: int):
@@ -207,6 +209,7 @@ class WovenGPU:
         self._check(self._L.wt_set_tubes(self._h, len(npt), _ip(npt), _dp(a[0]), _dp(a[1]), _ip(nT),
                                          _ip(filt), _dp(a[2]), _dp(a[3]), _dp(a[4])))
         self.nline = len(npt)
+        self._ntubes[self.instance] = len(npt)
 
     # --- stages --------------------------------------------------------------------------
     def build_segments(self) -> dict:
@@ -329,6 +332,12 @@ class WovenGPU:
         s = np.zeros(n)
         self._check(self._L.wt_get_subpoints(self._h, _dp(ci), _dp(s)))
         return ci, s
+
+    def tube_lengths(self) -> np.ndarray:
+        """2 pi Len_Disc of every tube of the selected instance (tube.f90:501)."""
+        out = np.zeros(self._ntubes.get(self.instance, 0))
+        self._check(self._L.wt_get_tube_lengths(self._h, _dp(out)))
+        return out
 
     def bins(self):
         n = self.segment_stats()["nsub_all"]

@@ -979,6 +979,17 @@ int wt_get_sections(wt_ctx *c, int32_t *ntf, double *dtf) {
     return WT_OK;
 }
 
+int wt_get_tube_lengths(wt_ctx *c, double *length) {
+    CHECK_CTX(c);
+    if (!c->have_segments) return set_err(c, WT_ESTATE, "no segments built");
+    if (!length) return set_err(c, WT_EINVAL, "length is NULL");
+    const wt::Instance &I = c->inst[c->cur];
+    HIPCALL(c, hipMemcpyAsync(length, c->geo.length + I.tube0, sizeof(double) * I.ntube, hipMemcpyDeviceToHost,
+                              c->stream), "copy");
+    HIPCALL(c, hipStreamSynchronize(c->stream), "copy");
+    return WT_OK;
+}
+
 int wt_get_subpoints(wt_ctx *c, double *ci_xyz, double *s) {
     CHECK_CTX(c);
     if (!c->have_segments) return set_err(c, WT_ESTATE, "no segments built");

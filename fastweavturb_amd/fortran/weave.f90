@@ -31,6 +31,10 @@ program weave_host
    type(wt_seg_stats) :: st
    type(c_ptr) :: ctx
    real(c_double) :: etot, up, enstrophy, lengthtot, dr
+   real(c_double) :: ms(32)
+   real(c_double), allocatable :: tube_len(:)
+   integer(8) :: clk_rate, clk_total0, clk0, clk1
+   real(8) :: t_init, t_total, t_post, t_spline, t_constr, share
    integer(c_int) :: rc
    integer(c_int64_t) :: sec0
    character(len=32) :: arg
@@ -129,9 +133,13 @@ program weave_host
    ! ---- GPU path ----
    rc = wt_create(ctx, int(nx, c_int32_t), int(prec, c_int32_t), 0_c_int32_t)
    call wt_check(c_null_ptr, rc, 'wt_create')
+   call system_clock(count_rate=clk_rate)
+   call system_clock(clk_total0)
    call wt_check(ctx, wt_set_params(ctx, p), 'wt_set_params')
    call wt_check(ctx, wt_set_tubes(ctx, int(nline, c_int32_t), npointlist, sigma_line, Gamma_line, &
                                    n_T_line, if_filter_line, cxall, cyall, czall), 'wt_set_tubes')
+   call system_clock(clk1)
+   t_init = real(clk1 - clk_total0, 8)/real(clk_rate, 8)
    call wt_check(ctx, wt_build_segments(ctx, st), 'wt_build_segments')
 
    ! test_data.dat: per tube, integer ntf_average over its n_sec+1 sections and ntf_max
@@ -150,6 +158,16 @@ program weave_host
 
    call wt_check(ctx, wt_bin(ctx), 'wt_bin')
    call wt_check(ctx, wt_deposit(ctx, c_null_ptr), 'wt_deposit')
+   call system_clock(clk1)
+   t_total = real(clk1 - clk_total0, 8)/real(clk_rate, 8)
+   rc = wt_stage_times(ctx, ms, 32_c_int32_t)
+   t_spline = ms(1)*1.0d-3
+   t_constr = (ms(2) + ms(3))*1.0d-3
+   allocate (tube_len(nline))
+   call wt_check(ctx, wt_get_tube_lengths(ctx, tube_len), 'wt_get_tube_lengths')
+   lengthtot = sum(tube_len)                        ! every tube, as tube.f90:502
+
+   call system_clock(clk0)
    call wt_check(ctx, wt_biot_savart(ctx, etot, up, enstrophy), 'wt_biot_savart')
 
    nek = wt_nek(int(nx, c_int32_t))
@@ -223,14 +241,40 @@ program weave_host
    write (99) data_box
    close (99)
 
-   lengthtot = 0
+   call system_clock(clk1)
+   t_post = real(clk1 - clk0, 8)/real(clk_rate, 8)
+
+   ! info.dat in the reference's layout (tube.f90:519-525, 532, 756-757, 941-952, 983, 1022,
+   ! 1033-1034, 1193-1207). All tubes go through the GPU in single launches, so a tube's spline and
+   ! construction times are the launches' times (segment stage; binning + deposition) shared out by
+   ! its section count. "Initialization time" is the upload of the tube table; the preprocessing
+   ! kernels (tube.f90:392-515) run inside the segment stage.
    open (192, file='./output/info.dat', status='replace')
+   write (192, *) 'Initialization time is', t_init
+   write (192, *) '=============================='
+   write (192, *) '   [i] [Section count] [Spline time] [Construction time]'
+   do i_line = p%line_s, p%line_e
+      share = real(npointlist(i_line), 8)/real(sum(npointlist(p%line_s:p%line_e)), 8)
+      write (192, '(I6)', advance='no') i_line
+      write (192, '(I16)', advance='no') npointlist(i_line)
+      write (192, '(ES18.3)', advance='no') t_spline*share
+      write (192, '(ES18.3)', advance='no') t_constr*share
+      write (192, *)
+   end do
+   write (192, *) '=============================='
+   write (192, *) 'Total construction time =', t_total
+   write (192, *) 'velocity calculation begin'
+   write (192, *) 'velocity calculation done'
+   write (192, *) 'Etot_temp =', etot
+   write (192, *) 'up_temp =', up
+   write (192, *) 'Post processing time =', t_post
+   write (192, *) '=============================='
    write (192, *) 'N = ', nx
+   write (192, *) 'Total length of curve = ', lengthtot
    write (192, *) 'Gamma_1 = ', Gamma_line(1)/up
    write (192, *) 'twist eta = ', p%eta_t
    write (192, *) 'Etot = ', etot/up/up
    write (192, *) 'Enstrophy = ', enstrophy
-   write (192, *) 'sub-segments = ', st%nseg_all
    close (192)
    write (*, *) 'up =', up, ' Etot =', etot, ' Enstrophy =', enstrophy
    call wt_destroy(ctx)

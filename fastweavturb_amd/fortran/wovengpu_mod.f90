@@ -111,6 +111,17 @@ module wovengpu
          integer(c_int32_t), intent(out) :: ntf(*)
          real(c_double), intent(out) :: dtf(*)
       end function
+      integer(c_int) function wt_get_tube_lengths(ctx, length) bind(C, name="wt_get_tube_lengths")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: length(*)
+      end function
+      integer(c_int) function wt_stage_times(ctx, ms, cap) bind(C, name="wt_stage_times")
+         import :: c_ptr, c_int, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: ms(*)
+         integer(c_int32_t), value :: cap
+      end function
    end interface
 
 contains

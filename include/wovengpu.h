@@ -210,6 +210,9 @@ int wt_set_field(wt_ctx *ctx, int32_t which, const void *host, int32_t host_is_f
  * start (bounding box lo[3], hi[3] and Rtube; entries that start no sub-segment get R = -1). */
 int wt_get_sections(wt_ctx *ctx, int32_t *ntf, double *dtf);
 int wt_get_subpoints(wt_ctx *ctx, double *ci_xyz, double *s);
+/* Length 2 pi Len_Disc of every tube of the selected instance, in tube order (tube.f90:501-502);
+ * their sum is info.dat's "Total length of curve" (tube.f90:502, 1199). After segments are built. */
+int wt_get_tube_lengths(wt_ctx *ctx, double *length);
 int wt_get_bins(wt_ctx *ctx, int32_t *lohi, double *R);
 
 /* Timings in milliseconds, measured with HIP events on the context stream. [0..3]: stages of

@@ -59,6 +59,40 @@ def test_fortran_host_outputs_match_oracle(oracle, tmp_path, precision):
     assert td.count("ntf_max=") == t.nline
     pout = open(tmp_path / "output/parameters_output.dat").read().split()
     assert pout[1] == "16"
+    # spectrum of the vorticity before normalisation (tube.f90:972-978): omega_raw = up * omega
+    Ew = np.loadtxt(tmp_path / "output/stat/spectrum_vor_temp.dat")
+    Ewref = oracle.spectrum(om) * st[1] ** 2
+    big = Ewref > 1e-6 * Ewref.max()
+    np.testing.assert_allclose(Ew[big, 1], Ewref[big], rtol=2e-5)
+    check_info(tmp_path, oracle, P, t2, st)
+
+
+def check_info(workdir, oracle, P, t, st):
+    """info.dat in the reference's layout (tube.f90:519-525, 941-952, 1033-1034, 1193-1207)."""
+    lines = open(workdir / "output/info.dat").read().splitlines()
+    sep = "=============================="
+    keys = [ln.strip() if ln.strip() == sep else (ln.split("=")[0].strip() if "=" in ln else ln.strip())
+            for ln in lines]
+    order = ["Initialization time is", "==============================", "[i] [Section count] [Spline time] [Construction time]"]
+    assert lines[0].strip().startswith(order[0]) and keys[1] == order[1] and keys[2] == order[2]
+    ntube = P.line_e - P.line_s + 1
+    rows = [ln.split() for ln in lines[3:3 + ntube]]
+    assert [int(r[0]) for r in rows] == list(range(P.line_s, P.line_e + 1))
+    assert [int(r[1]) for r in rows] == [int(n) for n in t.npoint[P.line_s - 1:P.line_e]]
+    tail = keys[3 + ntube:]
+    assert tail == ["==============================", "Total construction time", "velocity calculation begin",
+                    "velocity calculation done", "Etot_temp", "up_temp", "Post processing time",
+                    "==============================", "N", "Total length of curve", "Gamma_1", "twist eta", "Etot",
+                    "Enstrophy"]
+    val = {k: float(ln.split("=")[1]) for k, ln in zip(keys, lines) if "=" in ln and k != sep}
+    geo = oracle.Geometry(P, t.npoint, t.cx, t.cy, t.cz)
+    assert val["N"] == P.N
+    assert val["Total length of curve"] == pytest.approx(float(geo.length.sum()), rel=1e-12)
+    assert val["Etot_temp"] == pytest.approx(st[0], rel=1e-4)
+    assert val["up_temp"] == pytest.approx(st[1], rel=1e-4)
+    assert val["Gamma_1"] == pytest.approx(t.gamma[0] / st[1], rel=1e-4)
+    assert val["Etot"] == pytest.approx(1.5, rel=1e-5)
+    assert val["Enstrophy"] == pytest.approx(st[2], rel=1e-4)
 
 
 def test_fortran_host_on_reference_wt2_inputs(tmp_path):
