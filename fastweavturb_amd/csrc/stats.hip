@@ -8,6 +8,7 @@
 // partial row each and the rows are reduced in a fixed order, so results are reproducible.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "wt_internal.h"
@@ -24,16 +25,16 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Adds the block-wide sum of acc[0..np) into lds_acc[0..np) (called by all threads).
-__device__ __forceinline__ void block_accumulate(const double *acc, int np, double *lds_acc, double *scratch) {
+__device__ __forceinline__ void block_accumulate(const double *acc, int np, double *lds_acc, double *red) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int p = 0; p < np; p++) {
         double v = wave_sum(acc[p]);
-        if (lane == 0) scratch[wv * kMaxOrder + p] = v;
+        if (lane == 0) red[wv * kMaxOrder + p] = v;
     }
     __syncthreads();
     if (threadIdx.x < np) {
         double s = 0.0;
-        for (int w = 0; w < kSfThreads / 64; w++) s += scratch[w * kMaxOrder + threadIdx.x];
+        for (int w = 0; w < kSfThreads / 64; w++) s += red[w * kMaxOrder + threadIdx.x];
         lds_acc[threadIdx.x] += s;
     }
     __syncthreads();
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(kSfThreads) void k_sf_long(const T *__restrict__ u,
                                                         const int32_t *__restrict__ seps, double *__restrict__ partials) {
     extern __shared__ unsigned char smem_raw[];
     double *lds_acc = (double *)smem_raw;                       // nsep * np
-    double *scratch = lds_acc + (size_t)nsep * np;
+    double *red = lds_acc + (size_t)nsep * np;
     for (int i = threadIdx.x; i < nsep * np; i += blockDim.x) lds_acc[i] = 0.0;
     __syncthreads();
     const int64_t N2 = (int64_t)N * N, N3 = N2 * N;
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(kSfThreads) void k_sf_long(const T *__restrict__ u,
                     acc[p] += (px + py) + pz;
                 }
             }
-            block_accumulate(acc, np, lds_acc + (size_t)m * np, scratch);
+            block_accumulate(acc, np, lds_acc + (size_t)m * np, red);
         }
     }
     double *row = partials + (int64_t)blockIdx.x * nsep * np;
@@ -181,28 +182,57 @@ __global__ void k_reduce_rows(const double *__restrict__ partials, int64_t rows,
     out[col] = s * scale;
 }
 
+// Same sums as k_reduce_rows (rows added in order) written in the output layout: partial column
+// m * np + p goes to out[p * nsep + m].
+__global__ void k_reduce_rows_sf(const double *__restrict__ partials, int64_t rows, int nsep, int np, double scale,
+                                 double *__restrict__ out) {
+    const int cols = nsep * np;
+    int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= cols) return;
+    double s = 0.0;
+    for (int64_t r = 0; r < rows; r++) s += partials[r * cols + col];
+    const int m = col / np, p = col - m * np;
+    out[(int64_t)p * nsep + m] = s * scale;
+}
+
+__global__ void k_fill_zero(double *__restrict__ a, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = 0.0;
+}
+
+// Partial rows (blocks) of the structure-function launch of an N^3 field.
+static int64_t sf_rows(const wt_ctx *c, int mode, int N) {
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) cus = prop.multiProcessorCount;
+    if (mode == 1) return (int64_t)cus * 2;
+    const int rpg = kSfThreads * 8 / N > 0 ? kSfThreads * 8 / N : 1;
+    const int64_t ngroups = ((int64_t)N * N + rpg - 1) / rpg;
+    return ngroups < (int64_t)cus * 4 ? ngroups : (int64_t)cus * 4;
+}
+
+// Structure functions of instance c->cur into d_out[(p - pmin) * nsep + m] (device memory, the
+// caller's), with `partials` (device, rows * nsep * np doubles, rows = sf_rows) as the per-block
+// partial sums. Stream-ordered on the context stream; allocates nothing.
 template <typename T>
-static hipError_t sf_t(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const int32_t *d_seps, double *d_out) {
+static hipError_t sf_core(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const int32_t *d_seps, double *partials,
+                          double *d_out) {
     hipStream_t st = c->stream;
     const wt::Instance &I = c->inst[c->cur];
     const int N = I.N;
     const int np = pmax - pmin + 1;
-    int cus = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) cus = prop.multiProcessorCount;
     int dev_lds = 65536;
     (void)hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
     const T *u = (const T *)c->u + I.field_off;
     size_t acc_bytes = sizeof(double) * ((size_t)nsep * np + 4 * kMaxOrder);
-    int64_t blocks;
-    double *partials = nullptr;
+    const int64_t blocks = sf_rows(c, mode, N);
+    const int cols = nsep * np;
     if (mode == 1) {
-        // separations in chunks of at most 1024, one thread per separation (and point phase)
+        // separations in chunks of at most 1024, one thread per separation (and point phase); rows
+        // of blocks beyond a chunk's group count stay zero
+        hipLaunchKernelGGL(k_fill_zero, dim3(256), dim3(256), 0, st, partials, blocks * cols);
+        WT_HIP(hipGetLastError());
         const int64_t nlines = (int64_t)N * N;
-        blocks = (int64_t)cus * 2;
-        WT_HIP(hipMallocAsync((void **)&partials, sizeof(double) * blocks * nsep * np, st));
-        // rows of blocks beyond a chunk's group count stay zero
-        WT_HIP(hipMemsetAsync(partials, 0, sizeof(double) * blocks * nsep * np, st));
         for (int s0 = 0; s0 < nsep; s0 += 1024) {
             const int nc = nsep - s0 < 1024 ? nsep - s0 : 1024;
             const int threads = nc <= 256 ? 256 : (nc + 63) / 64 * 64;
@@ -221,34 +251,73 @@ static hipError_t sf_t(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const 
         }
     } else {
         if (acc_bytes > (size_t)dev_lds) return hipErrorInvalidValue;
-        int rpg = kSfThreads * 8 / N > 0 ? kSfThreads * 8 / N : 1;
-        int64_t ngroups = ((int64_t)N * N + rpg - 1) / rpg;
-        blocks = ngroups < (int64_t)cus * 4 ? ngroups : (int64_t)cus * 4;
-        WT_HIP(hipMallocAsync((void **)&partials, sizeof(double) * blocks * nsep * np, st));
         WT_HIP(hipFuncSetAttribute((const void *)k_sf_long<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)acc_bytes));
         hipLaunchKernelGGL(k_sf_long<T>, dim3((unsigned)blocks), dim3(kSfThreads), acc_bytes, st, u, N, np, nsep,
                            d_seps, partials);
     }
     WT_HIP(hipGetLastError());
-    double N3 = (double)N * N * N;
-    double scale = mode == 0 ? 1.0 / (3.0 * N3) : 1.0 / N3;
-    int cols = nsep * np;
-    // output layout S[(p - pmin) * nsep + m]; partial columns are m * np + p
-    double *tmp = nullptr;
-    WT_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * cols, st));
-    hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)((cols + 63) / 64)), dim3(64), 0, st, (const double *)partials,
-                       blocks, cols, scale, tmp);
-    WT_HIP(hipGetLastError());
-    std::vector<double> h(cols), o(cols);
-    WT_HIP(hipMemcpyAsync(h.data(), tmp, sizeof(double) * cols, hipMemcpyDeviceToHost, st));
-    WT_HIP(hipStreamSynchronize(st));
-    for (int m = 0; m < nsep; m++)
-        for (int p = 0; p < np; p++) o[(size_t)p * nsep + m] = h[(size_t)m * np + p];
-    WT_HIP(hipMemcpyAsync(d_out, o.data(), sizeof(double) * cols, hipMemcpyHostToDevice, st));
-    WT_HIP(hipFreeAsync(tmp, st));
-    WT_HIP(hipFreeAsync(partials, st));
-    return hipStreamSynchronize(st);
+    const double N3 = (double)N * N * N;
+    const double scale = mode == 0 ? 1.0 / (3.0 * N3) : 1.0 / N3;
+    hipLaunchKernelGGL(k_reduce_rows_sf, dim3((unsigned)((cols + 63) / 64)), dim3(64), 0, st, (const double *)partials,
+                       blocks, nsep, np, scale, d_out);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t sf_t(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const int32_t *d_seps, double *d_out) {
+    hipStream_t st = c->stream;
+    const int64_t rows = sf_rows(c, mode, c->inst[c->cur].N);
+    double *partials = nullptr;
+    WT_HIP(hipMallocAsync((void **)&partials, sizeof(double) * rows * nsep * (pmax - pmin + 1), st));
+    hipError_t e = sf_core<T>(c, mode, pmin, pmax, nsep, d_seps, partials, d_out);
+    const hipError_t f = hipFreeAsync(partials, st);
+    if (e == hipSuccess) e = f;
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
+}
+
+// ---- allocation-free variant -----------------------------------------------------------------
+// The per-block partial sums of wt_structure_functions_noalloc live in this file-scope array, one
+// copy per GPU, held until the process ends. Host code never passes `scratch` itself to a kernel
+// (on the host it names a stand-in, not device memory): sf_scratch_init resolves each device's
+// address with hipGetSymbolAddress once, and the kernels get that address.
+constexpr int64_t kSfScratchDoubles = (int64_t)4 << 20;     // 32 MiB per GPU
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kSfScratchDoubles];
+static double *g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Called with `device` current (wt_create), once per device.
+hipError_t sf_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *p = nullptr;
+    WT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = (double *)p;
+    return hipSuccess;
+}
+
+int64_t sf_noalloc_max_cols(const wt_ctx *c, int mode) {
+    return kSfScratchDoubles / sf_rows(c, mode, c->inst[c->cur].N);
+}
+
+hipError_t launch_sf_noalloc(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const int32_t *d_seps,
+                             double *d_out) {
+    if (pmax - pmin + 1 > kMaxOrder || pmin < 1) return hipErrorInvalidValue;
+    double *partials = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (c->device >= 0 && c->device < kMaxDevices) partials = g_scratch_ptr[c->device];
+    }
+    if (!partials) return hipErrorNotInitialized;
+    const int64_t rows = sf_rows(c, mode, c->inst[c->cur].N);
+    if (rows * nsep * (pmax - pmin + 1) > kSfScratchDoubles) return hipErrorInvalidValue;
+    hipError_t e = c->precision == 64 ? sf_core<double>(c, mode, pmin, pmax, nsep, d_seps, partials, d_out)
+                                      : sf_core<float>(c, mode, pmin, pmax, nsep, d_seps, partials, d_out);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e;
 }
 
 // ---- every instance of a batch in one launch ------------------------------------------------

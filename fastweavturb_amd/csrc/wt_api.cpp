@@ -182,6 +182,7 @@ int wt_create_batch(wt_ctx **out, int32_t ninst, const int32_t *n, int32_t preci
         return code;
     };
     if ((e = hipSetDevice(device)) != hipSuccess) return fail(e, "hipSetDevice");
+    if ((e = wt::sf_scratch_init(device)) != hipSuccess) return fail(e, "scratch address");
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
     for (auto &ev : c->ev)
         if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e, "event");
@@ -850,6 +851,26 @@ int wt_structure_functions(wt_ctx *c, int32_t mode, int32_t pmin, int32_t pmax, 
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d_seps);
     if (d_out) (void)hipFree(d_out);
+    if (e == hipErrorInvalidValue) return set_err(c, WT_ERANGE, "too many separations/orders for one block's LDS");
+    if (e != hipSuccess) return hip_err(c, e, "structure functions");
+    return WT_OK;
+}
+
+int wt_structure_functions_noalloc(wt_ctx *c, int32_t mode, int32_t pmin, int32_t pmax, int32_t nsep,
+                                   const int32_t *seps_dev, double *S_dev) {
+    CHECK_CTX(c);
+    if (!c->have_u) return set_err(c, WT_ESTATE, "no velocity resident");
+    if (mode != 0 && mode != 1) return set_err(c, WT_EINVAL, "mode must be 0 or 1");
+    if (mode == 0 && pmin != 1)
+        return set_err(c, WT_EINVAL, "mode 0 (Sp_calculate) computes orders 1..pmax; pmin must be 1");
+    if (pmin < 1 || pmax < pmin || pmax - pmin + 1 > 12)
+        return set_err(c, WT_EINVAL, "need 1 <= pmin <= pmax, at most 12 orders");
+    if (nsep <= 0 || !seps_dev || !S_dev) return set_err(c, WT_EINVAL, "need nsep > 0 and non-NULL device arrays");
+    const int64_t cols = (int64_t)nsep * (pmax - pmin + 1), maxc = wt::sf_noalloc_max_cols(c, mode);
+    if (cols > maxc)
+        return set_err(c, WT_ERANGE, "nsep * orders = %lld exceeds the %lld the fixed scratch holds at this grid size",
+                       (long long)cols, (long long)maxc);
+    hipError_t e = wt::launch_sf_noalloc(c, mode, pmin, pmax, nsep, seps_dev, S_dev);
     if (e == hipErrorInvalidValue) return set_err(c, WT_ERANGE, "too many separations/orders for one block's LDS");
     if (e != hipSuccess) return hip_err(c, e, "structure functions");
     return WT_OK;

@@ -207,6 +207,14 @@ void fft_reversal(int N, std::vector<int32_t> &rev);
 // stats.hip: structure functions of instance c->cur
 hipError_t launch_sf(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const int32_t *d_seps,
                      double *d_out);
+// Allocation-free structure functions: the per-block partial sums in a file-scope device array of
+// stats.hip whose address sf_scratch_init (called by wt_create with the device current) resolves
+// once per device. launch_sf_noalloc returns hipErrorInvalidValue when rows * nsep * orders exceeds
+// it; sf_noalloc_max_cols gives the largest nsep * orders for the selected instance.
+hipError_t sf_scratch_init(int device);
+int64_t sf_noalloc_max_cols(const wt_ctx *c, int mode);
+hipError_t launch_sf_noalloc(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const int32_t *d_seps,
+                             double *d_out);
 // PDF of the three components of omega (field 0) or u (field 1) of instance c->cur (cal_pdf)
 hipError_t launch_pdf(wt_ctx *c, int field, int nbins, float *val, float *pdf, int64_t *counts);
 // x-increment structure functions of every instance in one launch: d_seps[pos*nsep + m] per arena

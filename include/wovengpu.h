@@ -163,6 +163,17 @@ int32_t wt_nek(int32_t n);
  *   S[(p-pmin)*nsep + m] = sum (u_x(i+sep_m) - u_x(i))^p / N^3, p = pmin..pmax. */
 int wt_structure_functions(wt_ctx *ctx, int32_t mode, int32_t pmin, int32_t pmax,
                            int32_t nsep, const int32_t *seps, double *S);
+/* wt_structure_functions without any device allocation: no counterpart in the reference; it is
+ * for callers that cannot allocate (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). seps_dev (nsep int32) and S_dev (nsep * (pmax - pmin + 1) doubles, same layout
+ * as S above) are device pointers on the context's GPU. The per-block partial sums live in a fixed
+ * 32 MiB device array per GPU (held until the process ends), so nsep * (pmax - pmin + 1) may not
+ * exceed 2^22 / rows, rows = 2 x CUs in mode 1 and at most 4 x CUs in mode 0 (8192 and 4096 on a
+ * 256-CU MI355X); larger requests return WT_ERANGE. Every call on a GPU shares that array: calls that
+ * overlap on one GPU (several contexts or host threads) race. Synchronous on return; the results
+ * equal wt_structure_functions' bitwise. */
+int wt_structure_functions_noalloc(wt_ctx *ctx, int32_t mode, int32_t pmin, int32_t pmax, int32_t nsep,
+                                   const int32_t *seps_dev, double *S_dev);
 /* Separations of Sp_calculate (subroutine.f90:78, 97-98). Returns the count. */
 int32_t wt_sp_separations(int32_t n, int32_t *r, int32_t cap);
 
