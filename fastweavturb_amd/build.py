@@ -25,6 +25,7 @@ SOURCES = {
     "scan.hip": [],
     "fft.hip": [],
     "stats.hip": [],
+    "fbb.hip": [],
     "mlp.hip": [],
     "wt_api.cpp": ["-x", "hip"],
 }

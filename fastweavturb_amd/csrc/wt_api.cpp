@@ -1000,6 +1000,39 @@ int wt_get_sections(wt_ctx *c, int32_t *ntf, double *dtf) {
     return WT_OK;
 }
 
+int32_t wt_fbb_embedding(int32_t npoint) {
+    if (npoint < 3) return WT_EINVAL;
+    const int64_t m = wt::fbb_embedding(npoint);
+    return m > ((int64_t)1 << 20) ? WT_ERANGE : (int32_t)m;
+}
+
+int wt_fbb_centerlines(wt_ctx *c, int32_t ntube, const int32_t *npoint, const double *hurst, const int32_t *winding,
+                       double wiggle, uint64_t seed, const double *gauss, double *cx, double *cy, double *cz) {
+    CHECK_CTX(c);
+    if (ntube <= 0 || !npoint || !hurst || !winding || !cx || !cy || !cz)
+        return set_err(c, WT_EINVAL, "need ntube > 0 and non-NULL arrays");
+    if (!(wiggle >= 0.0) || !std::isfinite(wiggle)) return set_err(c, WT_EINVAL, "wiggle must be finite and >= 0");
+    for (int l = 0; l < ntube; l++) {
+        if (npoint[l] < 3) return set_err(c, WT_EINVAL, "tube %d has %d points (at least 3)", l, npoint[l]);
+        if (wt::fbb_embedding(npoint[l]) > ((int64_t)1 << 20))
+            return set_err(c, WT_ERANGE, "tube %d: %d points exceed the generator's 2^19", l, npoint[l]);
+        if (!(hurst[l] > 0.0 && hurst[l] < 1.0)) return set_err(c, WT_EINVAL, "tube %d: Hurst index outside (0, 1)", l);
+        if (!winding[3 * l] && !winding[3 * l + 1] && !winding[3 * l + 2])
+            return set_err(c, WT_EINVAL, "tube %d: zero winding vector", l);
+    }
+    hipError_t e = wt::fbb_centerlines(c, ntube, npoint, hurst, winding, wiggle, seed, gauss, cx, cy, cz);
+    if (e != hipSuccess) return hip_err(c, e, "fbb centerlines");
+    return WT_OK;
+}
+
+int wt_fbb_normals(wt_ctx *c, uint64_t seed, int32_t tube, int64_t count, double *out) {
+    CHECK_CTX(c);
+    if (tube < 0 || count <= 0 || !out) return set_err(c, WT_EINVAL, "need tube >= 0, count > 0 and an output array");
+    hipError_t e = wt::fbb_normals(c, seed, (uint64_t)tube, count, out);
+    if (e != hipSuccess) return hip_err(c, e, "fbb normals");
+    return WT_OK;
+}
+
 int wt_get_tube_lengths(wt_ctx *c, double *length) {
     CHECK_CTX(c);
     if (!c->have_segments) return set_err(c, WT_ESTATE, "no segments built");

@@ -220,6 +220,11 @@ hipError_t launch_pdf(wt_ctx *c, int field, int nbins, float *val, float *pdf, i
 // x-increment structure functions of every instance in one launch: d_seps[pos*nsep + m] per arena
 // position, d_out[(pos*np + p)*nsep + m]
 hipError_t launch_sf_batch(wt_ctx *c, int pmin, int pmax, int nsep, const int32_t *d_seps, double *d_out);
+// fbb.hip: fBb centerlines of ntube tubes into host arrays (gauss: fixed random inputs or NULL)
+hipError_t fbb_centerlines(wt_ctx *c, int ntube, const int32_t *npoint, const double *hurst, const int32_t *winding,
+                           double wiggle, uint64_t seed, const double *gauss, double *cx, double *cy, double *cz);
+hipError_t fbb_normals(wt_ctx *c, uint64_t seed, uint64_t tube, int64_t count, double *host);
+int64_t fbb_embedding(int64_t npoint);
 // scan.hip
 hipError_t exclusive_scan_i64(const int64_t *d_in, int64_t *d_out, int64_t n, hipStream_t s,
                               void **tmp, size_t *tmp_bytes);

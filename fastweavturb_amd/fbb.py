@@ -53,6 +53,21 @@ def _circulant_sqrt_eigs(n: int, H: float) -> np.ndarray:
     return v
 
 
+def embedding_size(n: int) -> int:
+    """Circulant embedding length m = 2 n' (n' = smallest 2^a 3^b >= n) of an n-sample fGn draw."""
+    return 2 * _smooth_at_least(n)
+
+
+def fgn_from_normals(n: int, H: float, zre: np.ndarray, zim: np.ndarray, count: int) -> np.ndarray:
+    """(count, n) fGn samples from given standard normals zre, zim of shape (pairs, m): the real
+    and imaginary parts of FFT(sqrt(lambda/m) (zre + i zim)), truncated to n."""
+    s = _circulant_sqrt_eigs(_smooth_at_least(n), H)
+    z = zre + 1j * zim
+    w = np.fft.fft(s[None, :] * z, axis=1)[:, :n]
+    out = np.concatenate([w.real, w.imag], axis=0)
+    return out[:count]
+
+
 def fgn_davies_harte(n: int, H: float, rng: np.random.Generator, count: int = 1) -> np.ndarray:
     """(count, n) samples of unit-variance fractional Gaussian noise with Hurst index H.
 
@@ -60,22 +75,25 @@ def fgn_davies_harte(n: int, H: float, rng: np.random.Generator, count: int = 1)
     imaginary parts of one complex circulant draw are two independent samples."""
     if n <= 1:
         return rng.standard_normal((count, max(n, 0)))
-    s = _circulant_sqrt_eigs(_smooth_at_least(n), H)
-    m = len(s)
+    m = embedding_size(n)
     pairs = (count + 1) // 2
-    z = rng.standard_normal((pairs, m)) + 1j * rng.standard_normal((pairs, m))
-    w = np.fft.fft(s[None, :] * z, axis=1)[:, :n]
-    out = np.concatenate([w.real, w.imag], axis=0)
-    return out[:count]
+    zre = rng.standard_normal((pairs, m))
+    zim = rng.standard_normal((pairs, m))
+    return fgn_from_normals(n, H, zre, zim, count)
+
+
+def bridge_from_fgn(npts: int, H: float, g: np.ndarray) -> np.ndarray:
+    """(3, npts) closed fractional Brownian bridge from (3, npts - 1) fGn samples
+    (FBB_with_parameters.m:92-102)."""
+    j = np.arange(1, npts + 1, dtype=np.float64) / npts
+    f = 0.5 * (j ** (2 * H) + 1.0 - (1.0 - j) ** (2 * H))
+    B = np.concatenate([np.zeros((3, 1)), np.cumsum(g, axis=1)], axis=1)
+    return B - f[None, :] * B[:, -1:]
 
 
 def fbb_path(npts: int, H: float, rng: np.random.Generator) -> np.ndarray:
     """(3, npts) closed fractional Brownian bridge with unnormalised amplitude."""
-    j = np.arange(1, npts + 1, dtype=np.float64) / npts
-    f = 0.5 * (j ** (2 * H) + 1.0 - (1.0 - j) ** (2 * H))
-    g = fgn_davies_harte(npts - 1, H, rng, 3)
-    B = np.concatenate([np.zeros((3, 1)), np.cumsum(g, axis=1)], axis=1)
-    return B - f[None, :] * B[:, -1:]
+    return bridge_from_fgn(npts, H, fgn_davies_harte(npts - 1, H, rng, 3))
 
 
 def lattice_winding(direction: np.ndarray) -> np.ndarray:
@@ -91,13 +109,26 @@ def tube_centerline(npts: int, H: float, winding: np.ndarray, wiggle: float,
                     rng: np.random.Generator) -> np.ndarray:
     """(3, npts) centerline wrapped into [-pi, pi): shift + drift + normalised fBb."""
     bb = fbb_path(npts, H, rng)
+    return _place(bb, npts, winding, wiggle, rng.random(3))
+
+
+def tube_centerline_from(npts: int, H: float, winding: np.ndarray, wiggle: float, zre: np.ndarray,
+                         zim: np.ndarray, u3: np.ndarray) -> np.ndarray:
+    """tube_centerline from given random inputs: normals zre, zim of shape (2, m),
+    m = embedding_size(npts - 1), and 3 uniforms u3 for the shift."""
+    bb = bridge_from_fgn(npts, H, fgn_from_normals(npts - 1, H, zre, zim, 3))
+    return _place(bb, npts, winding, wiggle, u3)
+
+
+def _place(bb: np.ndarray, npts: int, winding: np.ndarray, wiggle: float, u3: np.ndarray) -> np.ndarray:
+    """Step-rms normalisation, shift, winding drift and wrap (FBB_with_parameters.m:104-120)."""
     steps = np.diff(bb, axis=1)
     rms = float(np.sum(steps ** 2) / (npts - 1))
     drift_step = 2.0 * np.pi * float(np.linalg.norm(winding)) / npts
     dB = wiggle * drift_step
     bb *= dB / math.sqrt(rms) if rms > 0 else 0.0
     t = np.arange(npts, dtype=np.float64) / npts
-    x0 = (rng.random(3) - 0.5) * 2.0 * np.pi
+    x0 = (np.asarray(u3, dtype=np.float64) - 0.5) * 2.0 * np.pi
     x = x0[:, None] + 2.0 * np.pi * winding[:, None] * t[None, :] + bb
     return np.mod(x + np.pi, 2.0 * np.pi) - np.pi
 
@@ -226,3 +257,66 @@ def config_c3(density: float = 1.0, seed: int = 1234, N: int = 1024, nlev: int =
                         0.75, (2.0 ** -l) ** (4.0 / 3.0)) for l in range(nlev)]
     return woven_levels(N, levels, seed)
 
+
+
+# ---- the same constructions with the centerlines drawn on the GPU ------------------------------
+def woven_config_device(g, N: int, levels: list[Level], seed: int, gamma_exp: float = 4.0 / 3.0,
+                        gamma_jitter: float = 0.2, wiggle: float = 0.35, sigma_ratio: float = 1.0,
+                        Rtube_ratio: float = 3.0, ntf_times: int = 50, ntf_pre: int = 1000,
+                        dzeta_sup: float = 1e-4) -> tuple[RunParams, Tubes]:
+    """woven_config with the centerlines generated on the GPU of WovenGPU context g
+    (wt_fbb_centerlines, Philox stream keyed (seed, tube)). Per tube the host draws, from
+    default_rng(seed + tube), the point count and then the circulation jitter; the level axes are
+    woven_config's."""
+    npoint, H, W, sig, gam = [], [], [], [], []
+    tube = 0
+    for li, lev in enumerate(levels):
+        lrng = np.random.default_rng(seed + 1_000_000 + li)
+        axis = lrng.standard_normal(3)
+        axis /= np.linalg.norm(axis)
+        w = lattice_winding(axis)
+        for _ in range(lev.ntubes):
+            rng = np.random.default_rng(seed + tube)
+            lo, hi = lev.npoints
+            n = lo if hi <= lo else int(math.floor(lo * (hi / lo) ** rng.random()))
+            npoint.append(max(n, 8))
+            H.append(lev.hurst)
+            W.append(w)
+            sig.append(lev.radius / math.sqrt(2.0))
+            gam.append(lev.radius ** gamma_exp * rng.uniform(1 - gamma_jitter, 1 + gamma_jitter))
+            tube += 1
+    cx, cy, cz = g.fbb_centerlines(npoint, H, np.array(W), wiggle, seed)
+    t = Tubes(np.array(npoint, dtype=np.int32), cx, cy, cz, np.array(sig), np.array(gam),
+              np.ones(tube, dtype=np.int32), np.zeros(tube, dtype=np.int32))
+    p = RunParams(N=N, n_d=N, line_s=1, line_e=tube, sigma_ratio=sigma_ratio, eta_t=0.0,
+                  ntf_times=ntf_times, ntf_pre=ntf_pre, Rtube_ratio=Rtube_ratio, dzeta_sup=dzeta_sup)
+    t.validate()
+    return p, t
+
+
+def woven_levels_device(g, N: int, levels: list, seed: int, gamma_jitter: float = 0.2, wiggle: float = 0.35,
+                        Rtube_ratio: float = 3.0, ntf_times: int = 50, ntf_pre: int = 1000,
+                        dzeta_sup: float = 1e-4) -> tuple[RunParams, Tubes]:
+    """woven_levels (LevelSpecs: one point count and Hurst index per level) with the centerlines
+    generated on the GPU of context g; per level default_rng((seed, level)) draws the axis and the
+    circulation jitters."""
+    npoint, H, W, sig, gam = [], [], [], [], []
+    for li, lev in enumerate(levels):
+        rng = np.random.default_rng([int(seed), li])
+        axis = rng.standard_normal(3)
+        axis /= np.linalg.norm(axis)
+        w = lattice_winding(axis)
+        n = max(8, int(lev.npoints))
+        npoint.extend([n] * lev.ntubes)
+        H.extend([lev.hurst] * lev.ntubes)
+        W.extend([w] * lev.ntubes)
+        sig.extend([lev.radius / math.sqrt(2.0)] * lev.ntubes)
+        gam.append(lev.gamma * rng.uniform(1 - gamma_jitter, 1 + gamma_jitter, lev.ntubes))
+    nt = len(npoint)
+    cx, cy, cz = g.fbb_centerlines(npoint, H, np.array(W), wiggle, seed)
+    tb = Tubes(np.array(npoint, dtype=np.int32), cx, cy, cz, np.array(sig), np.concatenate(gam),
+               np.ones(nt, dtype=np.int32), np.zeros(nt, dtype=np.int32))
+    p = RunParams(N=N, n_d=N, line_s=1, line_e=nt, sigma_ratio=1.0, eta_t=0.0, ntf_times=ntf_times,
+                  ntf_pre=ntf_pre, Rtube_ratio=Rtube_ratio, dzeta_sup=dzeta_sup)
+    tb.validate()
+    return p, tb

@@ -133,3 +133,14 @@ def test_product_never_imports_the_oracle():
             if f.endswith((".py", ".cpp", ".hip", ".h", ".f90", "Makefile")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert not pat.search(text), f
+
+
+def test_centerline_from_given_inputs_equals_the_seeded_draw():
+    """tube_centerline_from on the normals and uniforms tube_centerline draws gives its output
+    bitwise (the form the device generator is checked against)."""
+    n, H, w = 700, 0.7, np.array([1, -1, 0])
+    x = fbb.tube_centerline(n, H, w, 0.35, np.random.default_rng(5))
+    rng = np.random.default_rng(5)
+    m = fbb.embedding_size(n - 1)
+    zre, zim, u3 = rng.standard_normal((2, m)), rng.standard_normal((2, m)), rng.random(3)
+    np.testing.assert_array_equal(fbb.tube_centerline_from(n, H, w, 0.35, zre, zim, u3), x)
