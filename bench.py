@@ -192,25 +192,26 @@ def cpu_baseline(p, t, nseg_tube, budget_s: float):
             "note": "the reference Fortran (tube.f90) needs FFTW3 and MPI, absent here; its C restatement is timed"}
 
 
-def time_to_target(n_targets: int, n_sweeps: int):
-    """Time-to-target-spectrum on 512^3 FP32 woven fields: hidden designs of the plain method's
-    family (seeded) are generated as targets; the surrogate (trained here on the GPU from the
-    shipped generator-produced pairs) predicts a design and refines it until all 32 log shells are
-    within +-5%; the plain sweep walks its (gamma, density, hurst) grid until one candidate is.
-    Each candidate costs one full 512^3 generation plus its statistics."""
+def time_to_target(n_targets: int, n_sweeps: int, n_synth: int = 4, sweep_cap: int = 100):
+    """Time-to-target-spectrum on 512^3 FP32 woven fields (every candidate costs one full 512^3
+    generation from the GPU-drawn centerline bank plus its statistics).
+    * Reachable targets: hidden designs of the plain method's family (seeded); the surrogate
+      (trained here on the GPU from the shipped generator-produced pairs) predicts a design and
+      refines it until all 32 log shells are within +-5%; the plain sweep walks its (gamma, density,
+      hurst) grid until one candidate is. Both are timed.
+    * Synthetic targets: seeded E(k) = C k^-5/3 exp(-beta k eta) spectra and the reference's model
+      spectrum (flatness inputs: the training pairs' median); the surrogate first, the sweep as the
+      fallback (capped at sweep_cap runs); the fraction each finds and the best shell error."""
     from fastweavturb_amd import surrogate as S
-    t0 = time.perf_counter()
     bank = S.CenterlineBank(S.PAIRS_RECIPE["bank_seed"])
-    t_bank = time.perf_counter() - t0
     X, Y, M = S.load_pairs()
     sur = S.Surrogate(seed=1)
     t0 = time.perf_counter()
     hist = sur.fit(X, Y, M, epochs=300)
     t_train = time.perf_counter() - t0
-    log(f"[ttt] bank {t_bank:.1f}s; surrogate trained on {len(X)} pairs in {t_train:.1f}s "
-        f"(loss {hist[0]:.3f} -> {hist[-1]:.4f})")
+    log(f"[ttt] surrogate trained on {len(X)} pairs in {t_train:.1f}s (loss {hist[0]:.3f} -> {hist[-1]:.4f})")
     ver = S.Verifier(bank, 512)
-    rows = []
+    rows, synth = [], []
     try:
         for k, d in enumerate(S.target_designs(n_targets)):
             tgt = ver.run(d)
@@ -223,6 +224,17 @@ def time_to_target(n_targets: int, n_sweeps: int):
                             "sweep_grid": sw["grid_size"]})
             log(f"[ttt] target {k}: {json.dumps(row)}")
             rows.append(row)
+        flat = np.median(X[:, S.NSHELL:], axis=0)
+        for name, tgt in S.synthetic_targets(n_synth, flat):
+            r = S.surrogate_design(sur, tgt, ver)
+            row = {"target": name, "surrogate_found": r["found"], "surrogate_runs": r["runs"],
+                   "surrogate_s": r["seconds"], "surrogate_best_error": float(min(r["errors"]))}
+            if not r["found"] and sweep_cap > 0:
+                sw = S.sweep_design(tgt, ver, max_runs=sweep_cap)
+                row.update({"sweep_found": sw["found"], "sweep_runs": sw["runs"], "sweep_s": sw["seconds"],
+                            "sweep_best_error": float(sw["best"][0]), "sweep_best": sw["best"][1]})
+            log(f"[ttt] synthetic: {json.dumps(row)}")
+            synth.append(row)
     finally:
         ver.close()
     s_ok = [r for r in rows if r["surrogate_found"]]
@@ -237,10 +249,13 @@ def time_to_target(n_targets: int, n_sweeps: int):
                        if both else None),
            # the network's training on the GPU, shared by the targets of this run
            "speedup_incl_training": (float(np.mean([r["sweep_s"] for r in both]) /
-                                           (np.mean([r["surrogate_s"] for r in both]) + t_train / len(rows)))
+                                           (np.mean([r["surrogate_s"] for r in both]) + t_train / (len(rows) + len(synth))))
                                      if both else None),
            "training": {"pairs": int(len(X)), "train_s": t_train, "epochs": 300},
-           "per_target": rows}
+           "per_target": rows,
+           "synthetic": {"targets": len(synth), "surrogate_found": sum(r["surrogate_found"] for r in synth),
+                         "sweep_fallback_found": sum(bool(r.get("sweep_found")) for r in synth),
+                         "sweep_cap": sweep_cap, "per_target": synth}}
     return out
 
 
@@ -408,6 +423,10 @@ def main():
                     help="targets of the time-to-target-spectrum measurement (0 skips it)")
     ap.add_argument("--ttt-sweeps", type=int, default=-1,
                     help="targets also searched by the plain sweep (-1: every target)")
+    ap.add_argument("--ttt-synthetic", type=int, default=4,
+                    help="seeded k^-5/3 synthetic target spectra (the reference's model spectrum is added)")
+    ap.add_argument("--ttt-sweep-cap", type=int, default=100,
+                    help="runs of the sweep fallback for a synthetic target the surrogate misses")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default FFT tile shapes")
     ap.add_argument("--tile-cache", default=os.path.join(ROOT, "fastweavturb_amd", "data", "fft_tiles.json"),
                     help="autotune cache (JSON): shapes recorded there are reused, new ones appended")
@@ -507,7 +526,8 @@ def main():
     ttt = None
     g.close()
     if args.ttt_targets > 0 and args.config == "c2":
-        ttt = time_to_target(args.ttt_targets, args.ttt_targets if args.ttt_sweeps < 0 else args.ttt_sweeps)
+        ttt = time_to_target(args.ttt_targets, args.ttt_targets if args.ttt_sweeps < 0 else args.ttt_sweeps,
+                             args.ttt_synthetic, args.ttt_sweep_cap)
     cpu = None
     if not args.no_cpu_baseline:
         t0 = time.time()

@@ -103,7 +103,8 @@ def load_library():
     L.wt_autotune.argtypes = [vp, C.c_char_p, C.c_char_p, i32]
     L.wt_pdf.argtypes = [vp, i32, i32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int64)]
     L.wt_filter_spec.argtypes = [vp, i32]
-    L.wt_fbb_centerlines.argtypes = [vp, i32, ip, dp, ip, C.c_double, C.c_uint64, dp, dp, dp, dp]
+    L.wt_fbb_centerlines.argtypes = [vp, i32, ip, dp, ip, C.c_double, C.c_uint64, C.POINTER(C.c_uint64), dp, dp,
+                                     dp, dp]
     L.wt_fbb_embedding.argtypes = [i32]
     L.wt_fbb_embedding.restype = i32
     L.wt_fbb_normals.argtypes = [vp, C.c_uint64, i32, C.c_int64, dp]
@@ -299,19 +300,22 @@ class WovenGPU:
         self._check(self._L.wt_filter_spec(self._h, int(k_c)))
 
     # --- fBb centerline generator ----------------------------------------------------------
-    def fbb_centerlines(self, npoint, hurst, winding, wiggle: float, seed: int, gauss=None):
+    def fbb_centerlines(self, npoint, hurst, winding, wiggle: float, seed: int, gauss=None, keys=None):
         """(cx, cy, cz) of closed fBb centerlines generated on this context's GPU: tube l has
         npoint[l] points, Hurst index hurst[l] and lattice winding winding[l] (3 ints). gauss: None
-        (the device's Philox stream keyed (seed, l)) or, tube after tube, 4m normals and 3 uniforms
-        (m = fbb_embedding(npoint[l]))."""
+        (the device's Philox stream keyed (seed, keys[l] or l)) or, tube after tube, 4m normals and
+        3 uniforms (m = fbb_embedding(npoint[l]))."""
         npt = np.ascontiguousarray(npoint, dtype=np.int32)
         H = np.ascontiguousarray(np.broadcast_to(np.asarray(hurst, dtype=np.float64), npt.shape))
         w = np.ascontiguousarray(np.asarray(winding, dtype=np.int32).reshape(len(npt), 3))
         P = int(npt.sum())
         out = [np.empty(P) for _ in range(3)]
         g = None if gauss is None else np.ascontiguousarray(gauss, dtype=np.float64)
+        k = None if keys is None else np.ascontiguousarray(keys, dtype=np.uint64)
         self._check(self._L.wt_fbb_centerlines(self._h, len(npt), _ip(npt), _dp(H), _ip(w), float(wiggle),
-                                               C.c_uint64(int(seed)), None if g is None else _dp(g),
+                                               C.c_uint64(int(seed)),
+                                               None if k is None else k.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                               None if g is None else _dp(g),
                                                _dp(out[0]), _dp(out[1]), _dp(out[2])))
         return out
 

@@ -11,7 +11,7 @@
 //   - step-rms normalisation of the 3-D increments to wiggle * 2 pi |w| / n (FBB:104-112);
 //   - a uniform shift x0 in [-pi, pi)^3, the winding drift 2 pi w i/n along the integer lattice
 //     vector w, and the wrap into [-pi, pi) (FBB:117-120).
-// Random numbers: Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11) keyed (seed, tube), blocks
+// Random numbers: Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11) keyed (seed, tube key), blocks
 // at counters 1, 2, ... -- the word stream of numpy's Philox bit generator with that key; a
 // uniform is (word >> 11) 2^-53, normals come from consecutive uniform pairs by Box-Muller. A
 // tube's stream holds its 4m normals (Re z_0, Re z_1, Im z_0, Im z_1) and then 3 uniforms (x0).
@@ -79,8 +79,9 @@ __device__ __forceinline__ double philox_uniform(uint64_t seed, uint64_t tube, u
 struct FbbTube {
     int64_t pt_off;      // first point in the output arrays
     int64_t gauss_off;   // first of its 4m + 3 inputs in the staged inputs (-1: Philox)
+    uint64_t key;        // second Philox key word (the tube's index unless the caller gives keys)
     int32_t n;           // points
-    int32_t tube;        // global tube index (Philox key)
+    int32_t pad;
     double H;
     double w[3];         // lattice winding
 };
@@ -112,8 +113,8 @@ __global__ void k_fbb_draw(Z *__restrict__ draws, const Z *__restrict__ lam, con
             re = gauss[t.gauss_off + (int64_t)d * m + k];
             im = gauss[t.gauss_off + 2 * (int64_t)m + (int64_t)d * m + k];
         } else {
-            re = philox_normal(seed, (uint64_t)t.tube, (uint64_t)d * m + k);
-            im = philox_normal(seed, (uint64_t)t.tube, 2 * (uint64_t)m + (uint64_t)d * m + k);
+            re = philox_normal(seed, t.key, (uint64_t)d * m + k);
+            im = philox_normal(seed, t.key, 2 * (uint64_t)m + (uint64_t)d * m + k);
         }
         draws[i] = Z{s * re, s * im};
     }
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(kPathThreads) void k_fbb_path(const Z *__restrict__
     double x0[3];
     for (int c = 0; c < 3; c++) {
         const double u = t.gauss_off >= 0 ? gauss[t.gauss_off + 4 * (int64_t)m + c]
-                                           : philox_uniform(seed, (uint64_t)t.tube, 4 * (uint64_t)m + c);
+                                           : philox_uniform(seed, t.key, 4 * (uint64_t)m + c);
         x0[c] = (u - 0.5) * 2.0 * 3.141592653589793115997963468544185161590576171875;
     }
     const int p0 = threadIdx.x == 0 ? 0 : g0 + 1, p1 = g1;      // this thread's points
@@ -427,7 +428,8 @@ hipError_t fft_batch(Z *d, Z *scratch, int64_t nlines, int m, hipStream_t st, st
 }  // namespace
 
 hipError_t fbb_centerlines(wt_ctx *c, int ntube, const int32_t *npoint, const double *hurst, const int32_t *winding,
-                           double wiggle, uint64_t seed, const double *gauss, double *cx, double *cy, double *cz) {
+                           double wiggle, uint64_t seed, const uint64_t *keys, const double *gauss, double *cx,
+                           double *cy, double *cz) {
     hipStream_t st = c->stream;
     std::vector<int64_t> pt(ntube + 1, 0), goff(ntube + 1, 0);
     for (int l = 0; l < ntube; l++) {
@@ -457,7 +459,7 @@ hipError_t fbb_centerlines(wt_ctx *c, int ntube, const int32_t *npoint, const do
                 const int l = ls[a + b];
                 tt[b].pt_off = pt[l];
                 tt[b].n = npoint[l];
-                tt[b].tube = l;
+                tt[b].key = keys ? keys[l] : (uint64_t)l;
                 tt[b].H = hurst[l];
                 for (int k = 0; k < 3; k++) tt[b].w[k] = (double)winding[3 * l + k];
                 tt[b].gauss_off = gauss ? gsum : -1;

@@ -1007,7 +1007,8 @@ int32_t wt_fbb_embedding(int32_t npoint) {
 }
 
 int wt_fbb_centerlines(wt_ctx *c, int32_t ntube, const int32_t *npoint, const double *hurst, const int32_t *winding,
-                       double wiggle, uint64_t seed, const double *gauss, double *cx, double *cy, double *cz) {
+                       double wiggle, uint64_t seed, const uint64_t *keys, const double *gauss, double *cx, double *cy,
+                       double *cz) {
     CHECK_CTX(c);
     if (ntube <= 0 || !npoint || !hurst || !winding || !cx || !cy || !cz)
         return set_err(c, WT_EINVAL, "need ntube > 0 and non-NULL arrays");
@@ -1020,7 +1021,7 @@ int wt_fbb_centerlines(wt_ctx *c, int32_t ntube, const int32_t *npoint, const do
         if (!winding[3 * l] && !winding[3 * l + 1] && !winding[3 * l + 2])
             return set_err(c, WT_EINVAL, "tube %d: zero winding vector", l);
     }
-    hipError_t e = wt::fbb_centerlines(c, ntube, npoint, hurst, winding, wiggle, seed, gauss, cx, cy, cz);
+    hipError_t e = wt::fbb_centerlines(c, ntube, npoint, hurst, winding, wiggle, seed, keys, gauss, cx, cy, cz);
     if (e != hipSuccess) return hip_err(c, e, "fbb centerlines");
     return WT_OK;
 }

@@ -222,7 +222,8 @@ hipError_t launch_pdf(wt_ctx *c, int field, int nbins, float *val, float *pdf, i
 hipError_t launch_sf_batch(wt_ctx *c, int pmin, int pmax, int nsep, const int32_t *d_seps, double *d_out);
 // fbb.hip: fBb centerlines of ntube tubes into host arrays (gauss: fixed random inputs or NULL)
 hipError_t fbb_centerlines(wt_ctx *c, int ntube, const int32_t *npoint, const double *hurst, const int32_t *winding,
-                           double wiggle, uint64_t seed, const double *gauss, double *cx, double *cy, double *cz);
+                           double wiggle, uint64_t seed, const uint64_t *keys, const double *gauss, double *cx,
+                           double *cy, double *cz);
 hipError_t fbb_normals(wt_ctx *c, uint64_t seed, uint64_t tube, int64_t count, double *host);
 int64_t fbb_embedding(int64_t npoint);
 // scan.hip

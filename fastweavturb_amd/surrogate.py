@@ -27,7 +27,6 @@ from __future__ import annotations
 import dataclasses
 import math
 import time
-from concurrent.futures import ProcessPoolExecutor
 
 import numpy as np
 
@@ -172,100 +171,51 @@ def level_max_tubes(l: int) -> int:
 
 
 class CenterlineBank:
-    """Unit fBb shapes of one tube seed. Per level l: the tubes of the largest density
-    (2 * 4^(l+1)), each a closed fBb (bridge as FBB_with_parameters.m:92-102) with unit step rms,
-    drawn for every Hurst index in HB from the same Gaussian numbers; the level's winding vector;
-    per tube a shift and a circulation jitter U(0.8, 1.2). A design takes the first tubes of each
-    level, interpolates linearly between the two nearest Hurst indices (then restores unit step
-    rms), scales the wiggle to 0.35 of the drift step (as `fbb.woven_config`), adds the drift and
-    the shift and wraps into [-pi, pi). A non-integer tube count c gives ceil(c) tubes, the last
+    """Fixed-seed centerlines of the inverse design, drawn on the GPU by the fBb generator
+    (wt_fbb_centerlines). Per level l: tube i has the Philox key l * 2^32 + i under the bank's
+    seed, so its random numbers are the same for every design; level_points(l) points; the level's
+    winding vector and each tube's circulation jitter U(0.8, 1.2) come from default_rng((seed, l, 0)).
+    A design takes the first tubes of each level, with the design's Hurst index of that level
+    (the same normals for every H, so the field changes continuously with H) and the wiggle 0.35 of
+    the drift step (as `fbb.woven_config`). A non-integer tube count c gives ceil(c) tubes, the last
     with its circulation scaled by the fraction, so the field changes continuously with every
     design parameter, and equal designs give identical fields."""
 
-    def __init__(self, seed: int, nlev: int = NLEV, wiggle: float = 0.35, arrays: dict | None = None):
+    def __init__(self, seed: int, nlev: int = NLEV, wiggle: float = 0.35):
         self.seed, self.nlev, self.wiggle = int(seed), nlev, wiggle
-        if arrays is not None:
-            self.a = arrays
-            return
-        self.a = {}
+        self.w, self.u = [], []
         for l in range(nlev):
-            n, cmax = level_points(l), level_max_tubes(l)
             r0 = np.random.default_rng([self.seed, l, 0])
             axis = r0.standard_normal(3)
             axis /= np.linalg.norm(axis)
-            self.a[f"w{l}"] = fbb.lattice_winding(axis).astype(np.float64)
-            self.a[f"x0_{l}"] = (r0.random((cmax, 3)) - 0.5) * 2.0 * np.pi
-            self.a[f"u{l}"] = r0.uniform(0.8, 1.2, cmax)
-            j = np.arange(1, n + 1, dtype=np.float64) / n
-            shapes = np.zeros((len(HB), 3, cmax, n), dtype=np.float32)
-            for h, H in enumerate(HB):
-                rng = np.random.default_rng([self.seed, l, 1])        # same numbers for every H
-                f = 0.5 * (j ** (2 * H) + 1.0 - (1.0 - j) ** (2 * H))
-                for c0 in range(0, cmax, 1024):
-                    cnt = min(1024, cmax - c0)
-                    g = fbb.fgn_davies_harte(n - 1, H, rng, 3 * cnt).reshape(cnt, 3, n - 1)
-                    B = np.concatenate([np.zeros((cnt, 3, 1)), np.cumsum(g, axis=2)], axis=2)
-                    B -= f[None, None, :] * B[:, :, -1:]
-                    rms = np.sqrt(np.sum(np.diff(B, axis=2) ** 2, axis=(1, 2)) / (n - 1))
-                    shapes[h, :, c0:c0 + cnt] = (B / rms[:, None, None]).transpose(1, 0, 2)
-            self.a[f"B{l}"] = shapes
-            # mean product of consecutive-H steps, for the step rms of an interpolated shape
-            d = np.diff(shapes.astype(np.float64), axis=3)
-            self.a[f"cross{l}"] = np.einsum("hcki,hcki->hk", d[:-1], d[1:]) / (n - 1)
-            self.a[f"sq{l}"] = np.einsum("hcki,hcki->hk", d, d) / (n - 1)
+            self.w.append(fbb.lattice_winding(axis).astype(np.int32))
+            self.u.append(r0.uniform(0.8, 1.2, level_max_tubes(l)))
 
-    def save(self, path: str):
-        np.savez(path, seed=self.seed, nlev=self.nlev, wiggle=self.wiggle, **self.a)
-
-    @staticmethod
-    def load(path: str, mmap: bool = True) -> "CenterlineBank":
-        z = np.load(path, mmap_mode="r" if mmap else None)
-        arrays = {k: z[k] for k in z.files if k not in ("seed", "nlev", "wiggle")}
-        return CenterlineBank(int(z["seed"]), int(z["nlev"]), float(z["wiggle"]), arrays)
-
-    def tubes(self, design: Design):
+    def tubes(self, design: Design, g):
         """Tube table of a design (sigma = r / sqrt(2), Gamma_l = r_0^(4/3) 2^a_l times the tube's
-        jitter, n_T = 1, no filter)."""
+        jitter, n_T = 1, no filter), centerlines generated on the GPU of WovenGPU context g."""
         if design.nlev > self.nlev:
             raise ValueError("design has more levels than the bank")
         r = design.radii()
-        npoint, cols, sig, gam = [], [[], [], []], [], []
-        tt = None
+        npoint, H, W, keys, sig, gam = [], [], [], [], [], []
         for l in range(design.nlev):
-            n = level_points(l)
             creal = min(float(level_max_tubes(l)), max(1.0, 2.0 ** design.b[l] * 4 * 4 ** l))
             cnt = int(math.ceil(creal - 1e-9))
-            H = float(np.clip(design.H[l], HB[0], HB[-1]))
-            h = min(len(HB) - 2, int((H - HB[0]) / 0.1))
-            t = (H - HB[h]) / 0.1
-            Bs = self.a[f"B{l}"]
-            ms = ((1 - t) ** 2 * np.asarray(self.a[f"sq{l}"][h, :cnt]) + t * t * np.asarray(self.a[f"sq{l}"][h + 1, :cnt])
-                  + 2 * t * (1 - t) * np.asarray(self.a[f"cross{l}"][h, :cnt]))
-            w = self.a[f"w{l}"]
-            dB = self.wiggle * 2.0 * np.pi * float(np.linalg.norm(w)) / n
-            scale = (dB / np.sqrt(ms)).astype(np.float32)
-            tt = np.arange(n, dtype=np.float64) / n
-            x0 = np.asarray(self.a[f"x0_{l}"][:cnt])
-            for c in range(3):
-                B = (np.float32(1 - t) * Bs[h, c, :cnt] + np.float32(t) * Bs[h + 1, c, :cnt]) * scale[:, None]
-                X = B.astype(np.float64)
-                X += x0[:, c, None] + 2.0 * np.pi * w[c] * tt[None, :]
-                X += np.pi
-                np.mod(X, 2.0 * np.pi, out=X)
-                X -= np.pi
-                cols[c].append(X.ravel())
-            npoint.extend([n] * cnt)
+            npoint.extend([level_points(l)] * cnt)
+            H.extend([float(np.clip(design.H[l], HB[0], HB[-1]))] * cnt)
+            W.extend([self.w[l]] * cnt)
+            keys.extend(((l << 32) + np.arange(cnt)).tolist())
             sig.extend([r[l] / math.sqrt(2.0)] * cnt)
-            gl = r[0] ** (4.0 / 3.0) * 2.0 ** design.a[l] * np.array(self.a[f"u{l}"][:cnt], dtype=np.float64)
+            gl = r[0] ** (4.0 / 3.0) * 2.0 ** design.a[l] * np.array(self.u[l][:cnt], dtype=np.float64)
             gl[-1] *= creal - (cnt - 1)
             gam.append(gl)
+        cx, cy, cz = g.fbb_centerlines(npoint, H, np.array(W), self.wiggle, self.seed, keys=np.array(keys, np.uint64))
         nt = len(npoint)
-        return Tubes(np.array(npoint, dtype=np.int32), np.concatenate(cols[0]), np.concatenate(cols[1]),
-                     np.concatenate(cols[2]), np.array(sig), np.concatenate(gam), np.ones(nt, dtype=np.int32),
-                     np.zeros(nt, dtype=np.int32))
+        return Tubes(np.array(npoint, dtype=np.int32), cx, cy, cz, np.array(sig), np.concatenate(gam),
+                     np.ones(nt, dtype=np.int32), np.zeros(nt, dtype=np.int32))
 
-    def config(self, design: Design, N: int, ntf_pre: int = 1000):
-        t = self.tubes(design)
+    def config(self, design: Design, N: int, g, ntf_pre: int = 1000):
+        t = self.tubes(design, g)
         p = RunParams(N=N, n_d=N, line_s=1, line_e=t.nline, sigma_ratio=1.0, eta_t=0.0, ntf_times=50,
                       ntf_pre=ntf_pre, Rtube_ratio=3.0, dzeta_sup=1e-4)
         return p, t
@@ -309,43 +259,11 @@ def draw_inverse_design(rng: np.random.Generator, jitter: bool, nlev: int = NLEV
 
 
 # ---- pair generation ---------------------------------------------------------------------------
-_WORKER_BANK = None
-
-
-def _init_worker(bank_path):
-    global _WORKER_BANK
-    _WORKER_BANK = CenterlineBank.load(bank_path) if bank_path else None
-
-
-def _make_instance(args):
-    """Builds one instance's tube table in a worker; the point arrays go back through a file in
-    /dev/shm (memory-backed) instead of the result pipe."""
-    import os
-    import tempfile
-    design, N, seed = args
-    p, t = _WORKER_BANK.config(design, N) if _WORKER_BANK is not None else instance_config(design, N, seed)
-    fd, path = tempfile.mkstemp(prefix="wt_inst_", suffix=".npy", dir="/dev/shm" if os.path.isdir("/dev/shm") else None)
-    with os.fdopen(fd, "wb") as f:
-        np.save(f, np.stack([t.cx, t.cy, t.cz]))
-    small = Tubes(t.npoint, np.zeros(0), np.zeros(0), np.zeros(0), t.sigma, t.gamma, t.n_T, t.if_filter)
-    return p, small, path
-
-
-def _take_instance(res):
-    import os
-    p, t, path = res
-    xyz = np.load(path)
-    os.unlink(path)
-    return p, Tubes(t.npoint, xyz[0], xyz[1], xyz[2], t.sigma, t.gamma, t.n_T, t.if_filter)
-
-
-def generate_pairs(jobs, ensemble: int = 64, workers: int = 8, log=None, bank_path: str | None = None):
+def generate_pairs(jobs, ensemble: int = 64, log=None, bank: CenterlineBank | None = None):
     """(statistics, design, mask, N) for each job (design, N, tube seed), from ensembles of
-    `ensemble` instances, each ensemble one batched context on the GPU. Tube tables are built on
-    `workers` host processes (spawned, so none inherits a GPU context) while the GPU runs the
-    previous ensemble; with `bank_path` they come from that centerline bank instead of fresh
-    draws of the job's seed."""
-    import multiprocessing as mp
+    `ensemble` instances, each ensemble one batched context on the GPU whose centerlines are drawn
+    on the GPU too: from the bank, or (bank None) fresh from the job's seed by
+    `fbb.woven_levels_device`."""
     from .api import WovenGPU
     n = len(jobs)
     X = np.zeros((n, NFEAT), dtype=np.float32)
@@ -353,37 +271,24 @@ def generate_pairs(jobs, ensemble: int = 64, workers: int = 8, log=None, bank_pa
     M = np.zeros((n, NLEV * NPAR), dtype=np.float32)
     Ns = np.zeros(n, dtype=np.int32)
     t0 = time.time()
-    ex = None
-    if workers > 1:
-        ex = ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn"),
-                                 initializer=_init_worker, initargs=(bank_path,))
-    else:
-        _init_worker(bank_path)
-    try:
-        def submit(lo):
-            chunk = jobs[lo:lo + ensemble]
-            return [ex.submit(_make_instance, j) for j in chunk] if ex else [_make_instance(j) for j in chunk]
-        pending = submit(0)
-        for lo in range(0, n, ensemble):
-            cfgs = [_take_instance(f.result()) for f in pending] if ex else [_take_instance(r) for r in pending]
-            pending = submit(lo + ensemble) if lo + ensemble < n else []
-            g = WovenGPU(precision=32, sizes=[p.N for p, _ in cfgs])
-            for k, (p, t) in enumerate(cfgs):
-                g.select(k)
-                g.set_params(p)
-                g.set_tubes(t)
-            g.generate()
-            for k in range(len(cfgs)):
-                i = lo + k
-                X[i] = measure(g, k)
-                Y[i], M[i] = jobs[i][0].vector()
-                Ns[i] = jobs[i][1]
-            g.close()
-            if log:
-                log(f"[pairs] {lo + len(cfgs)}/{n} in {time.time() - t0:.1f}s")
-    finally:
-        if ex:
-            ex.shutdown(cancel_futures=True)
+    for lo in range(0, n, ensemble):
+        chunk = jobs[lo:lo + ensemble]
+        g = WovenGPU(precision=32, sizes=[N for _, N, _ in chunk])
+        for k, (design, N, seed) in enumerate(chunk):
+            p, t = bank.config(design, N, g) if bank is not None else \
+                fbb.woven_levels_device(g, N, design.levels(), seed)
+            g.select(k)
+            g.set_params(p)
+            g.set_tubes(t)
+        g.generate()
+        for k in range(len(chunk)):
+            i = lo + k
+            X[i] = measure(g, k)
+            Y[i], M[i] = chunk[k][0].vector()
+            Ns[i] = chunk[k][1]
+        g.close()
+        if log:
+            log(f"[pairs] {lo + len(chunk)}/{n} in {time.time() - t0:.1f}s")
     return X, Y, M, Ns
 
 
@@ -447,7 +352,7 @@ def shell_error(f, target) -> float:
 
 class Verifier:
     """Runs designs as woven fields from a centerline bank on one GPU context of grid N and
-    measures their statistics; counts runs and GPU wall time."""
+    measures their statistics; counts runs and wall time (centerlines, generation, statistics)."""
 
     def __init__(self, bank: CenterlineBank, N: int = 512):
         from .api import WovenGPU
@@ -458,8 +363,8 @@ class Verifier:
         self.seconds = 0.0
 
     def run(self, design: Design) -> np.ndarray:
-        p, t = self.bank.config(design, self.N)
         t0 = time.perf_counter()
+        p, t = self.bank.config(design, self.N, self.g)
         self.g.set_params(p)
         self.g.set_tubes(t)
         self.g.generate()
@@ -536,12 +441,55 @@ def family_jobs(n: int, seed: int, N: int, jitter_frac: float = 0.3, nlev: int =
     return [(draw_inverse_design(rng, rng.random() < jitter_frac, nlev), N, 0) for _ in range(n)]
 
 
+def ensemble_design_jobs(n: int, seed: int, grids=GRIDS):
+    """Inverse-design pairs from the ensemble workload's draws (`draw_design`: grid sizes from
+    `grids`, 3 to 6 levels as the grid resolves them, Hurst index U[0.5, 0.9], density U[0.5, 2] and
+    circulation exponent U[1.1, 1.5] per instance, each level jittered), each level's density
+    snapped to the bank's discrete tube sets."""
+    rng = np.random.default_rng(seed)
+    lv = np.log2(DENSITIES)
+    out = []
+    for _ in range(n):
+        N = int(rng.choice(grids))
+        d = draw_design(rng, N)
+        d.b = np.array([lv[np.argmin(np.abs(lv - b))] for b in d.b])
+        out.append((d, N, 0))
+    return out
+
+
 def target_designs(n: int, seed: int = 9000, nlev: int = NLEV):
-    """Hidden designs of the inverse-design targets: the plain method's family with gamma,
-    density and hurst drawn inside the sweep's range."""
+    """Hidden designs of the reachable targets: the plain method's family with gamma, density and
+    hurst drawn inside the sweep's range."""
     rng = np.random.default_rng(seed)
     return [family_design(rng.uniform(1.12, 1.48), DENSITIES[int(rng.integers(len(DENSITIES)))],
                           rng.uniform(0.52, 0.88), nlev) for _ in range(n)]
+
+
+def spectrum_features(E, flat) -> np.ndarray:
+    """48 statistics of a target given as an integer-shell spectrum E[i-1], i = 1..nek, scaled to
+    the normalised fields' energy (sum 3/2), and 16 log10 flatness values."""
+    E = np.asarray(E, dtype=np.float64)
+    E = E * (1.5 / E.sum())
+    return np.concatenate([np.log10(np.maximum(log_shells(E), 1e-300)), np.asarray(flat, dtype=np.float64)])
+
+
+def synthetic_targets(n: int, flat, seed: int = 9000, N: int = 512) -> list:
+    """Seeded synthetic target spectra E(k) = C k^(-5/3) exp(-beta k eta), beta ~ U(1.5, 3),
+    eta ~ U(1/64, 1/16) (target s uses default_rng(seed + s)), plus the reference's model spectrum
+    (output/stat/spectrum_model_classical.eqn:3-21): E = 0.2 k^(-5/3) (k/4) / sqrt((k/4)^2 + 1)
+    exp(-2.3 k / 32). Each as (name, 48 statistics) with the flatness `flat` (16 log10 values)."""
+    k = np.arange(1, N // 2 + 1, dtype=np.float64)
+    out = []
+    for s in range(n):
+        rng = np.random.default_rng(seed + s)
+        beta, eta = rng.uniform(1.5, 3.0), rng.uniform(1.0 / 64, 1.0 / 16)
+        out.append((f"k-5/3 beta={beta:.3f} eta={eta:.4f}", spectrum_features(k ** (-5.0 / 3.0) * np.exp(-beta * k * eta),
+                                                                             flat)))
+    kl = 0.25 * k
+    model = 0.2 * (k + 1e-8) ** (-5.0 / 3.0) * (kl / np.sqrt(kl * kl + 1.0)) ** (5.0 / 3.0 - 2.0 / 3.0) * \
+        np.exp(-2.3 * k * 0.03125)
+    out.append(("reference model spectrum", spectrum_features(model, flat)))
+    return out
 
 
 def sweep_family(design: Design, grid: dict | None = None) -> dict:
@@ -555,7 +503,10 @@ def sweep_family(design: Design, grid: dict | None = None) -> dict:
 
 # ---- training set of the shipped surrogate --------------------------------------------------------
 PAIRS_FILE = "data/inverse_pairs.npz"
-PAIRS_RECIPE = {"n": 8192, "jobs_seed": 77, "bank_seed": 1234, "N": 128, "jitter_frac": 0.3}
+# half from the ensemble workload's draws over its grid sizes, half from the plain method's family
+# at 128^3 (30% of those jittered per level), all on the bank's centerlines
+PAIRS_RECIPE = {"n": 20480, "jobs_seed": 77, "bank_seed": 1234, "N": 128, "jitter_frac": 0.3,
+                "ensemble_frac": 0.5}
 
 
 def pairs_path() -> str:
@@ -568,25 +519,41 @@ def load_pairs():
     return z["X"], z["Y"], z["M"]
 
 
-def make_pairs(out: str, workers: int = 12, log=print, **recipe):
-    """Regenerates the shipped training set on the GPU (about 15 minutes for 8192 pairs)."""
-    import os
-    import tempfile
+def pair_jobs(rc: dict):
+    ne = int(round(rc["n"] * rc["ensemble_frac"]))
+    jobs = ensemble_design_jobs(ne, rc["jobs_seed"]) + family_jobs(rc["n"] - ne, rc["jobs_seed"] + 1, rc["N"],
+                                                                   rc["jitter_frac"])
+    # interleaved, so every ensemble mixes both kinds
+    order = np.random.default_rng(rc["jobs_seed"] + 2).permutation(len(jobs))
+    return [jobs[i] for i in order]
+
+
+def make_pairs(out: str, log=print, lo: int = 0, hi: int | None = None, **recipe):
+    """Regenerates the shipped training set on the GPU (PAIRS_RECIPE), or the pairs lo..hi of it
+    (parts written separately are concatenated in order by `merge_pairs`)."""
     rc = dict(PAIRS_RECIPE, **recipe)
     bank = CenterlineBank(rc["bank_seed"])
-    with tempfile.TemporaryDirectory() as d:
-        bp = os.path.join(d, "bank.npz")
-        bank.save(bp)
-        jobs = family_jobs(rc["n"], rc["jobs_seed"], rc["N"], rc["jitter_frac"])
-        X, Y, M, _ = generate_pairs(jobs, ensemble=16, workers=workers, log=log, bank_path=bp)
-    np.savez_compressed(out, X=X, Y=Y, M=M)
+    jobs = pair_jobs(rc)[lo:hi]
+    X, Y, M, Ns = generate_pairs(jobs, ensemble=32, log=log, bank=bank)
+    np.savez_compressed(out, X=X, Y=Y, M=M, N=Ns)
     return X, Y, M
+
+
+def merge_pairs(parts, out: str):
+    z = [np.load(p) for p in parts]
+    np.savez_compressed(out, **{k: np.concatenate([a[k] for a in z]) for k in ("X", "Y", "M", "N")})
 
 
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser(description="regenerate the inverse-design training pairs on the GPU")
     ap.add_argument("--out", default=pairs_path())
-    ap.add_argument("--workers", type=int, default=12)
+    ap.add_argument("--n", type=int, default=PAIRS_RECIPE["n"])
+    ap.add_argument("--lo", type=int, default=0)
+    ap.add_argument("--hi", type=int, default=None)
+    ap.add_argument("--merge", nargs="*", help="concatenate these part files into --out instead")
     a = ap.parse_args()
-    make_pairs(a.out, a.workers)
+    if a.merge:
+        merge_pairs(a.merge, a.out)
+    else:
+        make_pairs(a.out, n=a.n, lo=a.lo, hi=a.hi)

@@ -241,13 +241,15 @@ int wt_stage_times(wt_ctx *ctx, double *ms, int32_t cap);
  * f(t) = (t^2H + 1 - (1-t)^2H)/2 (FBB:92-102), scaled to a step rms of wiggle * 2 pi |w| / npoint
  * (FBB:104-112), shifted uniformly, drifted by 2 pi w t along the integer lattice vector
  * w = winding[3l..3l+2] and wrapped into [-pi, pi) (FBB:117-120). Outputs cx, cy, cz: host arrays
- * of sum(npoint) points. Random inputs: with gauss = NULL a Philox4x64-10 stream keyed (seed, l)
- * (numpy's Philox bit generator with key seed + 2^64 l), normals by Box-Muller; otherwise gauss
- * holds, tube after tube, 4m normals (real parts of the two complex draws, then their imaginary
- * parts) and 3 uniforms in [0, 1) for the shift. npoint >= 3 and at most 2^19. */
+ * of sum(npoint) points. Random inputs: with gauss = NULL a Philox4x64-10 stream keyed (seed, k_l),
+ * k_l = keys[l] (or l when keys is NULL) -- numpy's Philox bit generator with key seed + 2^64 k_l --,
+ * normals by Box-Muller; otherwise gauss holds, tube after tube, 4m normals (real parts of the two
+ * complex draws, then their imaginary parts) and 3 uniforms in [0, 1) for the shift. A tube's
+ * random numbers depend only on (seed, k_l), not on its Hurst index or the other tubes.
+ * npoint >= 3 and at most 2^19. */
 int wt_fbb_centerlines(wt_ctx *ctx, int32_t ntube, const int32_t *npoint, const double *hurst,
-                       const int32_t *winding, double wiggle, uint64_t seed, const double *gauss,
-                       double *cx, double *cy, double *cz);
+                       const int32_t *winding, double wiggle, uint64_t seed, const uint64_t *keys,
+                       const double *gauss, double *cx, double *cy, double *cz);
 /* Embedding size m of a tube of npoint points: 2 x the smallest 2^a 3^b >= npoint - 1. */
 int32_t wt_fbb_embedding(int32_t npoint);
 /* The first count standard normals of tube `tube`'s random stream (inspection). */

@@ -11,12 +11,10 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def trained(tmp_path_factory):
+def trained():
     bank = S.CenterlineBank(4321, nlev=3)
-    bp = str(tmp_path_factory.mktemp("bank") / "bank.npz")
-    bank.save(bp)
     jobs = S.family_jobs(1024, 5, 64, nlev=3)
-    X, Y, M, Ns = S.generate_pairs(jobs, ensemble=64, workers=4, bank_path=bp)
+    X, Y, M, Ns = S.generate_pairs(jobs, ensemble=64, bank=bank)
     assert np.all(Ns == 64) and np.all(np.isfinite(X))
     ntr = 896
     sur = S.Surrogate(seed=3)
@@ -53,3 +51,54 @@ def test_surrogate_reaches_targets_in_fewer_runs_than_the_sweep(trained):
         assert sum(s_runs) * 5 <= sum(w_runs)
     finally:
         ver.close()
+
+
+# ---- the centerline bank ---------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def gctx():
+    from fastweavturb_amd.api import WovenGPU
+    g = WovenGPU(16, 32)
+    yield g
+    g.close()
+
+
+def test_bank_is_deterministic(gctx):
+    bank = S.CenterlineBank(31, nlev=3)
+    d = S.family_design(1.3, 1.0, 0.63, nlev=3)
+    t1, t2 = bank.tubes(d, gctx), S.CenterlineBank(31, nlev=3).tubes(d, gctx)
+    np.testing.assert_array_equal(t1.cx, t2.cx)
+    np.testing.assert_array_equal(t1.gamma, t2.gamma)
+
+
+@pytest.mark.parametrize("H", [0.5, 0.63, 0.9])
+def test_bank_wiggle_and_wrap(gctx, H):
+    import math
+    bank = S.CenterlineBank(31, nlev=3)
+    t = bank.tubes(S.family_design(1.3, 1.0, H, nlev=3), gctx)
+    assert t.cx.min() >= -math.pi and t.cx.max() < math.pi
+    off = np.concatenate([[0], np.cumsum(t.npoint)])
+    k = 4 + 16                                  # first tube of level 2
+    a, e = off[k], off[k + 1]
+    n = e - a
+    xs = np.stack([t.cx[a:e], t.cy[a:e], t.cz[a:e]])
+    st = (np.diff(xs, axis=1) + math.pi) % (2 * math.pi) - math.pi
+    drift = 2 * math.pi * bank.w[2] / n
+    dev = st - drift[:, None]
+    assert np.sqrt((dev ** 2).sum() / (n - 1)) == pytest.approx(0.35 * np.linalg.norm(drift), rel=1e-9)
+
+
+def test_bank_is_continuous_in_density_and_hurst(gctx):
+    bank = S.CenterlineBank(31, nlev=3)
+    d1 = S.family_design(1.3, 1.0, 0.7, nlev=3)
+    d2 = S.family_design(1.3, 1.0 + 1e-6, 0.7, nlev=3)
+    t1, t2 = bank.tubes(d1, gctx), bank.tubes(d2, gctx)
+    # 4, 16, 64 tubes become 5, 17, 65 with the new tubes at ~1e-6 of a tube's circulation
+    assert t2.nline == t1.nline + 3
+    new = np.concatenate([[4], [4 + 1 + 16], [4 + 1 + 16 + 1 + 64]])
+    assert np.all(np.abs(t2.gamma[new]) < 1e-4 * np.abs(t1.gamma).max())
+    keep = np.setdiff1d(np.arange(t2.nline), new)
+    np.testing.assert_array_equal(t2.npoint[keep], t1.npoint)
+    # the same tube under a slightly different Hurst index moves slightly
+    t3 = bank.tubes(S.family_design(1.3, 1.0, 0.7001, nlev=3), gctx)
+    d = (t3.cx - t1.cx + np.pi) % (2 * np.pi) - np.pi
+    assert 0 < np.abs(d).max() < 1e-2

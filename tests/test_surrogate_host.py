@@ -66,47 +66,3 @@ def test_density_snapping():
     for i, v in enumerate(lv):
         assert S.snap_density([v + 0.2, v - 0.1]) == pytest.approx(v)
     assert S.snap_density([5.0]) == pytest.approx(1.0)
-
-
-@pytest.fixture(scope="module")
-def bank():
-    return S.CenterlineBank(31, nlev=3)
-
-
-def test_bank_is_deterministic_and_loads(bank, tmp_path):
-    b2 = S.CenterlineBank(31, nlev=3)
-    for k in bank.a:
-        np.testing.assert_array_equal(np.asarray(bank.a[k]), np.asarray(b2.a[k]))
-    p = str(tmp_path / "bank.npz")
-    bank.save(p)
-    b3 = S.CenterlineBank.load(p)
-    d = S.family_design(1.3, 1.0, 0.63, nlev=3)
-    t1, t3 = bank.tubes(d), b3.tubes(d)
-    np.testing.assert_array_equal(t1.cx, t3.cx)
-    np.testing.assert_array_equal(t1.gamma, t3.gamma)
-
-
-@pytest.mark.parametrize("H", [0.5, 0.63, 0.9])
-def test_bank_wiggle_and_wrap(bank, H):
-    d = S.family_design(1.3, 1.0, H, nlev=3)
-    t = bank.tubes(d)
-    assert t.cx.min() >= -math.pi and t.cx.max() < math.pi
-    off = np.concatenate([[0], np.cumsum(t.npoint)])
-    k = 4 + 16                                  # first tube of level 2
-    a, e = off[k], off[k + 1]
-    n = e - a
-    xs = np.stack([t.cx[a:e], t.cy[a:e], t.cz[a:e]])
-    st = (np.diff(xs, axis=1) + math.pi) % (2 * math.pi) - math.pi
-    drift = 2 * math.pi * np.asarray(bank.a["w2"]) / n
-    dev = st - drift[:, None]
-    assert np.sqrt((dev ** 2).sum() / (n - 1)) == pytest.approx(0.35 * np.linalg.norm(drift), rel=1e-5)
-
-
-def test_bank_fractional_tube_is_continuous(bank):
-    d1 = S.family_design(1.3, 1.0, 0.7, nlev=3)
-    d2 = S.family_design(1.3, 1.0 + 1e-6, 0.7, nlev=3)
-    t1, t2 = bank.tubes(d1), bank.tubes(d2)
-    # 4, 16, 64 tubes become 5, 17, 65 with the new tubes at ~1e-6 of a tube's circulation
-    assert t2.nline == t1.nline + 3
-    new = np.concatenate([[4], [4 + 1 + 16], [4 + 1 + 16 + 1 + 64]])
-    assert np.all(np.abs(t2.gamma[new]) < 1e-4 * np.abs(t1.gamma).max())
