@@ -3,7 +3,8 @@
 // The grid of each instance is cut into bricks of kBrickX x kBrickY x kBrickZ cells (partial
 // bricks at the upper edge when a brick edge does not divide N). A sub-segment is paired with every brick that its
 // integer bounding box (in extended indices, wrapped periodically) overlaps, once per periodic
-// image of the brick that the box meets. The deposition
+// image of the brick that the box meets, unless that part of the box lies entirely outside the
+// slab between the sub-segment's two end planes (pair_meets_slab). The deposition
 // kernel then owns one brick at a time and accumulates all of its pairs in LDS, so no global
 // atomics touch the field.
 //
@@ -80,6 +81,31 @@ __device__ __forceinline__ AxisImages axis_images(int32_t lo, int32_t hi, int32_
     return r;
 }
 
+// Whether the part of the sub-segment's box inside brick b (axis bricks bi[c], images im[c]) can
+// hold a cell with t0 >= 0 and t1 < 0 (tube.f90:806-813). t0 and t1 are linear in the cell, so
+// over that part their extremes sit at corners picked by the signs of tci0 and tci1; the bricks
+// where even the largest t0 is negative or the smallest t1 is not, beyond 1e-12 (well above the
+// rounding of either in the reference's expression for any cell), receive no contribution from
+// the sub-segment and get no pair.
+__device__ __forceinline__ bool pair_meets_slab(const SubSeg &sg, const int32_t *bi, const int32_t *im) {
+    double hi0[3], lo1[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int32_t bd = brick_dim(c), org = bi[c] * bd, ext = sg.N - org < bd ? sg.N - org : bd;
+        const int32_t sh = (im[c] - 1) * sg.N;
+        const int32_t a = sg.lo[c] > org + 1 + sh ? sg.lo[c] : org + 1 + sh;
+        const int32_t b = sg.hi[c] < org + ext + sh ? sg.hi[c] : org + ext + sh;
+        const double xa = mesh(a, sg), xb = mesh(b, sg);
+        hi0[c] = sg.tci0[c] > 0.0 ? xb : xa;
+        lo1[c] = sg.tci1[c] > 0.0 ? xa : xb;
+    }
+    const double t0 = ((hi0[0] - sg.ci0[0]) * sg.tci0[0] + (hi0[1] - sg.ci0[1]) * sg.tci0[1]) +
+                      (hi0[2] - sg.ci0[2]) * sg.tci0[2];
+    const double t1 = ((lo1[0] - sg.ci1[0]) * sg.tci1[0] + (lo1[1] - sg.ci1[1]) * sg.tci1[1]) +
+                      (lo1[2] - sg.ci1[2]) * sg.tci1[2];
+    return t0 >= -1e-12 && t1 < 1e-12;
+}
+
 // A sub-segment with more pairs than this (a box of many bricks: the widest cores at high N) is
 // expanded by a whole wave in k_pairs_big instead of one lane, so that no lane runs thousands of
 // steps while the rest of its wave waits.
@@ -140,12 +166,13 @@ __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restri
         int64_t key = 0;
         int32_t code = 0;
         if (act) {
-            int32_t bxi, byi, bzi, kx, ky, kz;
-            ab[0].at(t % nx, bxi, kx);
-            ab[1].at((t / nx) % ny, byi, ky);
-            ab[2].at(t / (nx * ny), bzi, kz);
-            key = brick0 + ((int64_t)bzi * nby + byi) * nbx + bxi;
-            code = kx + 3 * ky + 9 * kz;
+            int32_t bi[3], im[3];
+            ab[0].at(t % nx, bi[0], im[0]);
+            ab[1].at((t / nx) % ny, bi[1], im[1]);
+            ab[2].at(t / (nx * ny), bi[2], im[2]);
+            key = brick0 + ((int64_t)bi[2] * nby + bi[1]) * nbx + bi[0];
+            code = im[0] + 3 * im[1] + 9 * im[2];
+            act = pair_meets_slab(sg, bi, im);
         }
         uint32_t slot = wave_count_add(cnt, key, act);
         if (FILL && act) {
@@ -175,15 +202,16 @@ __global__ __launch_bounds__(256) void k_pairs_big(Geo g, Run r0, uint32_t *__re
         for (int c = 0; c < 3; c++) ab[c] = axis_images(sg.lo[c], sg.hi[c], sg.N, brick_dim(c));
         const int32_t nx = ab[0].count(), ny = ab[1].count(), n = nx * ny * ab[2].count();
         for (int32_t t = lane; t < n; t += 64) {
-            int32_t bxi, byi, bzi, kx, ky, kz;
-            ab[0].at(t % nx, bxi, kx);
-            ab[1].at((t / nx) % ny, byi, ky);
-            ab[2].at(t / (nx * ny), bzi, kz);
-            const int64_t key = r.brick0 + ((int64_t)bzi * r.nb[1] + byi) * r.nb[0] + bxi;
+            int32_t bi[3], im[3];
+            ab[0].at(t % nx, bi[0], im[0]);
+            ab[1].at((t / nx) % ny, bi[1], im[1]);
+            ab[2].at(t / (nx * ny), bi[2], im[2]);
+            if (!pair_meets_slab(sg, bi, im)) continue;
+            const int64_t key = r.brick0 + ((int64_t)bi[2] * r.nb[1] + bi[1]) * r.nb[0] + bi[0];
             const uint32_t slot = atomicAdd(cnt + key, 1u);
             if (FILL) {
                 pairs[off[key] + slot] = (uint32_t)q;
-                pair_img[off[key] + slot] = (uint8_t)(kx + 3 * ky + 9 * kz);
+                pair_img[off[key] + slot] = (uint8_t)(im[0] + 3 * im[1] + 9 * im[2]);
             }
         }
     }
@@ -201,12 +229,52 @@ static hipError_t balloc(wt_ctx *c, T **p, size_t count) {
     return hipSuccess;
 }
 
+// Count pass of the binning: pairs per brick, their offsets, the wide-box list.
+static hipError_t bin_count_kernels(wt_ctx *c) {
+    Geo &g = c->geo;
+    Bins &b = c->bins;
+    BinWs &w = c->binws;
+    hipStream_t st = c->stream;
+    WT_HIP(fill_zero(w.cnt, sizeof(uint32_t) * b.nbricks, st));
+    WT_HIP(fill_zero(w.err, sizeof(int32_t), st));
+    WT_HIP(fill_zero(w.nbig, sizeof(uint32_t), st));
+    kt_begin(c, kSlotBinCount);
+    hipLaunchKernelGGL(k_pairs<false>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, w.cnt,
+                       (const int64_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr, w.err, w.big, w.nbig,
+                       w.big_cap);
+    hipLaunchKernelGGL(k_pairs_big<false>, dim3(w.big_grid), dim3(256), 0, st, g, c->run, w.cnt,
+                       (const int64_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr, (const uint32_t *)w.big,
+                       (const uint32_t *)w.nbig, w.big_cap);
+    kt_end(c, kSlotBinCount);
+    WT_HIP(hipGetLastError());
+    return exclusive_scan_i32_to_i64((const int32_t *)w.cnt, 0, b.brick_off, b.nbricks, st, w.scan);
+}
+
+// Fill pass: the (sub-point, image) pairs of every brick at its offsets.
+static hipError_t bin_fill_kernels(wt_ctx *c) {
+    Geo &g = c->geo;
+    Bins &b = c->bins;
+    BinWs &w = c->binws;
+    hipStream_t st = c->stream;
+    WT_HIP(fill_zero(w.cnt, sizeof(uint32_t) * b.nbricks, st));
+    kt_begin(c, kSlotBinFill);
+    hipLaunchKernelGGL(k_pairs<true>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, w.cnt,
+                       (const int64_t *)b.brick_off, b.pairs, b.pair_img, w.err, w.big, w.nbig, w.big_cap);
+    hipLaunchKernelGGL(k_pairs_big<true>, dim3(w.big_grid), dim3(256), 0, st, g, c->run, w.cnt,
+                       (const int64_t *)b.brick_off, b.pairs, b.pair_img, (const uint32_t *)w.big,
+                       (const uint32_t *)w.nbig, w.big_cap);
+    kt_end(c, kSlotBinFill);
+    return hipGetLastError();
+}
+
 hipError_t launch_bins(wt_ctx *c) {
     Geo &g = c->geo;
     Bins &b = c->bins;
+    BinWs &w = c->binws;
     hipStream_t st = c->stream;
     for (void *p : c->bin_allocs) (void)hipFree(p);
     c->bin_allocs.clear();
+    w = BinWs{};
     // brick table: (instance, bx, by, bz) of every brick, instances in arena order
     int64_t nbricks = 0;
     for (int i = 0; i < c->ninst(); i++) {
@@ -222,50 +290,35 @@ hipError_t launch_bins(wt_ctx *c) {
                 for (int x = 0; x < r.nb[0]; x++)
                     tab[(size_t)(r.brick0 + ((int64_t)z * r.nb[1] + y) * r.nb[0] + x)] = make_int4(i, x, y, z);
     }
-    uint32_t *cnt = nullptr, *big = nullptr, *nbig = nullptr;
-    int32_t *err = nullptr;
     WT_HIP(balloc(c, &b.brick_tab, nbricks));
     WT_HIP(balloc(c, &b.brick_off, nbricks + 1));
-    WT_HIP(balloc(c, &cnt, nbricks));
-    WT_HIP(balloc(c, &err, 1));
+    WT_HIP(balloc(c, &w.cnt, nbricks));
+    WT_HIP(balloc(c, &w.err, 1));
+    WT_HIP(balloc(c, &w.scan, scan_scratch_elems(nbricks)));
     // list of wide boxes: sized for one sub-segment in 64 (at least 1024), grown if the count pass
     // finds more
-    uint32_t big_cap = (uint32_t)std::min<int64_t>(std::max<int64_t>(g.nsub_all / 64, 1024), g.nsub_all + 1);
-    WT_HIP(balloc(c, &big, big_cap));
-    WT_HIP(balloc(c, &nbig, 1));
+    w.big_cap = (uint32_t)std::min<int64_t>(std::max<int64_t>(g.nsub_all / 64, 1024), g.nsub_all + 1);
+    WT_HIP(balloc(c, &w.big, w.big_cap));
+    WT_HIP(balloc(c, &w.nbig, 1));
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
-    const unsigned big_grid = (unsigned)(4 * cus);
+    w.big_grid = (unsigned)(4 * cus);
     WT_HIP(hipMemcpyAsync(b.brick_tab, tab.data(), sizeof(int4) * nbricks, hipMemcpyHostToDevice, st));
     int64_t npairs = 0;
     for (int attempt = 0;; attempt++) {
-        WT_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * nbricks, st));
-        WT_HIP(hipMemsetAsync(err, 0, sizeof(int32_t), st));
-        WT_HIP(hipMemsetAsync(nbig, 0, sizeof(uint32_t), st));
-        kt_begin(c, kSlotBinCount);
-        hipLaunchKernelGGL(k_pairs<false>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, cnt,
-                           (const int64_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr, err, big, nbig, big_cap);
-        hipLaunchKernelGGL(k_pairs_big<false>, dim3(big_grid), dim3(256), 0, st, g, c->run, cnt,
-                           (const int64_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr, (const uint32_t *)big,
-                           (const uint32_t *)nbig, big_cap);
-        kt_end(c, kSlotBinCount);
-        WT_HIP(hipGetLastError());
-        void *tmp = nullptr;
-        size_t tmp_bytes = 0;
-        WT_HIP(exclusive_scan_i32_to_i64((const int32_t *)cnt, 0, b.brick_off, nbricks, st, &tmp, &tmp_bytes));
+        WT_HIP(bin_count_kernels(c));
         int32_t herr = 0;
         uint32_t hbig = 0;
         WT_HIP(hipMemcpyAsync(&npairs, b.brick_off + nbricks, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-        WT_HIP(hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        WT_HIP(hipMemcpyAsync(&hbig, nbig, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        WT_HIP(hipMemcpyAsync(&herr, w.err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        WT_HIP(hipMemcpyAsync(&hbig, w.nbig, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         WT_HIP(hipStreamSynchronize(st));
-        if (tmp) (void)hipFree(tmp);
         if (herr) return hipErrorInvalidValue;
-        if (hbig <= big_cap) break;
+        if (hbig <= w.big_cap) break;
         if (attempt > 0) return hipErrorUnknown;   // the list was sized from the first count
         // too many wide boxes for the list: grow it and count again
-        big_cap = hbig;
-        WT_HIP(balloc(c, &big, big_cap));
+        w.big_cap = hbig;
+        WT_HIP(balloc(c, &w.big, w.big_cap));
     }
     b.npairs = npairs;
     // the pair list is kept between calls and grown when needed
@@ -279,15 +332,14 @@ hipError_t launch_bins(wt_ctx *c) {
     }
     b.pairs = c->pairs_buf;
     b.pair_img = (uint8_t *)(c->pairs_buf + c->pairs_cap);
-    WT_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * nbricks, st));
-    kt_begin(c, kSlotBinFill);
-    hipLaunchKernelGGL(k_pairs<true>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, cnt,
-                       (const int64_t *)b.brick_off, b.pairs, b.pair_img, err, big, nbig, big_cap);
-    hipLaunchKernelGGL(k_pairs_big<true>, dim3(big_grid), dim3(256), 0, st, g, c->run, cnt,
-                       (const int64_t *)b.brick_off, b.pairs, b.pair_img, (const uint32_t *)big, (const uint32_t *)nbig,
-                       big_cap);
-    kt_end(c, kSlotBinFill);
-    return hipGetLastError();
+    return bin_fill_kernels(c);
+}
+
+// The binning again on the sub-segments and buffers of the last launch_bins (the same tubes give
+// the same counts): kernels only, for recording into a graph.
+hipError_t replay_bins(wt_ctx *c) {
+    WT_HIP(bin_count_kernels(c));
+    return bin_fill_kernels(c);
 }
 
 }  // namespace wt

@@ -888,12 +888,16 @@ static hipError_t deposit_t(wt_ctx *c, DepArgs &A) {
     return hipGetLastError();
 }
 
-hipError_t launch_deposit(wt_ctx *c, bool accumulate) {
+// The deposit kernel with counters zeroed on the device (context-owned, 5 words).
+static hipError_t deposit_kernels(wt_ctx *c, bool accumulate) {
     hipStream_t st = c->stream;
     const bool f64 = c->precision == 64;
-    unsigned long long *cnt = nullptr;
-    WT_HIP(hipMallocAsync((void **)&cnt, 5 * sizeof(unsigned long long), st));
-    WT_HIP(hipMemsetAsync(cnt, 0, 5 * sizeof(unsigned long long), st));
+    if (!c->dep_cnt) {
+        if (c->capturing) return hipErrorInvalidValue;     // the eager run allocates them
+        WT_HIP(hipMalloc((void **)&c->dep_cnt, 5 * sizeof(unsigned long long)));
+    }
+    unsigned long long *cnt = c->dep_cnt;
+    WT_HIP(fill_zero(cnt, 5 * sizeof(unsigned long long), st));
     DepArgs A{};
     A.g = c->geo;
     A.r = c->run;
@@ -908,22 +912,26 @@ hipError_t launch_deposit(wt_ctx *c, bool accumulate) {
     bool fast = !f64;   // FP32 core only for constant core radius and no twist in every instance
     for (const Run &r : c->runs_host)
         if (r.eta != 0.0 || r.sigma_ratio != 1.0) fast = false;
-    hipError_t e = f64 ? deposit_t<true, false>(c, A) : fast ? deposit_t<false, true>(c, A) : deposit_t<false, false>(c, A);
-    if (e != hipSuccess) {
-        (void)hipFreeAsync(cnt, st);
-        return e;
-    }
+    return f64 ? deposit_t<true, false>(c, A) : fast ? deposit_t<false, true>(c, A) : deposit_t<false, false>(c, A);
+}
+
+hipError_t launch_deposit(wt_ctx *c, bool accumulate) {
+    hipStream_t st = c->stream;
+    WT_HIP(deposit_kernels(c, accumulate));
     unsigned long long h[4];
-    WT_HIP(hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, st));
-    WT_HIP(hipFreeAsync(cnt, st));
+    WT_HIP(hipMemcpyAsync(h, c->dep_cnt, sizeof h, hipMemcpyDeviceToHost, st));
     WT_HIP(hipStreamSynchronize(st));
     c->dep_stats.columns = (int64_t)h[0];
     c->dep_stats.candidates = (int64_t)h[1];
     c->dep_stats.hits = (int64_t)h[2];
     c->dep_stats.overflow = (int32_t)(h[3] > 0);
-    c->dep_stats.fixed_point_shift = f64 ? 0 : c->fixed_shift;
+    c->dep_stats.fixed_point_shift = c->precision == 64 ? 0 : c->fixed_shift;
     c->dep_stats.pairs = c->bins.npairs;
     return hipSuccess;
 }
+
+// The deposit of the last binning again (graph capture): the counters stay on the device, where
+// they equal the eager run's for the same tubes.
+hipError_t replay_deposit(wt_ctx *c) { return deposit_kernels(c, false); }
 
 }  // namespace wt

@@ -440,52 +440,65 @@ static void dfree(std::vector<void *> &list) {
     list.clear();
 }
 
-hipError_t launch_segments(wt_ctx *c) {
+// The kernels of the geometry stage on buffers already laid out (the stage's own or, in a
+// replay, those of the last eager run): no allocation, copy to the host or synchronisation.
+static hipError_t segment_kernels(wt_ctx *c, bool replay) {
     Geo &g = c->geo;
+    SegWs &w = c->segws;
     hipStream_t st = c->stream;
-    dfree(c->seg_allocs);
     const int bs = 256;
     // per point
-    WT_HIP(hipMemsetAsync(c->geo.cze, 0, sizeof(double) * g.npoint_all, st));
+    WT_HIP(fill_zero(g.cze, sizeof(double) * g.npoint_all, st));
     hipLaunchKernelGGL(k_fill_owner, dim3(g.nline), dim3(256), 0, st, g.pt_base, g.nline, g.pt_tube);
     hipLaunchKernelGGL(k_fill_owner, dim3(g.nline), dim3(256), 0, st, g.sec_base, g.nline, g.sec_tube);
     hipLaunchKernelGGL(k_chords, dim3(grid_for(g.npoint_all, bs)), dim3(bs), 0, st, g);
-    double *tot = nullptr;
-    WT_HIP(dalloc(c, c->seg_allocs, &tot, g.nline));
-    hipLaunchKernelGGL(k_cumsum, dim3(grid_for(g.nline, 64)), dim3(64), 0, st, g, tot);
-    hipLaunchKernelGGL(k_cze_norm, dim3(grid_for(g.npoint_all, bs)), dim3(bs), 0, st, g, tot);
+    hipLaunchKernelGGL(k_cumsum, dim3(grid_for(g.nline, 64)), dim3(64), 0, st, g, w.tot);
+    hipLaunchKernelGGL(k_cze_norm, dim3(grid_for(g.npoint_all, bs)), dim3(bs), 0, st, g, w.tot);
     hipLaunchKernelGGL(k_derivative<false>, dim3(grid_for(g.npoint_all, bs)), dim3(bs), 0, st, g);
     hipLaunchKernelGGL(k_derivative<true>, dim3(grid_for(g.npoint_all, bs)), dim3(bs), 0, st, g);
     // per section
     hipLaunchKernelGGL(k_spline, dim3(grid_for(g.nsec_all, bs)), dim3(bs), 0, st, g);
-    int32_t *err = nullptr;
-    WT_HIP(dalloc(c, c->seg_allocs, &err, 1));
-    WT_HIP(hipMemsetAsync(err, 0, sizeof(int32_t), st));
+    WT_HIP(fill_zero(w.err, sizeof(int32_t), st));
     kt_begin(c, kSlotTheta);
     if (c->theta_exact_all) {
-        hipLaunchKernelGGL(k_theta, dim3(grid_for(g.nsec_all, 128)), dim3(128), 0, st, g, c->run, err);
+        hipLaunchKernelGGL(k_theta, dim3(grid_for(g.nsec_all, 128)), dim3(128), 0, st, g, c->run, w.err);
     } else {
-        int32_t *redo = nullptr;
-        unsigned int *nredo = nullptr;
-        WT_HIP(dalloc(c, c->seg_allocs, &redo, g.nsec_all));
-        WT_HIP(dalloc(c, c->seg_allocs, &nredo, 1));
-        WT_HIP(hipMemsetAsync(nredo, 0, sizeof(unsigned int), st));
-        hipLaunchKernelGGL(k_theta_fast, dim3(grid_for(g.nsec_all, 128)), dim3(128), 0, st, g, c->run, redo, nredo);
-        hipLaunchKernelGGL(k_theta_listed, dim3(1024), dim3(64), 0, st, g, c->run, (const int32_t *)redo,
-                           (const unsigned int *)nredo, err);
-        WT_HIP(hipMemcpyAsync(&c->theta_redo, nredo, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        WT_HIP(fill_zero(w.nredo, sizeof(unsigned int), st));
+        hipLaunchKernelGGL(k_theta_fast, dim3(grid_for(g.nsec_all, 128)), dim3(128), 0, st, g, c->run, w.redo, w.nredo);
+        hipLaunchKernelGGL(k_theta_listed, dim3(1024), dim3(64), 0, st, g, c->run, (const int32_t *)w.redo,
+                           (const unsigned int *)w.nredo, w.err);
     }
     kt_end(c, kSlotTheta);
     // sub-point layout
-    void *tmp = nullptr;
-    size_t tmp_bytes = 0;
-    WT_HIP(exclusive_scan_i32_to_i64(g.ntf, 1, g.sp_base, g.nsec_all, st, &tmp, &tmp_bytes));
+    WT_HIP(exclusive_scan_i32_to_i64(g.ntf, 1, g.sp_base, g.nsec_all, st, w.scan));
+    if (!replay) return hipGetLastError();
+    hipLaunchKernelGGL(k_fill_sub_sec, dim3(grid_for(g.nsec_all, bs)), dim3(bs), 0, st, g);
+    kt_begin(c, kSlotSubpoints);
+    hipLaunchKernelGGL(k_subpoints, dim3(grid_for(g.nsub_all, bs)), dim3(bs), 0, st, g);
+    kt_end(c, kSlotSubpoints);
+    return hipGetLastError();
+}
+
+hipError_t launch_segments(wt_ctx *c) {
+    Geo &g = c->geo;
+    SegWs &w = c->segws;
+    hipStream_t st = c->stream;
+    dfree(c->seg_allocs);
+    w = SegWs{};
+    const int bs = 256;
+    WT_HIP(dalloc(c, c->seg_allocs, &w.tot, g.nline));
+    WT_HIP(dalloc(c, c->seg_allocs, &w.err, 1));
+    WT_HIP(dalloc(c, c->seg_allocs, &w.redo, g.nsec_all));
+    WT_HIP(dalloc(c, c->seg_allocs, &w.nredo, 1));
+    WT_HIP(dalloc(c, c->seg_allocs, &w.scan, scan_scratch_elems(g.nsec_all)));
+    WT_HIP(segment_kernels(c, false));
+    if (!c->theta_exact_all)
+        WT_HIP(hipMemcpyAsync(&c->theta_redo, w.nredo, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
     int32_t herr = 0;
-    WT_HIP(hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    WT_HIP(hipMemcpyAsync(&herr, w.err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     int64_t nsub = 0;
     WT_HIP(hipMemcpyAsync(&nsub, g.sp_base + g.nsec_all, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     WT_HIP(hipStreamSynchronize(st));
-    if (tmp) (void)hipFree(tmp);
     if (herr) return hipErrorInvalidValue;
     // sub-point indices are 32-bit in the brick pair lists
     if (nsub >= (int64_t)UINT32_MAX) return hipErrorInvalidImage;
@@ -499,6 +512,10 @@ hipError_t launch_segments(wt_ctx *c) {
     kt_end(c, kSlotSubpoints);
     return hipGetLastError();
 }
+
+// The geometry stage again on the tubes and the layout of the last launch_segments (every size
+// is the same for the same tubes): kernels only, for recording into a graph.
+hipError_t replay_segments(wt_ctx *c) { return segment_kernels(c, true); }
 
 hipError_t launch_get_bins(wt_ctx *c, int32_t *d_lohi, double *d_R) {
     hipLaunchKernelGGL(k_get_bins, dim3(grid_for(c->geo.nsub_all, 256)), dim3(256), 0, c->stream,

@@ -115,8 +115,10 @@ void wt_destroy(wt_ctx *c) {
     free_list(c->geo_allocs);
     if (c->spec_exec) (void)hipGraphExecDestroy(c->spec_exec);
     if (c->spec_graph) (void)hipGraphDestroy(c->spec_graph);
+    if (c->gen_exec) (void)hipGraphExecDestroy(c->gen_exec);
+    if (c->gen_graph) (void)hipGraphDestroy(c->gen_graph);
     void *bufs[] = {c->omega, c->u, c->spec, c->pairs_buf, c->partials, c->scal, c->E_w, c->E_u, c->runs_dev,
-                    c->sf_items};
+                    c->sf_items, c->dep_cnt};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     for (void *p : c->tw_tab)
@@ -447,6 +449,15 @@ static int upload_tubes(wt_ctx *c) {
     return WT_OK;
 }
 
+// The whole-generation graph records the layout of one tube table; anything that re-lays out the
+// geometry, the binning or the FFT tiles drops it.
+static void drop_gen_graph(wt_ctx *c) {
+    if (c->gen_exec) (void)hipGraphExecDestroy(c->gen_exec);
+    if (c->gen_graph) (void)hipGraphDestroy(c->gen_graph);
+    c->gen_exec = nullptr;
+    c->gen_graph = nullptr;
+}
+
 int wt_build_segments(wt_ctx *c, wt_seg_stats *out) {
     CHECK_CTX(c);
     if (c->tubes_dirty) {
@@ -454,6 +465,7 @@ int wt_build_segments(wt_ctx *c, wt_seg_stats *out) {
         if (rc) return rc;
     }
     c->have_segments = c->have_bins = false;
+    drop_gen_graph(c);
     hipError_t e = wt::launch_segments(c);
     if (e == hipErrorInvalidValue)
         return set_err(c, WT_ERANGE, "turning angle not finite (degenerate spline: repeated or cusped points)");
@@ -481,6 +493,7 @@ int wt_get_seg_stats(wt_ctx *c, wt_seg_stats *out) {
 int wt_bin(wt_ctx *c) {
     CHECK_CTX(c);
     if (!c->have_segments) return set_err(c, WT_ESTATE, "wt_bin needs wt_build_segments");
+    drop_gen_graph(c);
     hipError_t e = wt::launch_bins(c);
     if (e == hipErrorInvalidValue)
         return set_err(c, WT_ERANGE, "a tube's cut-off radius spans more than the periodic box");
@@ -690,7 +703,8 @@ int wt_autotune(wt_ctx *c, const char *cache_path, char *report, int32_t report_
     std::string rep;
     hipError_t e = wt::spectral_autotune(c, cached, rep);
     if (e != hipSuccess) return hip_err(c, e, "autotune");
-    // the captured spectral stage used the old shapes
+    // the captured stages used the old shapes
+    drop_gen_graph(c);
     if (c->spec_exec) (void)hipGraphExecDestroy(c->spec_exec);
     if (c->spec_graph) (void)hipGraphDestroy(c->spec_graph);
     c->spec_exec = nullptr;
@@ -722,21 +736,16 @@ int wt_autotune(wt_ctx *c, const char *cache_path, char *report, int32_t report_
 
 int wt_set_graph(wt_ctx *c, int32_t enable) {
     CHECK_CTX(c);
-    c->use_graph = enable != 0;
+    if (enable < 0 || enable > 2) return set_err(c, WT_EINVAL, "graph mode must be 0, 1 or 2");
+    c->use_graph = enable == 1;
+    c->gen_graph_mode = enable == 2;
+    if (!c->gen_graph_mode) drop_gen_graph(c);
     return WT_OK;
 }
 
-int wt_biot_savart(wt_ctx *c, double *etot, double *up, double *enstrophy) {
-    CHECK_CTX(c);
-    if (!c->have_omega) return set_err(c, WT_ESTATE, "wt_biot_savart needs a deposited or set vorticity");
-    hipError_t e;
-    if (c->use_graph && c->spec_exec) {
-        e = hipGraphLaunch(c->spec_exec, c->stream);
-    } else {
-        e = wt::launch_spectral(c, 0, -1);
-        if (e == hipSuccess && c->use_graph) e = capture_spectral(c);   // replayed from the next call on
-    }
-    if (e != hipSuccess) return hip_err(c, e, "biot_savart");
+// Reads the instances' scalars after a spectral stage and checks that every velocity could be
+// normalised.
+static int finish_spectral(wt_ctx *c, double *etot, double *up, double *enstrophy) {
     std::vector<wt::DeviceScalars> all(c->ninst());
     HIPCALL(c, hipMemcpyAsync(all.data(), c->scal, sizeof(wt::DeviceScalars) * c->ninst(), hipMemcpyDeviceToHost,
                               c->stream), "scalars");
@@ -754,6 +763,50 @@ int wt_biot_savart(wt_ctx *c, double *etot, double *up, double *enstrophy) {
     return WT_OK;
 }
 
+int wt_biot_savart(wt_ctx *c, double *etot, double *up, double *enstrophy) {
+    CHECK_CTX(c);
+    if (!c->have_omega) return set_err(c, WT_ESTATE, "wt_biot_savart needs a deposited or set vorticity");
+    hipError_t e;
+    if (c->use_graph && c->spec_exec) {
+        e = hipGraphLaunch(c->spec_exec, c->stream);
+    } else {
+        e = wt::launch_spectral(c, 0, -1);
+        if (e == hipSuccess && c->use_graph) e = capture_spectral(c);   // replayed from the next call on
+    }
+    if (e != hipSuccess) return hip_err(c, e, "biot_savart");
+    return finish_spectral(c, etot, up, enstrophy);
+}
+
+// Records geometry, binning, deposition and the spectral stage of the current tubes as one graph,
+// on the buffers the eager run just laid out (same tubes, same sizes).
+static hipError_t capture_generate(wt_ctx *c) {
+    WT_HIP(wt::spectral_reserve(c));
+    WT_HIP(hipStreamSynchronize(c->stream));
+    WT_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    hipError_t e = wt::replay_segments(c);
+    if (e == hipSuccess) e = wt::replay_bins(c);
+    if (e == hipSuccess) e = wt::replay_deposit(c);
+    if (e == hipSuccess) e = wt::launch_spectral(c, 0, -1);
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e == hipSuccess) e = e2;
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return e;
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return e;
+    }
+    c->gen_graph = g;
+    c->gen_exec = x;
+    return hipSuccess;
+}
+
 int wt_instance_scalars(wt_ctx *c, double *etot, double *up, double *enstrophy) {
     CHECK_CTX(c);
     if (!c->have_u) return set_err(c, WT_ESTATE, "no velocity resident");
@@ -769,8 +822,21 @@ int wt_instance_scalars(wt_ctx *c, double *etot, double *up, double *enstrophy) 
 int wt_generate(wt_ctx *c, double *etot, double *up, double *enstrophy) {
     CHECK_CTX(c);
     if (c->tubes_dirty) {  // the upload is not part of the timed stages
+        drop_gen_graph(c);
         int rc = upload_tubes(c);
         if (rc) return rc;
+    }
+    if (c->gen_graph_mode && c->gen_exec) {   // replay: every stage of these tubes in one graph
+        HIPCALL(c, hipEventRecord(c->ev[0], c->stream), "event");
+        HIPCALL(c, hipGraphLaunch(c->gen_exec, c->stream), "generate graph");
+        HIPCALL(c, hipEventRecord(c->ev[4], c->stream), "event");
+        HIPCALL(c, hipEventSynchronize(c->ev[4]), "generate");
+        float ms = 0;
+        HIPCALL(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[4]), "event");
+        c->stage_ms[0] = ms;
+        c->stage_ms[1] = c->stage_ms[2] = c->stage_ms[3] = -1.0;
+        c->have_segments = c->have_bins = c->have_omega = true;
+        return finish_spectral(c, etot, up, enstrophy);
     }
     HIPCALL(c, hipEventRecord(c->ev[0], c->stream), "event");
     int rc = wt_build_segments(c, nullptr);
@@ -787,6 +853,10 @@ int wt_generate(wt_ctx *c, double *etot, double *up, double *enstrophy) {
         float ms = 0;
         HIPCALL(c, hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]), "event");
         c->stage_ms[i] = ms;
+    }
+    if (c->gen_graph_mode && filter_tubes(c).empty()) {
+        hipError_t e = capture_generate(c);
+        if (e != hipSuccess) return hip_err(c, e, "generate graph capture");
     }
     return WT_OK;
 }

@@ -85,6 +85,25 @@ struct Bins {
     int64_t npairs;
 };
 
+// Work buffers of the geometry and binning stages, kept from the last eager run so that the
+// stages can be replayed (recorded into a hipGraph) with no allocation.
+struct SegWs {
+    double *tot = nullptr;              // [nline] chord sums
+    int32_t *err = nullptr;             // [1] non-finite turning angle
+    int32_t *redo = nullptr;            // [nsec] sections the fast turning angle could not decide
+    unsigned int *nredo = nullptr;      // [1]
+    int64_t *scan = nullptr;            // scan workspace of the sub-point layout
+};
+struct BinWs {
+    uint32_t *cnt = nullptr;            // [nbricks] pairs per brick
+    int32_t *err = nullptr;             // [1]
+    uint32_t *big = nullptr;            // wide boxes expanded by a wave each
+    uint32_t *nbig = nullptr;           // [1]
+    uint32_t big_cap = 0;
+    unsigned big_grid = 0;
+    int64_t *scan = nullptr;            // scan workspace of the brick offsets
+};
+
 struct DeviceScalars {          // small device-resident results, one per instance
     double etot_raw, ens_raw, up, inv_up, enstrophy, etot_norm;
     unsigned long long overflow, hits, candidates;
@@ -129,6 +148,9 @@ struct wt_ctx {
     std::vector<void *> seg_allocs;       // sub-point level buffers
     wt::Bins bins{};
     std::vector<void *> bin_allocs;
+    wt::SegWs segws;
+    wt::BinWs binws;
+    unsigned long long *dep_cnt = nullptr;   // deposition counters [0..3] and brick queue cursor [4]
     uint32_t *pairs_buf = nullptr;        // brick pair list, grown on demand
     size_t pairs_cap = 0;
     void *omega = nullptr;                // arena: sum of 3*N_i^3 values (float or double)
@@ -159,6 +181,11 @@ struct wt_ctx {
     bool capturing = false;
     hipGraph_t spec_graph = nullptr;
     hipGraphExec_t spec_exec = nullptr;
+    // hipGraph of the whole generation (wt_set_graph(ctx, 2)), captured after an eager run for the
+    // current tubes and replayed while they stay unchanged
+    bool gen_graph_mode = false;
+    hipGraph_t gen_graph = nullptr;
+    hipGraphExec_t gen_exec = nullptr;
     // FFT tile shapes chosen by wt_autotune, 7 ints per grid size: N, x lines per block, x threads,
     // y columns per tile, y threads, z columns per tile, z threads (defaults for other N)
     std::vector<int32_t> fft_tiles;
@@ -185,11 +212,14 @@ inline void kt_end(wt_ctx *c, int slot) {
 }
 // segments.hip
 hipError_t launch_segments(wt_ctx *c);       // points -> sections -> sub-points
+hipError_t replay_segments(wt_ctx *c);       // the same kernels on the last layout (graph capture)
 hipError_t launch_bins(wt_ctx *c);           // per sub-point boxes -> work items
+hipError_t replay_bins(wt_ctx *c);           // the same kernels on the last layout (graph capture)
 hipError_t launch_get_bins(wt_ctx *c, int32_t *d_lohi, double *d_R);
 hipError_t launch_seg_stats(wt_ctx *c, wt_seg_stats *out);
 // deposit.hip: accumulate = false writes omega, true adds to it
 hipError_t launch_deposit(wt_ctx *c, bool accumulate = false);
+hipError_t replay_deposit(wt_ctx *c);        // the deposit kernel alone, counters left on the device
 // fft.hip: mode 0 = Biot-Savart + normalisation of every instance; mode 1 = spectrum of the
 // resident velocity of instance `only`
 hipError_t launch_spectral(wt_ctx *c, int mode, int only);
@@ -226,11 +256,13 @@ hipError_t fbb_centerlines(wt_ctx *c, int ntube, const int32_t *npoint, const do
                            double *cy, double *cz);
 hipError_t fbb_normals(wt_ctx *c, uint64_t seed, uint64_t tube, int64_t count, double *host);
 int64_t fbb_embedding(int64_t npoint);
-// scan.hip
-hipError_t exclusive_scan_i64(const int64_t *d_in, int64_t *d_out, int64_t n, hipStream_t s,
-                              void **tmp, size_t *tmp_bytes);
+// scan.hip: exclusive prefix sums with a caller-held workspace of scan_scratch_elems(n) int64, and
+// a zeroing kernel (no memset nodes in captured stages)
+int64_t scan_scratch_elems(int64_t n);
+hipError_t exclusive_scan_i64(const int64_t *d_in, int64_t *d_out, int64_t n, hipStream_t s, int64_t *ws);
 hipError_t exclusive_scan_i32_to_i64(const int32_t *d_in, int add, int64_t *d_out, int64_t n,
-                                     hipStream_t s, void **tmp, size_t *tmp_bytes);
+                                     hipStream_t s, int64_t *ws);
+hipError_t fill_zero(void *p, size_t bytes, hipStream_t st);
 int64_t nek_of(int N);
 }  // namespace wt
 

@@ -186,9 +186,16 @@ int32_t wt_sp_separations(int32_t n, int32_t *r, int32_t cap);
 int wt_batch_statistics(wt_ctx *ctx, int32_t pmin, int32_t pmax, int32_t nsep, const int32_t *seps, double *E,
                         double *S);
 
-/* Replay the spectral stage of wt_biot_savart / wt_generate (every FFT pass, Biot-Savart, spectra,
- * reductions and the normalising inverse, for all instances) from one hipGraph: captured after
- * the first eager run, launched as one graph afterwards (no counterpart in the reference). */
+/* hipGraph replay (no counterpart in the reference). enable = 1: the spectral stage of
+ * wt_biot_savart / wt_generate (every FFT pass, Biot-Savart, spectra, reductions and the
+ * normalising inverse, for all instances) is captured after the first eager run and launched as
+ * one graph afterwards, on any tubes. enable = 2: the whole wt_generate of the current tubes
+ * (geometry, binning, deposition and the spectral stage, every instance) is captured after an
+ * eager run and replayed as one graph while the tubes, parameters and tile shapes stay unchanged;
+ * a change runs the next generation eagerly and captures again. Replays leave the stage times
+ * [1..3] of wt_stage_times at -1 ([0] is the whole graph) and the counters of wt_get_seg_stats
+ * and wt_get_deposit_stats at those of the eager run, which the same tubes reproduce. Contexts
+ * with if_filter tubes are not captured. enable = 0: eager. */
 int wt_set_graph(wt_ctx *ctx, int32_t enable);
 
 /* Autotuner (no counterpart in the reference): times the candidate tile shapes of the FFT passes

@@ -1,0 +1,46 @@
+"""The binning's pair rule restated on the host (the checker of bricks.hip's k_pairs): a
+sub-segment is paired with every periodic brick image its box meets, unless the part of the box
+in that brick lies entirely outside the slab between the end planes (t0 >= 0, t1 < 0 of
+tube.f90:806-813, corner extremes with a 1e-12 margin)."""
+import numpy as np
+
+
+def _brick_images(lo, hi, N, bd=16):
+    """(brick, image, clipped a, clipped b) of one axis met by the extended range [lo, hi]."""
+    out = []
+    for k in (-1, 0, 1):
+        a, b = max(lo, 1 + k * N), min(hi, N + k * N)
+        if a <= b:
+            for br in range((a - k * N - 1) // bd, (b - k * N - 1) // bd + 1):
+                org, ext = br * bd, min(bd, N - br * bd)
+                out.append((br, k + 1, max(lo, org + 1 + k * N), min(hi, org + ext + k * N)))
+    return out
+
+
+def slab_pairs(geo, b, N, bd=16):
+    """The binning's (sub-segment, brick, image) pairs, restated on the host from the oracle's
+    boxes and chords: every brick image the box meets whose part of the box can hold a cell with
+    t0 >= 0 and t1 < 0 (corner extremes, 1e-12 margin). Returns (kept count, pruned list of
+    (segment, per-axis clipped ranges))."""
+    dx, start = 2 * np.pi / N, -np.pi
+    kept, pruned = 0, []
+    for s in range(geo.nseg):
+        if b["R"][s] < 0:
+            continue
+        q = geo.sp_base[geo.seg_sec[s]] + geo.seg_j[s]
+        t0v, t1v = geo.tci[q], geo.tci[q + 1]
+        axes = [_brick_images(b["lo"][s][c], b["hi"][s][c], N, bd) for c in range(3)]
+        for ex in axes[0]:
+            for ey in axes[1]:
+                for ez in axes[2]:
+                    rng = [(ex[2], ex[3]), (ey[2], ey[3]), (ez[2], ez[3])]
+                    hi0 = [((rng[c][1] if t0v[c] > 0 else rng[c][0]) - 1) * dx + start for c in range(3)]
+                    lo1 = [((rng[c][0] if t1v[c] > 0 else rng[c][1]) - 1) * dx + start for c in range(3)]
+                    ci0, ci1 = b["ci0"][s], b["ci1"][s]
+                    t0 = ((hi0[0] - ci0[0]) * t0v[0] + (hi0[1] - ci0[1]) * t0v[1]) + (hi0[2] - ci0[2]) * t0v[2]
+                    t1 = ((lo1[0] - ci1[0]) * t1v[0] + (lo1[1] - ci1[1]) * t1v[1]) + (lo1[2] - ci1[2]) * t1v[2]
+                    if t0 >= -1e-12 and t1 < 1e-12:
+                        kept += 1
+                    else:
+                        pruned.append((s, rng))
+    return kept, pruned

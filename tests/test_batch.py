@@ -131,3 +131,44 @@ def test_graph_replay_equals_eager_runs():
         # eager work on the graph's context between replays
         gg.select(0)
         gg.structure_functions(1, 2, 4, [1, 2])
+
+
+def test_whole_generation_graph_replay_equals_eager_runs():
+    """wt_set_graph(2): geometry, binning, deposition and the spectral stage of the current tubes
+    replayed from one hipGraph equal the eager path bitwise, replay after replay, with eager work
+    on the context (including overwritten fields) between replays; new tubes are captured again."""
+    from fastweavturb_amd.api import WovenGPU
+    sizes = [p.N for p, _ in _ensemble(1)]
+    gg = WovenGPU(precision=32, sizes=sizes)
+    gg.set_graph(2)
+    ge = WovenGPU(precision=32, sizes=sizes)
+    for seed in (1, 2):
+        cfg = _ensemble(seed)
+        for g in (gg, ge):
+            for i, (p, t) in enumerate(cfg):
+                g.select(i)
+                g.set_params(p)
+                g.set_tubes(t)
+        ge.generate()
+        ref = []
+        for i in range(len(cfg)):
+            ge.select(i)
+            ref.append((ge.get_fields("omega", fp64=False), ge.get_fields("u", fp64=False), ge.spectrum(1),
+                        ge.scalars()))
+        dep = ge.deposit_stats()
+        for rep in range(4):                     # eager + capture, then replays
+            gg.generate()
+            st = gg.stage_times()
+            assert (st[1] == -1.0) == (rep > 0), st[:4]
+            for i in range(len(cfg)):
+                gg.select(i)
+                np.testing.assert_array_equal(gg.get_fields("omega", fp64=False), ref[i][0])
+                np.testing.assert_array_equal(gg.get_fields("u", fp64=False), ref[i][1])
+                np.testing.assert_allclose(gg.spectrum(1), ref[i][2], rtol=1e-14)
+                assert gg.scalars() == ref[i][3]
+            assert gg.deposit_stats()["hits"] == dep["hits"]
+            # eager work between replays: statistics, and instance 0's fields overwritten
+            gg.select(0)
+            gg.structure_functions(1, 2, 4, [1, 2])
+            N0 = sizes[0]
+            gg.set_fields("omega", np.zeros((3, N0, N0, N0), dtype=np.float32))

@@ -9,6 +9,7 @@ import os
 import numpy as np
 import pytest
 
+from binning_rule import slab_pairs
 from fastweavturb_amd import fbb
 from fastweavturb_amd.woven_io import Tubes, read_inputs
 
@@ -143,22 +144,6 @@ def test_deposit_wide_cores(oracle, precision):
     assert err <= (1e-12 if precision == 64 else 1e-5), err
 
 
-def _expected_pairs(lohi, N, bd=16):
-    """(brick, image) pairs of each box, counted on the host the way the binning defines them."""
-    total = 0
-    for lo, hi in zip(lohi[:, :3], lohi[:, 3:]):
-        n = 1
-        for d in range(3):
-            cnt = 0
-            for k in (-1, 0, 1):
-                a, b = max(lo[d], 1 + k * N), min(hi[d], N + k * N)
-                if a <= b:
-                    cnt += (b - k * N - 1) // bd - (a - k * N - 1) // bd + 1
-            n *= cnt
-        total += n
-    return total
-
-
 def _planar_tubes(n=48):
     """Circles in the x-y and y-z planes and a straight tube along x: many chords have exact zero
     components, so end planes parallel to a column axis and chords along an axis both occur."""
@@ -192,20 +177,26 @@ def test_deposit_planar_tubes(oracle, precision):
     assert err <= (1e-12 if precision == 64 else 1e-5), err
 
 
-def test_wide_box_list_growth():
+def test_wide_box_list_growth(oracle):
     """More wide boxes (over 256 pairs each) than the list's first size: the list grows and the pair
-    lists stay complete (pair count equal to a host recount of the boxes; FP32 and FP64 agree)."""
+    lists stay complete (pair count equal to the host restatement of the binning; hits equal the
+    oracle's; FP32 and FP64 agree)."""
     p, t = fbb.config_small(N=128, seed=29, ntubes=(1,), npts=(400,), radii=(0.9,), ntf_pre=100)
     t.sigma = np.array([0.8])
+    P = oracle_params(oracle, p, t)
+    geo = oracle.Geometry(P, t.npoint, t.cx, t.cy, t.cz)
+    kept, _ = slab_pairs(geo, geo.bins(t.sigma, t.n_T), 128)
+    _, hits = geo.deposit(t.sigma, t.gamma, t.n_T)
     out = {}
     for precision in (64, 32):
         g = gpu_ctx(p, t, precision)
         g.build_segments()
         g.bin()
-        lohi, R = g.bins()
+        _, R = g.bins()
         valid = R >= 0.0
         d = g.deposit()
-        assert d["pairs"] == _expected_pairs(lohi[valid], 128)
+        assert d["pairs"] == kept
+        assert d["hits"] == hits
         out[precision] = (d["hits"], g.get_fields("omega"))
         g.close()
     assert int(valid.sum()) > 1024

@@ -317,3 +317,24 @@ def test_deposit_planes_equals_the_slice_of_the_full_deposit(oracle):
         np.testing.assert_array_equal(part, full[:, k0 - 1:k1])
         total += h
     assert total == hits
+
+
+def test_binning_slab_rule_drops_only_bricks_without_hits(oracle):
+    """Every brick part the binning's slab rule drops holds no cell that passes the reference's
+    predicate (brute force over every cell of every dropped part), on straight, curved and
+    varying-core tubes; and the rule drops a real share of the box's bricks."""
+    from binning_rule import slab_pairs
+    cases = [fbb.config_small(N=32, seed=4, ntubes=(1, 1), npts=(24, 20), radii=(0.7, 0.5), ntf_pre=40),
+             fbb.config_small(N=48, seed=8, ntubes=(1, 2), npts=(30, 40), radii=(0.6, 0.25), ntf_pre=40,
+                              sigma_ratio=4.0)]
+    for p, t in cases:
+        P = params(oracle, p.N, t.nline, sigma_ratio=p.sigma_ratio, ntf_pre=40)
+        geo = oracle.Geometry(P, t.npoint, t.cx, t.cy, t.cz)
+        b = geo.bins(t.sigma, t.n_T)
+        kept, pruned = slab_pairs(geo, b, p.N)
+        assert len(pruned) > 0.2 * (kept + len(pruned))
+        for s, rng in pruned[::3]:
+            for kp in range(rng[2][0], rng[2][1] + 1):
+                for jp in range(rng[1][0], rng[1][1] + 1):
+                    for ip in range(rng[0][0], rng[0][1] + 1):
+                        assert not geo.pair_hits(s, ip, jp, kp), (s, ip, jp, kp)
