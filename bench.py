@@ -523,6 +523,16 @@ def main():
         fftc["ours_biot_savart_stage_ms"] = float(stage_ms[3] / args.steps)
         fftc["ours_kernels_ms"] = float(sum(kernels[k] for k in spectral))
         log(f"[bench] fft compare {json.dumps(fftc)}")
+    # the same generations replayed from one hipGraph (wt_set_graph mode 2): no host work or
+    # launches between the stages; reported beside the eager figure, which carries the stage times
+    g.set_graph(2)
+    g.generate()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        g.generate()
+    graph_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    g.set_graph(0)
+    log(f"[bench] whole-generation graph: {graph_ms:.2f} ms/step")
     ttt = None
     g.close()
     if args.ttt_targets > 0 and args.config == "c2":
@@ -540,7 +550,7 @@ def main():
         "config": {"workload": wname, "N": p.N, "tubes": t.nline, "points": int(t.npoint_all),
                    "sub_segments": int(seg["nseg_all"]), "levels": 6 if args.config == "c2" else None},
         "stages_ms": dict(zip(["segments", "bin", "deposit", "biot_savart"], (stage_ms / args.steps).tolist())),
-        "kernels_ms": kernels, "roofline": roof, "headline_roofline": headline, "cpu_baseline": cpu,
+        "kernels_ms": kernels, "graph_ms_per_step": graph_ms, "roofline": roof, "headline_roofline": headline, "cpu_baseline": cpu,
         "time_to_target": ttt,
         "fft_vs_rocfft": fftc,
     }

@@ -81,29 +81,45 @@ __device__ __forceinline__ AxisImages axis_images(int32_t lo, int32_t hi, int32_
     return r;
 }
 
+// Steps through the (brick, image) entries of one axis in the order of AxisImages::at.
+struct AxisWalk {
+    int32_t bi, im;
+    __device__ void start(const AxisImages &a) {
+        im = a.l[0] >= a.f[0] ? 0 : (a.l[1] >= a.f[1] ? 1 : 2);
+        bi = a.f[im];
+    }
+    // past the axis's last entry it leaves bi beyond l[2]; callers restart the axis instead
+    __device__ void next(const AxisImages &a) {
+        if (++bi <= a.l[im] || im == 2) return;
+        im = (im == 0 && a.l[1] >= a.f[1]) ? 1 : 2;
+        bi = a.f[im];
+    }
+};
+
 // Whether the part of the sub-segment's box inside brick b (axis bricks bi[c], images im[c]) can
 // hold a cell with t0 >= 0 and t1 < 0 (tube.f90:806-813). t0 and t1 are linear in the cell, so
 // over that part their extremes sit at corners picked by the signs of tci0 and tci1; the bricks
 // where even the largest t0 is negative or the smallest t1 is not, beyond 1e-12 (well above the
 // rounding of either in the reference's expression for any cell), receive no contribution from
-// the sub-segment and get no pair.
+// the sub-segment and get no pair. Both extremes are sums of one term per axis (slab_terms),
+// added in the order x, y, z.
+__device__ __forceinline__ void slab_terms(const SubSeg &sg, int c, int32_t bi, int32_t im, double &h0, double &l1) {
+    const int32_t bd = brick_dim(c), org = bi * bd, ext = sg.N - org < bd ? sg.N - org : bd;
+    const int32_t sh = (im - 1) * sg.N;
+    const int32_t a = sg.lo[c] > org + 1 + sh ? sg.lo[c] : org + 1 + sh;
+    const int32_t b = sg.hi[c] < org + ext + sh ? sg.hi[c] : org + ext + sh;
+    const double xa = mesh(a, sg), xb = mesh(b, sg);
+    h0 = ((sg.tci0[c] > 0.0 ? xb : xa) - sg.ci0[c]) * sg.tci0[c];
+    l1 = ((sg.tci1[c] > 0.0 ? xa : xb) - sg.ci1[c]) * sg.tci1[c];
+}
+
+__device__ __forceinline__ bool slab_met(double t0, double t1) { return t0 >= -1e-12 && t1 < 1e-12; }
+
 __device__ __forceinline__ bool pair_meets_slab(const SubSeg &sg, const int32_t *bi, const int32_t *im) {
-    double hi0[3], lo1[3];
+    double h0[3], l1[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const int32_t bd = brick_dim(c), org = bi[c] * bd, ext = sg.N - org < bd ? sg.N - org : bd;
-        const int32_t sh = (im[c] - 1) * sg.N;
-        const int32_t a = sg.lo[c] > org + 1 + sh ? sg.lo[c] : org + 1 + sh;
-        const int32_t b = sg.hi[c] < org + ext + sh ? sg.hi[c] : org + ext + sh;
-        const double xa = mesh(a, sg), xb = mesh(b, sg);
-        hi0[c] = sg.tci0[c] > 0.0 ? xb : xa;
-        lo1[c] = sg.tci1[c] > 0.0 ? xa : xb;
-    }
-    const double t0 = ((hi0[0] - sg.ci0[0]) * sg.tci0[0] + (hi0[1] - sg.ci0[1]) * sg.tci0[1]) +
-                      (hi0[2] - sg.ci0[2]) * sg.tci0[2];
-    const double t1 = ((lo1[0] - sg.ci1[0]) * sg.tci1[0] + (lo1[1] - sg.ci1[1]) * sg.tci1[1]) +
-                      (lo1[2] - sg.ci1[2]) * sg.tci1[2];
-    return t0 >= -1e-12 && t1 < 1e-12;
+    for (int c = 0; c < 3; c++) slab_terms(sg, c, bi[c], im[c], h0[c], l1[c]);
+    return slab_met((h0[0] + h0[1]) + h0[2], (l1[0] + l1[1]) + l1[2]);
 }
 
 // A sub-segment with more pairs than this (a box of many bricks: the widest cores at high N) is
@@ -161,18 +177,45 @@ __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restri
         int32_t w = __shfl_xor(nmax, o, 64);
         nmax = w > nmax ? w : nmax;
     }
+    // each lane walks its box's entries x fastest, as AxisImages::at orders them; the y and z
+    // slab terms and the row's brick key change only when the walk leaves a row
+    AxisWalk wk[3];
+    double h0y = 0.0, l1y = 0.0, h0z = 0.0, l1z = 0.0;
+    int32_t jx = 0, jy = 0;
+    int64_t rowkey = 0;
+    if (n > 0) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) wk[c].start(ab[c]);
+        slab_terms(sg, 1, wk[1].bi, wk[1].im, h0y, l1y);
+        slab_terms(sg, 2, wk[2].bi, wk[2].im, h0z, l1z);
+        rowkey = brick0 + ((int64_t)wk[2].bi * nby + wk[1].bi) * nbx;
+    }
     for (int32_t t = 0; t < nmax; t++) {
         bool act = t < n;
         int64_t key = 0;
         int32_t code = 0;
         if (act) {
-            int32_t bi[3], im[3];
-            ab[0].at(t % nx, bi[0], im[0]);
-            ab[1].at((t / nx) % ny, bi[1], im[1]);
-            ab[2].at(t / (nx * ny), bi[2], im[2]);
-            key = brick0 + ((int64_t)bi[2] * nby + bi[1]) * nbx + bi[0];
-            code = im[0] + 3 * im[1] + 9 * im[2];
-            act = pair_meets_slab(sg, bi, im);
+            double h0x, l1x;
+            slab_terms(sg, 0, wk[0].bi, wk[0].im, h0x, l1x);
+            key = rowkey + wk[0].bi;
+            code = wk[0].im + 3 * wk[1].im + 9 * wk[2].im;
+            act = slab_met((h0x + h0y) + h0z, (l1x + l1y) + l1z);
+            if (++jx < nx) {
+                wk[0].next(ab[0]);
+            } else if (t + 1 < n) {
+                jx = 0;
+                wk[0].start(ab[0]);
+                if (++jy < ny) {
+                    wk[1].next(ab[1]);
+                } else {
+                    jy = 0;
+                    wk[1].start(ab[1]);
+                    wk[2].next(ab[2]);
+                    slab_terms(sg, 2, wk[2].bi, wk[2].im, h0z, l1z);
+                }
+                slab_terms(sg, 1, wk[1].bi, wk[1].im, h0y, l1y);
+                rowkey = brick0 + ((int64_t)wk[2].bi * nby + wk[1].bi) * nbx;
+            }
         }
         uint32_t slot = wave_count_add(cnt, key, act);
         if (FILL && act) {
