@@ -428,6 +428,7 @@ def main():
     ap.add_argument("--ttt-sweep-cap", type=int, default=100,
                     help="runs of the sweep fallback for a synthetic target the surrogate misses")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default FFT tile shapes")
+    ap.add_argument("--no-graph", action="store_true", help="skip timing the whole-generation graph replay")
     ap.add_argument("--tile-cache", default=os.path.join(ROOT, "fastweavturb_amd", "data", "fft_tiles.json"),
                     help="autotune cache (JSON): shapes recorded there are reused, new ones appended")
     ap.add_argument("--no-fft-compare", action="store_true",
@@ -525,14 +526,16 @@ def main():
         log(f"[bench] fft compare {json.dumps(fftc)}")
     # the same generations replayed from one hipGraph (wt_set_graph mode 2): no host work or
     # launches between the stages; reported beside the eager figure, which carries the stage times
-    g.set_graph(2)
-    g.generate()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
+    graph_ms = None
+    if not args.no_graph:
+        g.set_graph(2)
         g.generate()
-    graph_ms = (time.perf_counter() - t0) / args.steps * 1e3
-    g.set_graph(0)
-    log(f"[bench] whole-generation graph: {graph_ms:.2f} ms/step")
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            g.generate()
+        graph_ms = (time.perf_counter() - t0) / args.steps * 1e3
+        g.set_graph(0)
+        log(f"[bench] whole-generation graph: {graph_ms:.2f} ms/step")
     ttt = None
     g.close()
     if args.ttt_targets > 0 and args.config == "c2":
