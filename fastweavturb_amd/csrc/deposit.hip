@@ -612,17 +612,22 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
     W.segn = __builtin_amdgcn_readfirstlane(W.segn - cnt);
     const int32_t bl0 = ib, steps = len < kColCap ? len : kColCap;
     W.ncols += (uint32_t)steps;
-    for (int32_t col = 0; __ballot(col < steps) != 0ull; col++, ib++) {
-        int32_t nc = 0, ia0 = 0;
-        if (col < steps) nc = column_range(L, ib, &ia0);
+    // two columns per lane and step (about a third of the columns hold a candidate, so the
+    // queue rounds of both are shared)
+    for (int32_t col = 0; __ballot(col < steps) != 0ull; col += 2, ib += 2) {
+        int32_t nc0 = 0, ia0 = 0, nc1 = 0, ia1 = 0;
+        if (col < steps) nc0 = column_range(L, ib, &ia0);
+        if (col + 1 < steps) nc1 = column_range(L, ib + 1, &ia1);
+        const int32_t nc = nc0 + nc1;
         unsigned long long m;
         for (int u = 0; (m = __ballot(u < nc)) != 0ull; u++) {
             if (u < nc) {
-                const int32_t ia = ia0 + u;
+                const bool second = u >= nc0;
+                const int32_t ia = second ? ia1 + (u - nc0) : ia0 + u, jb = second ? ib + 1 : ib;
                 HitQ q;
-                q.ip = (int16_t)(ax == 0 ? ia : (bx == 0 ? ib : ic));
-                q.jp = (int16_t)(ax == 1 ? ia : (bx == 1 ? ib : ic));
-                q.kp = (int16_t)(ax == 2 ? ia : (bx == 2 ? ib : ic));
+                q.ip = (int16_t)(ax == 0 ? ia : (bx == 0 ? jb : ic));
+                q.jp = (int16_t)(ax == 1 ? ia : (bx == 1 ? jb : ic));
+                q.kp = (int16_t)(ax == 2 ? ia : (bx == 2 ? jb : ic));
                 q.owner = (int16_t)own;
                 W.candq[W.candn + lanes_below(m)] = q;
             }
