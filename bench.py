@@ -285,7 +285,14 @@ def rocfft_compare(N: int, reps: int = 5) -> dict:
             "rocfft_transforms_ms": (fwd + inv) / reps, "via": "torch.fft (hipFFT/rocFFT)", "N": N}
 
 
-def ensemble_bench(steps: int, warmup: int, n: int = 64, graph: bool = True, tile_cache: str | None = None) -> dict:
+def _force_tiles(g, tiles):
+    if tiles:
+        g.set_deposit_tiles(*[int(v) for v in tiles.split(",")])
+        log(f"[bench] deposition tiles set to {tiles}")
+
+
+def ensemble_bench(steps: int, warmup: int, n: int = 64, graph: bool = True, tile_cache: str | None = None,
+                   tiles: str | None = None) -> dict:
     """The ensemble workload: n uneven instances (grid sizes from {64, 96, 128, 192, 256}, 3-6
     levels, Hurst index, density and circulation exponent drawn per instance, tube seed 5000 + i)
     generated together in one context, the spectral stage replayed from one hipGraph, then the
@@ -300,11 +307,12 @@ def ensemble_bench(steps: int, warmup: int, n: int = 64, graph: bool = True, til
     log(f"[c4] {n} instances built in {time.time() - t0:.1f}s; sizes {dict(zip(*np.unique(sizes, return_counts=True)))}")
     g = WovenGPU(precision=32, sizes=sizes)
     g.set_graph(graph)
-    log("[c4] autotune: " + g.autotune(tile_cache).strip().replace("\n", "; "))
     for i, (p, t) in enumerate(cfgs):
         g.select(i)
         g.set_params(p)
         g.set_tubes(t)
+    log("[c4] autotune: " + g.autotune(tile_cache).strip().replace("\n", "; "))
+    _force_tiles(g, tiles)
     for _ in range(max(1, warmup)):
         g.generate()
     t0 = time.perf_counter()
@@ -335,13 +343,12 @@ def ensemble_bench(steps: int, warmup: int, n: int = 64, graph: bool = True, til
             "features_first": feats[0].round(4).tolist()}
 
 
-def c3_bench(densities=(0.5, 1.0, 2.0), tile_cache: str | None = None) -> dict:
+def c3_bench(densities=(0.5, 1.0, 2.0), tile_cache: str | None = None, tiles: str | None = None) -> dict:
     """The 1024^3 workload: 8 levels (r = 2^-l), density swept over `densities`, fields resident
     on the GPU, structure functions of orders 2-6 of u_x at every separation 1..512."""
     from fastweavturb_amd import fbb
     from fastweavturb_amd.api import WovenGPU
     g = WovenGPU(1024, 32)
-    log("[c3] autotune: " + g.autotune(tile_cache).strip().replace("\n", "; "))
     seps = np.arange(1, 513, dtype=np.int32)
     runs = []
     for d in densities:
@@ -350,6 +357,9 @@ def c3_bench(densities=(0.5, 1.0, 2.0), tile_cache: str | None = None) -> dict:
         log(f"[c3] density {d}: {t.nline} tubes, {t.npoint_all} points (built in {time.time() - t0:.1f}s)")
         g.set_params(p)
         g.set_tubes(t)
+        if d == densities[0]:              # FFT and deposition tiles (same levels at every density)
+            log("[c3] autotune: " + g.autotune(tile_cache).strip().replace("\n", "; "))
+            _force_tiles(g, tiles)
         g.generate()                       # warm-up: grows the pair list and tube arenas once
         t0 = time.perf_counter()
         etot, up, ens = g.generate()
@@ -429,7 +439,10 @@ def main():
                     help="runs of the sweep fallback for a synthetic target the surrogate misses")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default FFT tile shapes")
     ap.add_argument("--no-graph", action="store_true", help="skip timing the whole-generation graph replay")
-    ap.add_argument("--tile-cache", default=os.path.join(ROOT, "fastweavturb_amd", "data", "fft_tiles.json"),
+    ap.add_argument("--deposit-tiles", default=None,
+                    help="bx,by,bz,colcap for every grid size after the autotune (default: the tuned ones)")
+    ap.add_argument("--c3-densities", default="0.5,1,2", help="tube densities of the 1024^3 workload")
+    ap.add_argument("--tile-cache", default=os.path.join(ROOT, "fastweavturb_amd", "data", "tiles.json"),
                     help="autotune cache (JSON): shapes recorded there are reused, new ones appended")
     ap.add_argument("--no-fft-compare", action="store_true",
                     help="skip timing rocFFT's transforms of the same Biot-Savart stage")
@@ -449,10 +462,12 @@ def main():
         print(json.dumps(rocfft_compare(args.rocfft_only)), flush=True)
         return
     if args.config == "c3":
-        print(json.dumps(c3_bench(tile_cache=args.tile_cache)), flush=True)
+        print(json.dumps(c3_bench(tuple(float(v) for v in args.c3_densities.split(",")), tile_cache=args.tile_cache,
+                                  tiles=args.deposit_tiles)), flush=True)
         return
     if args.config == "c4":
-        print(json.dumps(ensemble_bench(args.steps, args.warmup, tile_cache=args.tile_cache)), flush=True)
+        print(json.dumps(ensemble_bench(args.steps, args.warmup, tile_cache=args.tile_cache, tiles=args.deposit_tiles)),
+              flush=True)
         return
     fftc = None
     if not args.no_fft_compare:
@@ -477,6 +492,7 @@ def main():
     g.set_tubes(t)
     if not args.no_autotune:
         log("[bench] autotune: " + g.autotune(args.tile_cache).strip().replace("\n", "; "))
+    _force_tiles(g, args.deposit_tiles)
     for i in range(args.warmup):
         g.generate()
         log(f"[bench] warmup {i} done, stages ms {np.round(g.stage_times()[:4], 2).tolist()}")

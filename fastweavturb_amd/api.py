@@ -101,6 +101,8 @@ def load_library():
     L.wt_batch_statistics.argtypes = [vp, i32, i32, i32, ip, dp, dp]
     L.wt_set_graph.argtypes = [vp, i32]
     L.wt_autotune.argtypes = [vp, C.c_char_p, C.c_char_p, i32]
+    L.wt_set_deposit_tiles.argtypes = [vp, i32, C.POINTER(C.c_int32)]
+    L.wt_get_deposit_tiles.argtypes = [vp, i32, C.POINTER(C.c_int32)]
     L.wt_pdf.argtypes = [vp, i32, i32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int64)]
     L.wt_filter_spec.argtypes = [vp, i32]
     L.wt_fbb_centerlines.argtypes = [vp, i32, ip, dp, ip, C.c_double, C.c_uint64, C.POINTER(C.c_uint64), dp, dp,
@@ -279,11 +281,22 @@ class WovenGPU:
         return [E[offs[i]:offs[i + 1]] for i in range(n)], S
 
     def autotune(self, cache_path: str | None = None) -> str:
-        """Times and keeps the fastest FFT tile shapes for this context's grid sizes; returns the
-        per-size report."""
-        buf = C.create_string_buffer(4096)
+        """Times and keeps the fastest FFT tile shapes for this context's grid sizes and, once
+        tubes are set, the fastest deposition tiles; returns the per-size report."""
+        buf = C.create_string_buffer(8192)
         self._check(self._L.wt_autotune(self._h, cache_path.encode() if cache_path else None, buf, len(buf)))
         return buf.value.decode()
+
+    def set_deposit_tiles(self, bx: int, by: int, bz: int, colcap: int = 8, N: int = 0):
+        """Brick shape and columns per pass of the deposition for grid size N (0: every instance)."""
+        t = (C.c_int32 * 4)(bx, by, bz, colcap)
+        self._check(self._L.wt_set_deposit_tiles(self._h, int(N), t))
+
+    def deposit_tiles(self, N: int = 0) -> tuple:
+        """(bx, by, bz, colcap) of grid size N (0: the selected instance's)."""
+        t = (C.c_int32 * 4)()
+        self._check(self._L.wt_get_deposit_tiles(self._h, int(N), t))
+        return tuple(int(v) for v in t)
 
     def pdf(self, field: int = 1, nbins: int = 1000):
         """(centres, pdf, counts) of the three components of omega (0) or u (1), as cal_pdf."""

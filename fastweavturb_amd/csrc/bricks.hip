@@ -1,6 +1,6 @@
 // Spatial binning of the deposition work (tube.f90:793-798 boxes).
 //
-// The grid of each instance is cut into bricks of kBrickX x kBrickY x kBrickZ cells (partial
+// The grid of each instance is cut into bricks of Run::bd[0] x bd[1] x bd[2] cells (partial
 // bricks at the upper edge when a brick edge does not divide N). A sub-segment is paired with every brick that its
 // integer bounding box (in extended indices, wrapped periodically) overlaps, once per periodic
 // image of the brick that the box meets, unless that part of the box lies entirely outside the
@@ -65,7 +65,7 @@ struct AxisImages {
     }
 };
 
-__device__ __forceinline__ AxisImages axis_images(int32_t lo, int32_t hi, int32_t N, int32_t bd) {
+__device__ __forceinline__ AxisImages axis_images(int32_t lo, int32_t hi, int32_t N, int32_t lbd) {
     AxisImages r;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -74,8 +74,8 @@ __device__ __forceinline__ AxisImages axis_images(int32_t lo, int32_t hi, int32_
         r.f[k] = 0;
         r.l[k] = -1;
         if (a <= b) {
-            r.f[k] = (a - s - 1) / bd;
-            r.l[k] = (b - s - 1) / bd;
+            r.f[k] = (a - s - 1) >> lbd;
+            r.l[k] = (b - s - 1) >> lbd;
         }
     }
     return r;
@@ -103,8 +103,9 @@ struct AxisWalk {
 // rounding of either in the reference's expression for any cell), receive no contribution from
 // the sub-segment and get no pair. Both extremes are sums of one term per axis (slab_terms),
 // added in the order x, y, z.
-__device__ __forceinline__ void slab_terms(const SubSeg &sg, int c, int32_t bi, int32_t im, double &h0, double &l1) {
-    const int32_t bd = brick_dim(c), org = bi * bd, ext = sg.N - org < bd ? sg.N - org : bd;
+__device__ __forceinline__ void slab_terms(const SubSeg &sg, int c, int32_t bi, int32_t im, int32_t bd, double &h0,
+                                           double &l1) {
+    const int32_t org = bi * bd, ext = sg.N - org < bd ? sg.N - org : bd;
     const int32_t sh = (im - 1) * sg.N;
     const int32_t a = sg.lo[c] > org + 1 + sh ? sg.lo[c] : org + 1 + sh;
     const int32_t b = sg.hi[c] < org + ext + sh ? sg.hi[c] : org + ext + sh;
@@ -115,10 +116,11 @@ __device__ __forceinline__ void slab_terms(const SubSeg &sg, int c, int32_t bi, 
 
 __device__ __forceinline__ bool slab_met(double t0, double t1) { return t0 >= -1e-12 && t1 < 1e-12; }
 
-__device__ __forceinline__ bool pair_meets_slab(const SubSeg &sg, const int32_t *bi, const int32_t *im) {
+__device__ __forceinline__ bool pair_meets_slab(const SubSeg &sg, const int32_t *bi, const int32_t *im,
+                                                const int32_t *bd) {
     double h0[3], l1[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) slab_terms(sg, c, bi[c], im[c], h0[c], l1[c]);
+    for (int c = 0; c < 3; c++) slab_terms(sg, c, bi[c], im[c], bd[c], h0[c], l1[c]);
     return slab_met((h0[0] + h0[1]) + h0[2], (l1[0] + l1[1]) + l1[2]);
 }
 
@@ -131,10 +133,12 @@ constexpr int32_t kBigPairs = 256;
 // sub-segments with more than kBigPairs pairs in big[0 .. *nbig) (entries beyond big_cap are not
 // stored: the host then grows the list and repeats the pass).
 // Pass 1 (FILL = true): writes every pair (sub-point index, image code kx + 3ky + 9kz) into its
-// brick's list.
+// brick's list, after the brick's wide[key] pairs of wide boxes (k_pairs_big), which lead it: the
+// deposition spreads those expensive pairs over its waves before it takes the rest in runs.
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restrict__ cnt,
-                                               const int64_t *__restrict__ off, uint32_t *__restrict__ pairs,
+                                               const int64_t *__restrict__ off, const uint32_t *__restrict__ wide,
+                                               uint32_t *__restrict__ pairs,
                                                uint8_t *__restrict__ pair_img, int32_t *__restrict__ err,
                                                uint32_t *__restrict__ big, uint32_t *__restrict__ nbig,
                                                uint32_t big_cap) {
@@ -144,12 +148,14 @@ __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restri
     AxisImages ab[3];
     int32_t n = 0, nx = 1, ny = 1;
     int64_t brick0 = 0;
-    int32_t nbx = 1, nby = 1;
+    int32_t nbx = 1, nby = 1, bd[3] = {1, 1, 1};
     if (ok) {
         const Run &r = g.runs[g.tube_inst[sg.tube]];
         brick0 = r.brick0;
         nbx = r.nb[0];
         nby = r.nb[1];
+#pragma unroll
+        for (int c = 0; c < 3; c++) bd[c] = r.bd[c];
 #pragma unroll
         for (int c = 0; c < 3; c++)
             if (sg.lo[c] < 1 - sg.N || sg.hi[c] > 2 * sg.N) ok = false;
@@ -157,7 +163,7 @@ __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restri
             if (!FILL) atomicOr(err, 2);
         } else {
 #pragma unroll
-            for (int c = 0; c < 3; c++) ab[c] = axis_images(sg.lo[c], sg.hi[c], sg.N, brick_dim(c));
+            for (int c = 0; c < 3; c++) ab[c] = axis_images(sg.lo[c], sg.hi[c], sg.N, r.lbd[c]);
             nx = ab[0].count();
             ny = ab[1].count();
             n = nx * ny * ab[2].count();
@@ -186,8 +192,8 @@ __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restri
     if (n > 0) {
 #pragma unroll
         for (int c = 0; c < 3; c++) wk[c].start(ab[c]);
-        slab_terms(sg, 1, wk[1].bi, wk[1].im, h0y, l1y);
-        slab_terms(sg, 2, wk[2].bi, wk[2].im, h0z, l1z);
+        slab_terms(sg, 1, wk[1].bi, wk[1].im, bd[1], h0y, l1y);
+        slab_terms(sg, 2, wk[2].bi, wk[2].im, bd[2], h0z, l1z);
         rowkey = brick0 + ((int64_t)wk[2].bi * nby + wk[1].bi) * nbx;
     }
     for (int32_t t = 0; t < nmax; t++) {
@@ -196,7 +202,7 @@ __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restri
         int32_t code = 0;
         if (act) {
             double h0x, l1x;
-            slab_terms(sg, 0, wk[0].bi, wk[0].im, h0x, l1x);
+            slab_terms(sg, 0, wk[0].bi, wk[0].im, bd[0], h0x, l1x);
             key = rowkey + wk[0].bi;
             code = wk[0].im + 3 * wk[1].im + 9 * wk[2].im;
             act = slab_met((h0x + h0y) + h0z, (l1x + l1y) + l1z);
@@ -211,26 +217,29 @@ __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restri
                     jy = 0;
                     wk[1].start(ab[1]);
                     wk[2].next(ab[2]);
-                    slab_terms(sg, 2, wk[2].bi, wk[2].im, h0z, l1z);
+                    slab_terms(sg, 2, wk[2].bi, wk[2].im, bd[2], h0z, l1z);
                 }
-                slab_terms(sg, 1, wk[1].bi, wk[1].im, h0y, l1y);
+                slab_terms(sg, 1, wk[1].bi, wk[1].im, bd[1], h0y, l1y);
                 rowkey = brick0 + ((int64_t)wk[2].bi * nby + wk[1].bi) * nbx;
             }
         }
         uint32_t slot = wave_count_add(cnt, key, act);
         if (FILL && act) {
-            pairs[off[key] + slot] = (uint32_t)q;
-            pair_img[off[key] + slot] = (uint8_t)code;
+            const int64_t at = off[key] + wide[key] + slot;
+            pairs[at] = (uint32_t)q;
+            pair_img[at] = (uint8_t)code;
         }
     }
 }
 
 // The sub-segments listed by k_pairs, one wave each (grid-stride over the list): lane t takes
-// pairs t, t + 64, ... of the box; each increments its brick's counter (pairs of one box are
-// distinct bricks or images) and, in the fill pass, writes its pair.
+// pairs t, t + 64, ... of the box (pairs of one box are distinct bricks or images). The count pass
+// adds each to its brick's total (cnt) and wide-pair count (wide); the fill pass writes it at the
+// slot its brick's wide-pair counter (wide: zeroed slots) gives, at the head of the brick's list.
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_pairs_big(Geo g, Run r0, uint32_t *__restrict__ cnt,
-                                                   const int64_t *__restrict__ off, uint32_t *__restrict__ pairs,
+                                                   uint32_t *__restrict__ wide, const int64_t *__restrict__ off,
+                                                   uint32_t *__restrict__ pairs,
                                                    uint8_t *__restrict__ pair_img, const uint32_t *__restrict__ big,
                                                    const uint32_t *__restrict__ nbig, uint32_t big_cap) {
     const uint32_t nb = *nbig < big_cap ? *nbig : big_cap;
@@ -242,19 +251,22 @@ __global__ __launch_bounds__(256) void k_pairs_big(Geo g, Run r0, uint32_t *__re
         const Run &r = g.runs[g.tube_inst[sg.tube]];
         AxisImages ab[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) ab[c] = axis_images(sg.lo[c], sg.hi[c], sg.N, brick_dim(c));
+        for (int c = 0; c < 3; c++) ab[c] = axis_images(sg.lo[c], sg.hi[c], sg.N, r.lbd[c]);
         const int32_t nx = ab[0].count(), ny = ab[1].count(), n = nx * ny * ab[2].count();
         for (int32_t t = lane; t < n; t += 64) {
             int32_t bi[3], im[3];
             ab[0].at(t % nx, bi[0], im[0]);
             ab[1].at((t / nx) % ny, bi[1], im[1]);
             ab[2].at(t / (nx * ny), bi[2], im[2]);
-            if (!pair_meets_slab(sg, bi, im)) continue;
+            if (!pair_meets_slab(sg, bi, im, r.bd)) continue;
             const int64_t key = r.brick0 + ((int64_t)bi[2] * r.nb[1] + bi[1]) * r.nb[0] + bi[0];
-            const uint32_t slot = atomicAdd(cnt + key, 1u);
             if (FILL) {
+                const uint32_t slot = atomicAdd(wide + key, 1u);   // wslot: the wide pairs lead the list
                 pairs[off[key] + slot] = (uint32_t)q;
                 pair_img[off[key] + slot] = (uint8_t)(im[0] + 3 * im[1] + 9 * im[2]);
+            } else {
+                atomicAdd(cnt + key, 1u);
+                atomicAdd(wide + key, 1u);
             }
         }
     }
@@ -279,13 +291,14 @@ static hipError_t bin_count_kernels(wt_ctx *c) {
     BinWs &w = c->binws;
     hipStream_t st = c->stream;
     WT_HIP(fill_zero(w.cnt, sizeof(uint32_t) * b.nbricks, st));
+    WT_HIP(fill_zero(b.brick_wide, sizeof(uint32_t) * b.nbricks, st));
     WT_HIP(fill_zero(w.err, sizeof(int32_t), st));
     WT_HIP(fill_zero(w.nbig, sizeof(uint32_t), st));
     kt_begin(c, kSlotBinCount);
     hipLaunchKernelGGL(k_pairs<false>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, w.cnt,
-                       (const int64_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr, w.err, w.big, w.nbig,
-                       w.big_cap);
-    hipLaunchKernelGGL(k_pairs_big<false>, dim3(w.big_grid), dim3(256), 0, st, g, c->run, w.cnt,
+                       (const int64_t *)nullptr, (const uint32_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr,
+                       w.err, w.big, w.nbig, w.big_cap);
+    hipLaunchKernelGGL(k_pairs_big<false>, dim3(w.big_grid), dim3(256), 0, st, g, c->run, w.cnt, b.brick_wide,
                        (const int64_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr, (const uint32_t *)w.big,
                        (const uint32_t *)w.nbig, w.big_cap);
     kt_end(c, kSlotBinCount);
@@ -300,10 +313,12 @@ static hipError_t bin_fill_kernels(wt_ctx *c) {
     BinWs &w = c->binws;
     hipStream_t st = c->stream;
     WT_HIP(fill_zero(w.cnt, sizeof(uint32_t) * b.nbricks, st));
+    WT_HIP(fill_zero(w.wslot, sizeof(uint32_t) * b.nbricks, st));
     kt_begin(c, kSlotBinFill);
     hipLaunchKernelGGL(k_pairs<true>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, w.cnt,
-                       (const int64_t *)b.brick_off, b.pairs, b.pair_img, w.err, w.big, w.nbig, w.big_cap);
-    hipLaunchKernelGGL(k_pairs_big<true>, dim3(w.big_grid), dim3(256), 0, st, g, c->run, w.cnt,
+                       (const int64_t *)b.brick_off, (const uint32_t *)b.brick_wide, b.pairs, b.pair_img, w.err, w.big,
+                       w.nbig, w.big_cap);
+    hipLaunchKernelGGL(k_pairs_big<true>, dim3(w.big_grid), dim3(256), 0, st, g, c->run, w.cnt, w.wslot,
                        (const int64_t *)b.brick_off, b.pairs, b.pair_img, (const uint32_t *)w.big,
                        (const uint32_t *)w.nbig, w.big_cap);
     kt_end(c, kSlotBinFill);
@@ -336,6 +351,8 @@ hipError_t launch_bins(wt_ctx *c) {
     WT_HIP(balloc(c, &b.brick_tab, nbricks));
     WT_HIP(balloc(c, &b.brick_off, nbricks + 1));
     WT_HIP(balloc(c, &w.cnt, nbricks));
+    WT_HIP(balloc(c, &w.wslot, nbricks));
+    WT_HIP(balloc(c, &b.brick_wide, nbricks));
     WT_HIP(balloc(c, &w.err, 1));
     WT_HIP(balloc(c, &w.scan, scan_scratch_elems(nbricks)));
     // list of wide boxes: sized for one sub-segment in 64 (at least 1024), grown if the count pass

@@ -2,7 +2,7 @@
 //
 // The reference walks every cell of each sub-segment's integer bounding box and keeps the
 // cells that pass three tests (t0 >= 0, t1 < 0, rho_c < Rtube). Here the grid is cut into
-// bricks of kBrickX x kBrickY x kBrickZ cells (bricks.hip pairs every sub-segment with the bricks
+// bricks of Run::bd cells, chosen per grid size (bricks.hip pairs every sub-segment with the bricks
 // its box overlaps). A persistent workgroup owns one brick at a time and accumulates it in LDS, then
 // writes the brick's cells once: the field sees no atomics and needs no clearing pass.
 //
@@ -308,7 +308,8 @@ struct AccT<true> {
 };
 
 // Evaluates queue entries [qn - cnt, qn) on lanes 0..cnt-1 and adds them to the brick.
-constexpr int kCells = kBrickX * kBrickY * kBrickZ;
+// LDS stride between the accumulator's components (the largest brick)
+constexpr int kCells = kBrickCells;
 
 // The sub-segment of the lane's pair, as the candidate test and the hit evaluation need it. It
 // stays in the owning lane's registers; other lanes read it with cross-lane shuffles.
@@ -566,7 +567,8 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
             if (FAST) hit_math_f32(hc, sg, h.ip, h.jp, h.kp, v);
             else hit_math_f64(W.A->g, sg, h.ip, h.jp, h.kp, v);
             const int N = W.ri->N;
-            const int cell = (wrap0(h.ip, N) - W.ox) + kBrickX * ((wrap0(h.jp, N) - W.oy) + kBrickY * (wrap0(h.kp, N) - W.oz));
+            const int cell = (wrap0(h.ip, N) - W.ox) +
+                             ((((wrap0(h.kp, N) - W.oz) << W.ri->lbd[1]) + (wrap0(h.jp, N) - W.oy)) << W.ri->lbd[0]);
             if (F64) {
 #pragma unroll
                 for (int c = 0; c < 3; c++) unsafeAtomicAdd((double *)W.acc + c * kCells + cell, v[c]);
@@ -585,12 +587,11 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
     W.novf = __builtin_amdgcn_readfirstlane(W.novf + (uint32_t)__popcll(__ballot(ovf)));
 }
 
-// Columns walked per row segment and pass: a longer segment's remainder is queued again, so that a
-// pass is not held for its longest segment while the other lanes idle.
-constexpr int32_t kColCap = 8;
+// Columns walked per row segment and pass (Run::colcap): a longer segment's remainder is queued
+// again, so that a pass is not held for its longest segment while the other lanes idle.
 
 // Walks the columns of row segments [segn - cnt, segn), one segment per lane and one column of
-// it per step, at most kColCap columns each (the remainders go back to the segment queue, which
+// it per step, at most colcap columns each (the remainders go back to the segment queue, which
 // has room: at most 63 segments stay queued plus at most 64 remainders); candidates go to the
 // candidate queue.
 template <bool F64, bool FAST>
@@ -610,7 +611,8 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
     }
     const int bx = ax == 2 ? 0 : ax + 1;
     W.segn = __builtin_amdgcn_readfirstlane(W.segn - cnt);
-    const int32_t bl0 = ib, steps = len < kColCap ? len : kColCap;
+    const int32_t cap = W.ri->colcap;
+    const int32_t bl0 = ib, steps = len < cap ? len : cap;
     W.ncols += (uint32_t)steps;
     // two columns per lane and step (about a third of the columns hold a candidate, so the
     // queue rounds of both are shared)
@@ -636,10 +638,10 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
             if (W.candn >= 64) stage_candidates<F64, FAST>(W, me, 64);
         }
     }
-    const bool more = len > kColCap;
+    const bool more = len > cap;
     const unsigned long long mr = __ballot(more);
     if (more) W.segs[W.segn + lanes_below(mr)] =
-        RowSeg{(int16_t)ic, (int16_t)(bl0 + kColCap), (uint8_t)(len - kColCap), (uint8_t)own, 0};
+        RowSeg{(int16_t)ic, (int16_t)(bl0 + cap), (uint8_t)(len - cap), (uint8_t)own, 0};
     W.segn = __builtin_amdgcn_readfirstlane(W.segn + __popcll(mr));
     wave_sync_lds();
 }
@@ -682,46 +684,67 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
         W.ri = &ri;
         const double inv_dx = 1.0 / ri.dx;   // cells per unit length
         const int N = ri.N;
-        const int ox = bt.y * kBrickX, oy = bt.z * kBrickY, oz = bt.w * kBrickZ;
-        const int ex = N - ox < kBrickX ? N - ox : kBrickX;
-        const int ey = N - oy < kBrickY ? N - oy : kBrickY;
-        const int ez = N - oz < kBrickZ ? N - oz : kBrickZ;
+        const int bdx = ri.bd[0], bdy = ri.bd[1], bdz = ri.bd[2], lbx = ri.lbd[0], lby = ri.lbd[1];
+        const int ox = bt.y * bdx, oy = bt.z * bdy, oz = bt.w * bdz;
+        const int ex = N - ox < bdx ? N - ox : bdx;
+        const int ey = N - oy < bdy ? N - oy : bdy;
+        const int ez = N - oz < bdz ? N - oz : bdz;
+        const int ncell = bdx * bdy * bdz;
         W.ox = ox;
         W.oy = oy;
         W.oz = oz;
-        for (int i = threadIdx.x; i < 3 * kCells; i += kDepThreads) acc[i] = (Acc)0;
+        for (int i = threadIdx.x; i < ncell; i += kDepThreads) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) acc[c * kCells + i] = (Acc)0;
+        }
         __syncthreads();
         const int64_t p0 = A.b.brick_off[bi], p1 = A.b.brick_off[bi + 1];
-        // chunks of 64 pairs are claimed one step ahead, and the next chunk's pair index and image
-        // read while the current one is processed
+        // The brick's list starts with its H pairs of wide boxes (each a large share of the brick's
+        // work; neighbours along a tube sit next to each other). They come first, in Kh = ceil(H / 64)
+        // chunks whose chunk k holds pairs k, k + Kh, k + 2 Kh, ... (lane j: pair k + j Kh), so that
+        // they spread over the waves instead of holding one wave while the others wait at the brick's
+        // barrier; the other pairs follow in runs of 64 (neighbouring thin sub-segments of about the
+        // same cost share a wave). Chunks are claimed one step ahead, and the next chunk's pair index
+        // and image read while the current one is processed.
+        const int64_t P = p1 - p0, H = A.b.brick_wide[bi], Kh = (H + 63) >> 6, K = Kh + ((P - H + 63) >> 6);
+        auto pair_of = [&](int64_t k) -> int64_t {
+            if (k < Kh) {
+                const int64_t i = k + (int64_t)lane * Kh;
+                return i < H ? i : P;
+            }
+            return H + ((k - Kh) << 6) + lane;
+        };
         unsigned long long next = 0;
-        if (lane == 0) next = atomicAdd(&s_misc[1], 64ull);
+        if (lane == 0) next = atomicAdd(&s_misc[1], 1ull);
         next = __shfl(next, 0, 64);
         uint32_t q_next = 0;
         int32_t code_next = 0, s_next = 0, l_next = 0;
-        if (p0 + (int64_t)next + lane < p1) {
-            q_next = A.b.pairs[p0 + (int64_t)next + lane];
-            code_next = A.b.pair_img[p0 + (int64_t)next + lane];
+        int64_t pn = pair_of((int64_t)next);
+        if ((int64_t)next < K && pn < P) {
+            q_next = A.b.pairs[p0 + pn];
+            code_next = A.b.pair_img[p0 + pn];
             s_next = g.sub_sec[q_next];
             l_next = g.sub_tube[q_next];
         }
         for (;;) {
             const unsigned long long chunk = next;
-            if (p0 + (int64_t)chunk >= p1) break;
-            const int64_t pi = p0 + (int64_t)chunk + lane;
+            if ((int64_t)chunk >= K) break;
+            const int64_t pi = pn;
             const uint32_t q_cur = q_next;
             const int32_t code = code_next, s_cur = s_next, l_cur = l_next;
-            if (lane == 0) next = atomicAdd(&s_misc[1], 64ull);
+            if (lane == 0) next = atomicAdd(&s_misc[1], 1ull);
             next = __shfl(next, 0, 64);
-            if (p0 + (int64_t)next + lane < p1) {
-                q_next = A.b.pairs[p0 + (int64_t)next + lane];
-                code_next = A.b.pair_img[p0 + (int64_t)next + lane];
+            pn = pair_of((int64_t)next);
+            const bool have_next = (int64_t)next < K && pn < P;
+            if (have_next) {
+                q_next = A.b.pairs[p0 + pn];
+                code_next = A.b.pair_img[p0 + pn];
             }
             // ---- stage 0 (lane = pair): the sub-segment to registers, its column walk to LDS ----
             SegRec me;
             RowBand rb;
             int32_t b0 = 0, b1 = -1, c0 = 0, c1 = -1;
-            bool have = pi < p1;
+            bool have = pi < P;
             {
                 SubSeg sg;
                 ColWalk w;
@@ -818,7 +841,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                 }
             }
             wave_sync_lds();
-            if (p0 + (int64_t)next + lane < p1) {   // q_next has arrived by now
+            if (have_next) {   // q_next has arrived by now
                 s_next = g.sub_sec[q_next];
                 l_next = g.sub_tube[q_next];
             }
@@ -840,8 +863,8 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
         __syncthreads();
         // the brick's cells are owned by this workgroup: plain stores (or read-add-store), once
         const int64_t N3 = (int64_t)N * N * N;
-        for (int i = threadIdx.x; i < kCells; i += kDepThreads) {
-            const int x = i % kBrickX, y = (i / kBrickX) % kBrickY, z = i / (kBrickX * kBrickY);
+        for (int i = threadIdx.x; i < ncell; i += kDepThreads) {
+            const int x = i & (bdx - 1), y = (i >> lbx) & (bdy - 1), z = i >> (lbx + lby);
             if (x < ex && y < ey && z < ez) {
                 const int64_t idx = ri.field_off + (ox + x) + (int64_t)N * ((oy + y) + (int64_t)N * (oz + z));
 #pragma unroll

@@ -22,6 +22,19 @@ int64_t nek_of(int N) {
     float v = (float)(3 * N * N) / 12.0f;  // get_spectrum: int(sqrt((nx^2+ny^2+nz^2)/12.0))
     return (int64_t)std::sqrt(v);
 }
+
+bool dep_tiles_valid(const int32_t t[4]) {
+    for (int a = 0; a < 3; a++)
+        if (t[a] < kBrickMin || t[a] > kBrickMax || (t[a] & (t[a] - 1))) return false;
+    return (int64_t)t[0] * t[1] * t[2] <= kBrickCells && t[3] >= 1 && t[3] <= kColCapMax;
+}
+
+void dep_tiles_of(const wt_ctx *c, int N, int32_t out[4]) {
+    for (int k = 0; k < 4; k++) out[k] = kDefaultTiles[k];
+    for (size_t i = 0; i + 5 <= c->dep_tiles.size(); i += 5)
+        if (c->dep_tiles[i] == N)
+            for (int k = 0; k < 4; k++) out[k] = c->dep_tiles[i + 1 + k];
+}
 }  // namespace wt
 
 static thread_local char g_create_err[256] = "";
@@ -366,7 +379,14 @@ static int upload_tubes(wt_ctx *c) {
         r.dx = wt::kPi2 / I.N;  // lx/nx (tube.f90:274)
         r.start = -wt::kPi;
         r.field_off = I.field_off;
-        for (int a = 0; a < 3; a++) r.nb[a] = (I.N + wt::brick_dim(a) - 1) / wt::brick_dim(a);
+        int32_t tiles[4];
+        wt::dep_tiles_of(c, I.N, tiles);
+        for (int a = 0; a < 3; a++) {
+            r.bd[a] = tiles[a];
+            r.lbd[a] = __builtin_ctz((unsigned)tiles[a]);
+            r.nb[a] = (I.N + tiles[a] - 1) / tiles[a];
+        }
+        r.colcap = tiles[3];
         r.brick0 = brick0;
         brick0 += (int64_t)r.nb[0] * r.nb[1] * r.nb[2];
         // fixed-point scale of the FP32 accumulation: |omega| <= sum Gamma/(2 pi sigma^2) over the
@@ -651,12 +671,17 @@ static hipError_t capture_spectral(wt_ctx *c) {
     return hipSuccess;
 }
 
-// Autotune cache file: one JSON object, key "<arch>:<precision>:<N>" -> {"ppb": .., "tx": ..,
-// "wy": .., "ty": .., "wz": .., "tz": ..}. Entries of other devices, precisions and sizes are kept.
+// Autotune cache file: one JSON object of flat integer records. FFT tiles: key
+// "<arch>:<precision>:<N>" -> {"ppb", "tx", "wy", "ty", "wz", "tz"}; deposition tiles: key
+// "<arch>:<precision>:dep:<N>:r<class>" -> {"bx", "by", "bz", "colcap"}, where the class is the
+// rounded log2 of the instance's point-weighted mean cut-off radius in cells. Entries of other
+// devices, precisions, sizes and radius classes are kept.
 static const char *kTileFields[6] = {"ppb", "tx", "wy", "ty", "wz", "tz"};
+static const char *kDepFields[4] = {"bx", "by", "bz", "colcap"};
+using CacheEntry = std::pair<std::string, std::vector<std::pair<std::string, int>>>;
 
-static std::vector<std::pair<std::string, std::vector<int>>> read_tile_cache(const char *path) {
-    std::vector<std::pair<std::string, std::vector<int>>> out;
+static std::vector<CacheEntry> read_tile_cache(const char *path) {
+    std::vector<CacheEntry> out;
     FILE *f = fopen(path, "rb");
     if (!f) return out;
     std::string txt;
@@ -672,18 +697,232 @@ static std::vector<std::pair<std::string, std::vector<int>>> read_tile_cache(con
         size_t b0 = txt.find('{', k1);
         size_t b1 = txt.find('}', b0);
         if (k1 == std::string::npos || b0 == std::string::npos || b1 == std::string::npos) break;
-        std::string key = txt.substr(k0 + 1, k1 - k0 - 1), body = txt.substr(b0, b1 - b0);
-        std::vector<int> v;
-        for (const char *fld : kTileFields) {
-            size_t q = body.find(std::string("\"") + fld + "\"");
+        CacheEntry e;
+        e.first = txt.substr(k0 + 1, k1 - k0 - 1);
+        const std::string body = txt.substr(b0 + 1, b1 - b0 - 1);
+        for (size_t q = body.find('"'); q != std::string::npos;) {
+            const size_t q1 = body.find('"', q + 1), colon = body.find(':', q1);
             int x = 0;
-            if (q == std::string::npos || sscanf(body.c_str() + body.find(':', q) + 1, "%d", &x) != 1) break;
-            v.push_back(x);
+            if (q1 == std::string::npos || colon == std::string::npos || sscanf(body.c_str() + colon + 1, "%d", &x) != 1)
+                break;
+            e.second.emplace_back(body.substr(q + 1, q1 - q - 1), x);
+            q = body.find('"', colon + 1);   // values are integers: the next quote opens the next name
         }
-        if (v.size() == 6) out.emplace_back(key, v);
+        out.push_back(e);
         p = b1;
     }
     return out;
+}
+
+// The values of `fields` in entry e, in that order; false if one is missing.
+static bool cache_values(const CacheEntry &e, const char *const *fields, int nf, std::vector<int> &v) {
+    v.clear();
+    for (int i = 0; i < nf; i++) {
+        bool found = false;
+        for (const auto &kv : e.second)
+            if (kv.first == fields[i]) { v.push_back(kv.second); found = true; break; }
+        if (!found) return false;
+    }
+    return true;
+}
+
+static void cache_put(std::vector<CacheEntry> &entries, const std::string &key, const char *const *fields,
+                      const int32_t *v, int nf) {
+    CacheEntry e;
+    e.first = key;
+    for (int i = 0; i < nf; i++) e.second.emplace_back(fields[i], v[i]);
+    for (auto &en : entries)
+        if (en.first == key) { en = e; return; }
+    entries.push_back(e);
+}
+
+static bool write_tile_cache(const char *path, const std::vector<CacheEntry> &entries) {
+    FILE *f = fopen(path, "wb");
+    if (!f) return false;
+    fprintf(f, "{\n");
+    for (size_t i = 0; i < entries.size(); i++) {
+        fprintf(f, "  \"%s\": {", entries[i].first.c_str());
+        for (size_t j = 0; j < entries[i].second.size(); j++)
+            fprintf(f, "%s\"%s\": %d", j ? ", " : "", entries[i].second[j].first.c_str(), entries[i].second[j].second);
+        fprintf(f, "}%s\n", i + 1 < entries.size() ? "," : "");
+    }
+    fprintf(f, "}\n");
+    fclose(f);
+    return true;
+}
+
+
+// Re-lays out the bricks of every instance after a change of the deposition tiles: the run table
+// here and on the device now, the brick table and pair lists at the next binning.
+static int apply_dep_tiles(wt_ctx *c) {
+    drop_gen_graph(c);
+    c->have_bins = false;
+    if (c->runs_host.size() != (size_t)c->ninst()) return WT_OK;   // tubes not uploaded yet
+    int64_t brick0 = 0;
+    for (int i = 0; i < c->ninst(); i++) {
+        wt::Run &r = c->runs_host[i];
+        int32_t t[4];
+        wt::dep_tiles_of(c, r.N, t);
+        for (int a = 0; a < 3; a++) {
+            r.bd[a] = t[a];
+            r.lbd[a] = __builtin_ctz((unsigned)t[a]);
+            r.nb[a] = (r.N + t[a] - 1) / t[a];
+        }
+        r.colcap = t[3];
+        r.brick0 = brick0;
+        brick0 += (int64_t)r.nb[0] * r.nb[1] * r.nb[2];
+    }
+    c->run = c->runs_host[0];
+    HIPCALL(c, hipMemcpyAsync(c->runs_dev, c->runs_host.data(), sizeof(wt::Run) * c->ninst(), hipMemcpyHostToDevice,
+                              c->stream), "runs");
+    HIPCALL(c, hipStreamSynchronize(c->stream), "runs");
+    return WT_OK;
+}
+
+static void set_tiles_of(wt_ctx *c, int N, const int32_t t[4]) {
+    for (size_t i = 0; i + 5 <= c->dep_tiles.size(); i += 5)
+        if (c->dep_tiles[i] == N) {
+            std::copy(t, t + 4, c->dep_tiles.begin() + i + 1);
+            return;
+        }
+    c->dep_tiles.push_back(N);
+    c->dep_tiles.insert(c->dep_tiles.end(), t, t + 4);
+}
+
+int wt_set_deposit_tiles(wt_ctx *c, int32_t N, const int32_t *tiles) {
+    CHECK_CTX(c);
+    if (!tiles || !wt::dep_tiles_valid(tiles))
+        return set_err(c, WT_EINVAL, "deposition tiles: brick edges powers of two in [%d, %d], at most %d cells, "
+                       "1 <= colcap <= %d", wt::kBrickMin, wt::kBrickMax, wt::kBrickCells, wt::kColCapMax);
+    if (N < 0) return set_err(c, WT_EINVAL, "N must be >= 0");
+    bool any = false;
+    for (const wt::Instance &I : c->inst)
+        if (N == 0 || I.N == N) {
+            set_tiles_of(c, I.N, tiles);
+            any = true;
+        }
+    if (!any) return set_err(c, WT_EINVAL, "no instance of the context has N = %d", N);
+    return apply_dep_tiles(c);
+}
+
+int wt_get_deposit_tiles(wt_ctx *c, int32_t N, int32_t *tiles) {
+    CHECK_CTX(c);
+    if (!tiles) return set_err(c, WT_EINVAL, "NULL tiles");
+    wt::dep_tiles_of(c, N > 0 ? N : c->N, tiles);
+    return WT_OK;
+}
+
+// Radius class of the instances of grid size N: rounded log2 of the point-weighted mean cut-off
+// radius Rtube_ratio * sigma_max in cells.
+static int radius_class(const wt_ctx *c, int N) {
+    double w = 0.0, s = 0.0;
+    for (const wt::Instance &I : c->inst) {
+        if (I.N != N) continue;
+        const double ratio = std::max(1.0, I.params.sigma_ratio);
+        for (size_t l = 0; l < I.sigma.size(); l++) {
+            w += I.npoint[l];
+            s += I.npoint[l] * I.params.Rtube_ratio * ratio * I.sigma[l] * N / wt::kPi2;
+        }
+    }
+    return w > 0.0 ? (int)std::lround(std::log2(std::max(s / w, 1e-3))) : 0;
+}
+
+// Binning plus deposition of the current tubes with the current tiles, in ms (best of two).
+static int time_bin_deposit(wt_ctx *c, double *ms) {
+    hipEvent_t e0, e1;
+    HIPCALL(c, hipEventCreate(&e0), "event");
+    HIPCALL(c, hipEventCreate(&e1), "event");
+    double best = 1e30;
+    int rc = WT_OK;
+    for (int rep = 0; rep < 2 && rc == WT_OK; rep++) {
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = set_err(c, WT_EDEVICE, "event"); break; }
+        if ((rc = wt_bin(c)) || (rc = wt_deposit(c, nullptr))) break;
+        float t = 0.0f;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&t, e0, e1) != hipSuccess) { rc = set_err(c, WT_EDEVICE, "event"); break; }
+        best = std::min(best, (double)t);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = best;
+    return rc;
+}
+
+// Chooses the deposition tiles of every grid size of the context on its tubes (or takes them from
+// the cache): brick shapes of 2^9..2^12 cells, then the column cap of the fastest, one grid size
+// at a time with the others kept. The defaults are timed first; another choice is kept only if it
+// is more than 0.5% faster than the best so far.
+static int deposit_autotune(wt_ctx *c, const std::string &pre, std::vector<CacheEntry> &entries,
+                            std::string &rep) {
+    static const int32_t kShapes[][3] = {{16, 16, 16}, {32, 16, 8}, {16, 32, 8}, {8, 16, 32}, {32, 32, 4},
+                                          {16, 16, 8}, {8, 8, 8}};
+    static const int32_t kCaps[] = {4, 8, 16};
+    std::vector<int> Ns;
+    for (const wt::Instance &I : c->inst)
+        if (std::find(Ns.begin(), Ns.end(), I.N) == Ns.end()) Ns.push_back(I.N);
+    int rc = wt_build_segments(c, nullptr);
+    if (rc) return rc;
+    for (int N : Ns) {
+        const std::string key = pre + "dep:" + std::to_string(N) + ":r" + std::to_string(radius_class(c, N));
+        std::vector<int> v;
+        bool hit = false;
+        for (const auto &e : entries)
+            if (e.first == key && cache_values(e, kDepFields, 4, v)) {
+                int32_t t[4] = {v[0], v[1], v[2], v[3]};
+                if (wt::dep_tiles_valid(t)) {
+                    set_tiles_of(c, N, t);
+                    if ((rc = apply_dep_tiles(c))) return rc;
+                    hit = true;
+                }
+            }
+        char line[256];
+        if (hit) {
+            snprintf(line, sizeof line, "deposit N=%d (%s) from cache: %dx%dx%d, colcap %d\n", N,
+                     key.c_str() + pre.size(), v[0], v[1], v[2], v[3]);
+            rep += line;
+            continue;
+        }
+        int32_t cur[4];
+        wt::dep_tiles_of(c, N, cur);
+        int32_t best[4] = {cur[0], cur[1], cur[2], cur[3]};
+        double best_ms = 1e30;
+        auto trial = [&](const int32_t *t) -> int {
+            set_tiles_of(c, N, t);
+            int r = apply_dep_tiles(c);
+            double ms = 0.0;
+            if (!r) r = time_bin_deposit(c, &ms);
+            if (r) return r;
+            if (ms < 0.995 * best_ms) {   // a candidate must beat the best so far by more than timing noise
+                best_ms = ms;
+                std::copy(t, t + 4, best);
+            }
+            return WT_OK;
+        };
+        const double ms_default = [&]() {
+            int32_t t[4] = {wt::kDefaultTiles[0], wt::kDefaultTiles[1], wt::kDefaultTiles[2], wt::kDefaultTiles[3]};
+            rc = trial(t);
+            return best_ms;
+        }();
+        if (rc) return rc;
+        for (const auto &sh : kShapes) {
+            int32_t t[4] = {sh[0], sh[1], sh[2], wt::kDefaultTiles[3]};
+            if (std::equal(t, t + 3, wt::kDefaultTiles)) continue;
+            if ((rc = trial(t))) return rc;
+        }
+        const int32_t shape[3] = {best[0], best[1], best[2]};
+        for (int cap : kCaps) {
+            if (cap == wt::kDefaultTiles[3]) continue;
+            int32_t t[4] = {shape[0], shape[1], shape[2], cap};
+            if ((rc = trial(t))) return rc;
+        }
+        set_tiles_of(c, N, best);
+        if ((rc = apply_dep_tiles(c))) return rc;
+        cache_put(entries, key, kDepFields, best, 4);
+        snprintf(line, sizeof line, "deposit N=%d (%s): %dx%dx%d, colcap %d: binning + deposit %.2f ms (default %.2f ms)\n",
+                 N, key.c_str() + pre.size(), best[0], best[1], best[2], best[3], best_ms, ms_default);
+        rep += line;
+    }
+    return WT_OK;
 }
 
 int wt_autotune(wt_ctx *c, const char *cache_path, char *report, int32_t report_cap) {
@@ -693,13 +932,16 @@ int wt_autotune(wt_ctx *c, const char *cache_path, char *report, int32_t report_
     std::string arch = prop.gcnArchName;
     if (arch.find(':') != std::string::npos) arch = arch.substr(0, arch.find(':'));
     const std::string pre = arch + ":" + std::to_string(c->precision) + ":";
-    auto entries = cache_path ? read_tile_cache(cache_path) : decltype(read_tile_cache(nullptr)){};
+    std::vector<CacheEntry> entries = cache_path ? read_tile_cache(cache_path) : std::vector<CacheEntry>{};
     std::vector<int32_t> cached;
-    for (auto &e : entries)
-        if (e.first.compare(0, pre.size(), pre) == 0) {
+    for (const auto &e : entries) {
+        std::vector<int> v;
+        if (e.first.compare(0, pre.size(), pre) == 0 && e.first.find("dep:") == std::string::npos &&
+            cache_values(e, kTileFields, 6, v)) {
             cached.push_back(atoi(e.first.c_str() + pre.size()));
-            cached.insert(cached.end(), e.second.begin(), e.second.end());
+            cached.insert(cached.end(), v.begin(), v.end());
         }
+    }
     std::string rep;
     hipError_t e = wt::spectral_autotune(c, cached, rep);
     if (e != hipSuccess) return hip_err(c, e, "autotune");
@@ -710,26 +952,17 @@ int wt_autotune(wt_ctx *c, const char *cache_path, char *report, int32_t report_
     c->spec_exec = nullptr;
     c->spec_graph = nullptr;
     c->have_spectra = false;
-    if (cache_path) {
-        for (size_t i = 0; i + 7 <= c->fft_tiles.size(); i += 7) {
-            const std::string key = pre + std::to_string(c->fft_tiles[i]);
-            std::vector<int> v(c->fft_tiles.begin() + i + 1, c->fft_tiles.begin() + i + 7);
-            bool found = false;
-            for (auto &en : entries)
-                if (en.first == key) { en.second = v; found = true; }
-            if (!found) entries.emplace_back(key, v);
-        }
-        FILE *f = fopen(cache_path, "wb");
-        if (!f) return set_err(c, WT_EINVAL, "cannot write the autotune cache %s", cache_path);
-        fprintf(f, "{\n");
-        for (size_t i = 0; i < entries.size(); i++) {
-            fprintf(f, "  \"%s\": {", entries[i].first.c_str());
-            for (int j = 0; j < 6; j++) fprintf(f, "%s\"%s\": %d", j ? ", " : "", kTileFields[j], entries[i].second[j]);
-            fprintf(f, "}%s\n", i + 1 < entries.size() ? "," : "");
-        }
-        fprintf(f, "}\n");
-        fclose(f);
+    for (size_t i = 0; i + 7 <= c->fft_tiles.size(); i += 7)
+        cache_put(entries, pre + std::to_string(c->fft_tiles[i]), kTileFields, &c->fft_tiles[i + 1], 6);
+    bool tubes = !c->inst.empty();
+    for (const wt::Instance &I : c->inst) tubes = tubes && I.have_tubes && I.have_params;
+    if (tubes && !batch_has_filters(c)) {
+        int rc = deposit_autotune(c, pre, entries, rep);
+        if (rc) return rc;
+        c->have_omega = c->have_u = false;
     }
+    if (cache_path && !write_tile_cache(cache_path, entries))
+        return set_err(c, WT_EINVAL, "cannot write the autotune cache %s", cache_path);
     if (report && report_cap > 0) snprintf(report, (size_t)report_cap, "%s", rep.c_str());
     return WT_OK;
 }

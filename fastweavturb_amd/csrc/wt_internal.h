@@ -30,14 +30,21 @@ struct Run {
     double start;               // -pi
     int64_t field_off;          // element offset of the instance's 3*N^3 field block
     int64_t brick0;             // first global brick of the instance
-    int32_t nb[3];              // bricks per axis, ceil(N / kBrickDim[axis])
+    int32_t nb[3];              // bricks per axis, ceil(N / bd[axis])
+    int32_t bd[3], lbd[3];      // deposition brick: cells per axis (powers of two) and their log2
+    int32_t colcap;             // deposition: columns per row segment and pass
     int32_t fx_shift;           // FP32 mode: the instance's accumulator scale 2^fx_shift, from its own
     double fx_scale, fx_inv;    // tubes only, so a batched instance rounds like a single run
 };
 
-// The deposition accumulates one brick of kBrickX * kBrickY * kBrickZ cells at a time in LDS.
-constexpr int kBrickX = 16, kBrickY = 16, kBrickZ = 16;
-__host__ __device__ constexpr int brick_dim(int axis) { return axis == 0 ? kBrickX : (axis == 1 ? kBrickY : kBrickZ); }
+// The deposition accumulates one brick of Run::bd[0] x bd[1] x bd[2] cells at a time in LDS:
+// powers of two from kBrickMin to kBrickMax per axis and at most kBrickCells cells; the column
+// stage walks at most Run::colcap columns of a row segment per pass (1..kColCapMax). Defaults
+// apply until wt_set_deposit_tiles or wt_autotune choose others for a grid size.
+constexpr int kBrickCells = 4096;
+constexpr int kBrickMin = 4, kBrickMax = 64;
+constexpr int kColCapMax = 64;
+constexpr int kDefaultTiles[4] = {16, 16, 16, 8};   // bx, by, bz, colcap
 
 // Device-side view of the tube table and geometry (all pointers are device memory).
 struct Geo {
@@ -80,6 +87,7 @@ struct Bins {
     int64_t nbricks;
     int4 *brick_tab;            // [nbricks] (instance, bx, by, bz)
     int64_t *brick_off;         // [nbricks+1]
+    uint32_t *brick_wide;       // [nbricks] leading pairs of each brick's list that come from wide boxes
     uint32_t *pairs;            // [npairs]
     uint8_t *pair_img;          // [npairs] periodic image of the brick, kx + 3ky + 9kz (k = 0..2)
     int64_t npairs;
@@ -95,7 +103,8 @@ struct SegWs {
     int64_t *scan = nullptr;            // scan workspace of the sub-point layout
 };
 struct BinWs {
-    uint32_t *cnt = nullptr;            // [nbricks] pairs per brick
+    uint32_t *cnt = nullptr;            // [nbricks] pairs per brick (fill pass: slots after the wide ones)
+    uint32_t *wslot = nullptr;          // [nbricks] fill pass: slots of the wide boxes' pairs
     int32_t *err = nullptr;             // [1]
     uint32_t *big = nullptr;            // wide boxes expanded by a wave each
     uint32_t *nbig = nullptr;           // [1]
@@ -189,6 +198,8 @@ struct wt_ctx {
     // FFT tile shapes chosen by wt_autotune, 7 ints per grid size: N, x lines per block, x threads,
     // y columns per tile, y threads, z columns per tile, z threads (defaults for other N)
     std::vector<int32_t> fft_tiles;
+    // deposition tiles per grid size, 5 ints each: N, bx, by, bz, colcap (kDefaultTiles for others)
+    std::vector<int32_t> dep_tiles;
     // batched structure functions: work items and per-position separations
     void *sf_items = nullptr;
     int64_t sf_nitems = 0;
@@ -220,6 +231,9 @@ hipError_t launch_seg_stats(wt_ctx *c, wt_seg_stats *out);
 // deposit.hip: accumulate = false writes omega, true adds to it
 hipError_t launch_deposit(wt_ctx *c, bool accumulate = false);
 hipError_t replay_deposit(wt_ctx *c);        // the deposit kernel alone, counters left on the device
+// the deposition tiles of grid size N (bx, by, bz, colcap)
+void dep_tiles_of(const wt_ctx *c, int N, int32_t out[4]);
+bool dep_tiles_valid(const int32_t t[4]);
 // fft.hip: mode 0 = Biot-Savart + normalisation of every instance; mode 1 = spectrum of the
 // resident velocity of instance `only`
 hipError_t launch_spectral(wt_ctx *c, int mode, int only);

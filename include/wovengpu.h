@@ -200,11 +200,27 @@ int wt_set_graph(wt_ctx *ctx, int32_t enable);
 
 /* Autotuner (no counterpart in the reference): times the candidate tile shapes of the FFT passes
  * (x lines per block, y and z columns per tile, threads per block) for every grid size of the
- * context on the device and keeps the fastest for later calls. With cache_path, shapes already
- * recorded there for this GPU architecture and precision are reused without measuring, and the
- * chosen shapes are written back (JSON). report (may be NULL) receives one line per grid size.
- * The half spectrum is overwritten; call before wt_generate / wt_biot_savart. */
+ * context on the device and keeps the fastest for later calls. When every instance has its
+ * parameters and tubes (and none filters), it also times the binning plus deposition of those
+ * tubes for the candidate deposition tiles of each grid size (brick shapes of 8^3..16^3 cells in
+ * several aspect ratios, then columns per pass 4, 8, 16 for the fastest shape; the other grid
+ * sizes keep theirs meanwhile) and keeps the fastest (wt_set_deposit_tiles). With cache_path,
+ * shapes already recorded there for this GPU architecture and precision (deposition tiles also
+ * for the grid size and the tubes' radius class) are reused without measuring, and the chosen
+ * shapes are written back (JSON). report (may be NULL) receives one line per grid size and stage.
+ * The half spectrum, and after a deposition tuning the resident fields, are overwritten; call
+ * before wt_generate / wt_biot_savart. */
 int wt_autotune(wt_ctx *ctx, const char *cache_path, char *report, int32_t report_cap);
+
+/* Deposition tiles (no counterpart in the reference) of the instances with grid size N (N = 0:
+ * every instance): tiles = {bx, by, bz, colcap}, the cells of a brick that one workgroup
+ * accumulates in LDS (powers of two from 4 to 64, at most 4096 cells) and the columns of a row
+ * segment walked per pass (1..64). Defaults {16, 16, 16, 8}. The field does not depend on them:
+ * FP32 results are bitwise the same for every choice, FP64 results equal up to the order of the
+ * brick's additions. A change re-bins at the next deposition. wt_get_deposit_tiles reads those of
+ * grid size N (0: the selected instance's). */
+int wt_set_deposit_tiles(wt_ctx *ctx, int32_t N, const int32_t *tiles);
+int wt_get_deposit_tiles(wt_ctx *ctx, int32_t N, int32_t *tiles);
 
 /* Replaces cal_pdf / export_pdf (subroutine.f90:1-62) as called at tube.f90:1166-1185: the PDF
  * of all three components of the resident omega (field 0) or u (field 1), taken as FP32 values

@@ -17,10 +17,10 @@ def _brick_images(lo, hi, N, bd=16):
     return out
 
 
-def slab_pairs(geo, b, N, bd=16):
+def slab_pairs(geo, b, N, bd=(16, 16, 16)):
     """The binning's (sub-segment, brick, image) pairs, restated on the host from the oracle's
     boxes and chords: every brick image the box meets whose part of the box can hold a cell with
-    t0 >= 0 and t1 < 0 (corner extremes, 1e-12 margin). Returns (kept count, pruned list of
+    t0 >= 0 and t1 < 0 (corner extremes, 1e-12 margin); bd: the brick edge per axis. Returns (kept count, pruned list of
     (segment, per-axis clipped ranges))."""
     dx, start = 2 * np.pi / N, -np.pi
     kept, pruned = 0, []
@@ -29,7 +29,7 @@ def slab_pairs(geo, b, N, bd=16):
             continue
         q = geo.sp_base[geo.seg_sec[s]] + geo.seg_j[s]
         t0v, t1v = geo.tci[q], geo.tci[q + 1]
-        axes = [_brick_images(b["lo"][s][c], b["hi"][s][c], N, bd) for c in range(3)]
+        axes = [_brick_images(b["lo"][s][c], b["hi"][s][c], N, bd[c]) for c in range(3)]
         for ex in axes[0]:
             for ey in axes[1]:
                 for ez in axes[2]:

@@ -1,6 +1,7 @@
-"""FFT tile autotuner (wt_autotune): the tuned shapes give the same fields (the per-line FFT
-arithmetic is unchanged; only the grouping of the spectral partial sums moves, so up differs in
-the last bits), and the JSON cache round-trips."""
+"""Autotuner (wt_autotune): the tuned FFT and deposition tiles give the same fields (the per-line
+FFT arithmetic is unchanged; only the grouping of the spectral partial sums moves, so up differs in
+the last bits; the deposition's fixed-point sums do not depend on the bricks), and the JSON cache
+round-trips."""
 import json
 
 import numpy as np
@@ -47,4 +48,6 @@ def test_tuned_shapes_give_the_same_fields(tmp_path):
     # a second context reads the cache instead of measuring
     h = _ctx()
     rep2 = h.autotune(path)
-    assert rep2.count("from cache") == len(set(SIZES))
+    assert rep2.count("from cache") == 2 * len(set(SIZES))      # FFT and deposition tiles
+    dep = [k for k in tab if ":32:dep:" in k]
+    assert len(dep) == len(set(SIZES)) and all(set(tab[k]) == {"bx", "by", "bz", "colcap"} for k in dep)

@@ -21,7 +21,7 @@ from fastweavturb_amd import fbb
 pytestmark = pytest.mark.gpu
 
 TILES = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fastweavturb_amd", "data",
-                     "fft_tiles.json")
+                     "tiles.json")
 
 
 def rel_l2(a, b):
