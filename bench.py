@@ -321,6 +321,15 @@ def ensemble_bench(steps: int, warmup: int, n: int = 64, graph: bool = True, til
         g.generate()
         stage += g.stage_times()[:4]
     dt = (time.perf_counter() - t0) / steps
+    # the same generations replayed from one hipGraph of the whole ensemble (wt_set_graph mode 2)
+    g.set_graph(2)
+    g.generate()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.generate()
+    dt_graph = (time.perf_counter() - t0) / steps
+    g.set_graph(graph)
+    log(f"[c4] eager {dt * 1e3:.1f} ms, whole-ensemble graph {dt_graph * 1e3:.1f} ms per generation")
     seps = np.stack([S.flat_separations(N) for N in sizes])
     t1 = time.perf_counter()
     E, Sf = g.batch_statistics(2, 6, seps)
@@ -339,6 +348,7 @@ def ensemble_bench(steps: int, warmup: int, n: int = 64, graph: bool = True, til
                        "graph": graph},
             "mpts_per_s": pts / dt / 1e6, "stages_ms": dict(zip(["segments", "bin", "deposit", "biot_savart"],
                                                                  (stage / steps).tolist())),
+            "graph_ms_per_step": dt_graph * 1e3, "graph_instances_per_s": n / dt_graph,
             "statistics_ms": t_stats * 1e3, "sub_segments": int(seg["nseg_all"]), "hits": int(dep["hits"]),
             "features_first": feats[0].round(4).tolist()}
 

@@ -102,3 +102,23 @@ def test_bank_is_continuous_in_density_and_hurst(gctx):
     t3 = bank.tubes(S.family_design(1.3, 1.0, 0.7001, nlev=3), gctx)
     d = (t3.cx - t1.cx + np.pi) % (2 * np.pi) - np.pi
     assert 0 < np.abs(d).max() < 1e-2
+
+
+def test_shipped_surrogate_reaches_a_512_target():
+    """The inverse design at its stated size: the 48-256-256-256-24 network trained on the shipped
+    generator pairs brings a seeded 512^3 target's 32 log shells within +-5% in at most 6
+    verification runs of 512^3 fields."""
+    bank = S.CenterlineBank(S.PAIRS_RECIPE["bank_seed"])
+    X, Y, M = S.load_pairs()
+    assert len(X) == S.PAIRS_RECIPE["n"] and X.shape[1] == 48 and Y.shape[1] == 24
+    sur = S.Surrogate(seed=1)
+    hist = sur.fit(X, Y, M, epochs=300)
+    assert hist[-1] < 0.2 * hist[0]
+    ver = S.Verifier(bank, 512)
+    try:
+        tgt = ver.run(S.target_designs(1)[0])
+        r = S.surrogate_design(sur, tgt, ver, max_runs=6)
+        print(f"512^3 target: {r['runs']} runs, shell errors {np.round(r['errors'], 4).tolist()}", flush=True)
+        assert r["found"], r["errors"]
+    finally:
+        ver.close()
