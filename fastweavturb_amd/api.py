@@ -107,6 +107,8 @@ def load_library():
     L.wt_filter_spec.argtypes = [vp, i32]
     L.wt_fbb_centerlines.argtypes = [vp, i32, ip, dp, ip, C.c_double, C.c_uint64, C.POINTER(C.c_uint64), dp, dp,
                                      dp, dp]
+    L.wt_fbb_reference_centerlines.argtypes = [vp, i32, ip, dp, dp, C.c_uint64, C.POINTER(C.c_uint64), dp, dp, dp,
+                                               dp]
     L.wt_fbb_embedding.argtypes = [i32]
     L.wt_fbb_embedding.restype = i32
     L.wt_fbb_normals.argtypes = [vp, C.c_uint64, i32, C.c_int64, dp]
@@ -330,6 +332,25 @@ class WovenGPU:
                                                None if k is None else k.ctypes.data_as(C.POINTER(C.c_uint64)),
                                                None if g is None else _dp(g),
                                                _dp(out[0]), _dp(out[1]), _dp(out[2])))
+        return out
+
+    def fbb_reference_centerlines(self, npoint, hurst, dB, seed: int, gauss=None, keys=None):
+        """(cx, cy, cz) of centerlines built as FBB_with_parameters.m builds them (no winding drift,
+        step rms dB[l], tubes of fewer than 128 points sampled from a 128-point path, the wrap
+        mod(x, 2 pi) - pi); random inputs as for fbb_centerlines, with m the embedding of
+        max(npoint[l], 128) points."""
+        npt = np.ascontiguousarray(npoint, dtype=np.int32)
+        H = np.ascontiguousarray(np.broadcast_to(np.asarray(hurst, dtype=np.float64), npt.shape))
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dB, dtype=np.float64), npt.shape))
+        P = int(npt.sum())
+        out = [np.empty(P) for _ in range(3)]
+        g = None if gauss is None else np.ascontiguousarray(gauss, dtype=np.float64)
+        k = None if keys is None else np.ascontiguousarray(keys, dtype=np.uint64)
+        self._check(self._L.wt_fbb_reference_centerlines(self._h, len(npt), _ip(npt), _dp(H), _dp(d),
+                                                         C.c_uint64(int(seed)),
+                                                         None if k is None else k.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                         None if g is None else _dp(g),
+                                                         _dp(out[0]), _dp(out[1]), _dp(out[2])))
         return out
 
     def fbb_normals(self, seed: int, tube: int, count: int) -> np.ndarray:

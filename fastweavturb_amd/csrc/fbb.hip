@@ -8,9 +8,14 @@
 //     fGn samples (x from Re w_0, y from Re w_1, z from Im w_0);
 //   - the bridge B_i - f_i B_{n-1} of the cumulative sums B, f_i = (t^2H + 1 - (1-t)^2H) / 2,
 //     t = (i+1)/n (FBB:92-102);
-//   - step-rms normalisation of the 3-D increments to wiggle * 2 pi |w| / n (FBB:104-112);
-//   - a uniform shift x0 in [-pi, pi)^3, the winding drift 2 pi w i/n along the integer lattice
-//     vector w, and the wrap into [-pi, pi) (FBB:117-120).
+//   - step-rms normalisation of the 3-D increments to dB (FBB:104-112);
+//   - a uniform shift x0 in [-pi, pi)^3 and the wrap into [-pi, pi) (FBB:117-120).
+// Two modes. Winding mode (the benchmark workloads): dB = wiggle * 2 pi |w| / n and a drift
+// 2 pi w i/n along an integer lattice vector w, so that a closed centerline has a mean direction;
+// the wrap is numpy's mod(x + pi, 2 pi) - pi. Reference mode: the construction as the reference
+// writes it, with the level's dB and no drift; a tube of n < 128 points takes its points from a
+// 128-point path at ceil(j 128 / n), j = 1..n (FBB:78-84); the wrap is mod(x, 2 pi) - pi with
+// mod(x, y) = x - floor(x / y) y.
 // Random numbers: Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11) keyed (seed, tube key), blocks
 // at counters 1, 2, ... -- the word stream of numpy's Philox bit generator with that key; a
 // uniform is (word >> 11) 2^-53, normals come from consecutive uniform pairs by Box-Muller. A
@@ -81,9 +86,12 @@ struct FbbTube {
     int64_t gauss_off;   // first of its 4m + 3 inputs in the staged inputs (-1: Philox)
     uint64_t key;        // second Philox key word (the tube's index unless the caller gives keys)
     int32_t n;           // points
-    int32_t pad;
+    int32_t ngen;        // points of the generated path: n, or 128 for a reference-mode tube of n < 128
     double H;
-    double w[3];         // lattice winding
+    double w[3];         // lattice winding (0 in reference mode)
+    double dB;           // step rms of the scaled bridge
+    int32_t ref;         // 1: reference mode
+    int32_t pad;
 };
 
 // ---- kernels ------------------------------------------------------------------------------------
@@ -221,12 +229,59 @@ __device__ __forceinline__ double wrap_pi(double x) {
     return r - 3.141592653589793115997963468544185161590576171875;
 }
 
+// mod(x, 2 pi) - pi as the reference writes it (FBB:120), mod(x, y) = x - floor(x / y) y.
+__device__ __forceinline__ double wrap_ref(double x) {
+#pragma clang fp contract(off)
+    const double b = 6.28318530717958623199592693708837032318115234375;
+    const double q = floor(x / b);
+    return (x - q * b) - 3.141592653589793115997963468544185161590576171875;
+}
+
+// A reference-mode tube of n < 128 points, by one thread: the 128-point path P_k (P_0 = 0, the
+// cumulative fGn samples), its points P at ceil(j 128 / n) - 1 for j = 1..n, the bridge, the scale
+// to dB and the shifted, wrapped positions.
+__device__ void fbb_path_subsampled(const FbbTube &t, const Z *w0, const Z *w1, const double x0[3],
+                                    double *const out[3]) {
+    const int n = t.n, ng = t.ngen - 1;
+    const double r = (double)t.ngen / (double)n;
+    double s[3] = {0.0, 0.0, 0.0};
+    int i = 0;
+    for (int k = 0; k <= ng && i < n; k++) {
+        while (i < n && (int)ceil((double)(i + 1) * r) - 1 == k) {
+            for (int c = 0; c < 3; c++) out[c][i] = s[c];
+            i++;
+        }
+        if (k < ng) {
+            s[0] += w0[k].x;
+            s[1] += w1[k].x;
+            s[2] += w0[k].y;
+        }
+    }
+    const double e = 2.0 * t.H, dn = (double)n;
+    const double last[3] = {out[0][n - 1], out[1][n - 1], out[2][n - 1]};
+    double ss = 0.0, prev[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < n; j++) {
+        const double tt = (double)(j + 1) / dn;
+        const double f = 0.5 * ((pow(tt, e) + 1.0) - pow(1.0 - tt, e));
+        for (int c = 0; c < 3; c++) {
+            const double v = out[c][j] - f * last[c];
+            if (j > 0) ss += (v - prev[c]) * (v - prev[c]);
+            out[c][j] = v;
+            prev[c] = v;
+        }
+    }
+    const double rms = ss / (double)(n - 1);
+    const double scale = rms > 0.0 ? t.dB / sqrt(rms) : 0.0;
+    for (int j = 0; j < n; j++)
+        for (int c = 0; c < 3; c++) out[c][j] = wrap_ref(out[c][j] * scale + x0[c]);
+}
+
 // One block per tube: fGn samples of the two draws, cumulative sums, the bridge, the step-rms
 // scale and the final wrapped positions, written to x, y, z at the tube's points. Thread t owns
 // the points 1 + t*per .. (t+1)*per (thread 0 also point 0) and only ever reads back its own
 // writes; values it needs from other chunks come through LDS.
 __global__ __launch_bounds__(kPathThreads) void k_fbb_path(const Z *__restrict__ w, const FbbTube *__restrict__ tubes,
-                                                          int m, double wiggle, const double *__restrict__ gauss,
+                                                          int m, const double *__restrict__ gauss,
                                                           uint64_t seed, double *__restrict__ X,
                                                           double *__restrict__ Y, double *__restrict__ Zc) {
     __shared__ double part[3][kPathThreads];
@@ -235,7 +290,17 @@ __global__ __launch_bounds__(kPathThreads) void k_fbb_path(const Z *__restrict__
     const FbbTube t = tubes[blockIdx.x];
     const int n = t.n, ng = n - 1;
     const Z *w0 = w + (int64_t)(2 * blockIdx.x) * m, *w1 = w0 + m;
-    double *out[3] = {X + t.pt_off, Y + t.pt_off, Zc + t.pt_off};
+    double *const out[3] = {X + t.pt_off, Y + t.pt_off, Zc + t.pt_off};
+    double x0[3];
+    for (int c = 0; c < 3; c++) {
+        const double u = t.gauss_off >= 0 ? gauss[t.gauss_off + 4 * (int64_t)m + c]
+                                           : philox_uniform(seed, t.key, 4 * (uint64_t)m + c);
+        x0[c] = (u - 0.5) * 2.0 * 3.141592653589793115997963468544185161590576171875;
+    }
+    if (t.ngen != t.n) {   // block-uniform: a short reference-mode tube
+        if (threadIdx.x == 0) fbb_path_subsampled(t, w0, w1, x0, out);
+        return;
+    }
     const int per = (ng + kPathThreads - 1) / kPathThreads;
     const int g0 = min(ng, (int)threadIdx.x * per), g1 = min(ng, g0 + per);
     double s[3] = {0.0, 0.0, 0.0};
@@ -287,21 +352,17 @@ __global__ __launch_bounds__(kPathThreads) void k_fbb_path(const Z *__restrict__
         }
     }
     const double rms = block_sum(ss, red) / (double)(n - 1);
-    const double wn = sqrt((t.w[0] * t.w[0] + t.w[1] * t.w[1]) + t.w[2] * t.w[2]);
-    const double dB = wiggle * (2.0 * 3.141592653589793115997963468544185161590576171875 * wn / dn);
-    const double scale = rms > 0.0 ? dB / sqrt(rms) : 0.0;
-    double x0[3];
-    for (int c = 0; c < 3; c++) {
-        const double u = t.gauss_off >= 0 ? gauss[t.gauss_off + 4 * (int64_t)m + c]
-                                           : philox_uniform(seed, t.key, 4 * (uint64_t)m + c);
-        x0[c] = (u - 0.5) * 2.0 * 3.141592653589793115997963468544185161590576171875;
-    }
+    const double scale = rms > 0.0 ? t.dB / sqrt(rms) : 0.0;
     const int p0 = threadIdx.x == 0 ? 0 : g0 + 1, p1 = g1;      // this thread's points
     for (int i = p0; i <= p1 && i < n; i++) {
         const double tt = (double)i / dn;
         for (int c = 0; c < 3; c++) {
-            const double drift = 2.0 * 3.141592653589793115997963468544185161590576171875 * t.w[c] * tt;
-            out[c][i] = wrap_pi((x0[c] + drift) + out[c][i] * scale);
+            if (t.ref) {
+                out[c][i] = wrap_ref(out[c][i] * scale + x0[c]);
+            } else {
+                const double drift = 2.0 * 3.141592653589793115997963468544185161590576171875 * t.w[c] * tt;
+                out[c][i] = wrap_pi((x0[c] + drift) + out[c][i] * scale);
+            }
         }
     }
 }
@@ -427,19 +488,22 @@ hipError_t fft_batch(Z *d, Z *scratch, int64_t nlines, int m, hipStream_t st, st
 
 }  // namespace
 
+int32_t fbb_generated_points(int32_t npoint, bool ref) { return ref && npoint < 128 ? 128 : npoint; }
+
 hipError_t fbb_centerlines(wt_ctx *c, int ntube, const int32_t *npoint, const double *hurst, const int32_t *winding,
-                           double wiggle, uint64_t seed, const uint64_t *keys, const double *gauss, double *cx,
-                           double *cy, double *cz) {
+                           double wiggle, const double *dB, uint64_t seed, const uint64_t *keys, const double *gauss,
+                           double *cx, double *cy, double *cz) {
     hipStream_t st = c->stream;
+    const bool ref = winding == nullptr;
     std::vector<int64_t> pt(ntube + 1, 0), goff(ntube + 1, 0);
     for (int l = 0; l < ntube; l++) {
         pt[l + 1] = pt[l] + npoint[l];
-        goff[l + 1] = goff[l] + 4 * fbb_embedding(npoint[l]) + 3;
+        goff[l + 1] = goff[l] + 4 * fbb_embedding(fbb_generated_points(npoint[l], ref)) + 3;
     }
     const int64_t P = pt[ntube];
     // tubes grouped by embedding size
     std::map<int, std::vector<int>> groups;
-    for (int l = 0; l < ntube; l++) groups[(int)fbb_embedding(npoint[l])].push_back(l);
+    for (int l = 0; l < ntube; l++) groups[(int)fbb_embedding(fbb_generated_points(npoint[l], ref))].push_back(l);
     double *dxyz = nullptr;
     WT_HIP(hipMallocAsync((void **)&dxyz, sizeof(double) * 3 * (size_t)P, st));
     std::map<int, PlanTab> plans;
@@ -459,9 +523,17 @@ hipError_t fbb_centerlines(wt_ctx *c, int ntube, const int32_t *npoint, const do
                 const int l = ls[a + b];
                 tt[b].pt_off = pt[l];
                 tt[b].n = npoint[l];
+                tt[b].ngen = fbb_generated_points(npoint[l], ref);
                 tt[b].key = keys ? keys[l] : (uint64_t)l;
                 tt[b].H = hurst[l];
-                for (int k = 0; k < 3; k++) tt[b].w[k] = (double)winding[3 * l + k];
+                tt[b].ref = ref ? 1 : 0;
+                for (int k = 0; k < 3; k++) tt[b].w[k] = ref ? 0.0 : (double)winding[3 * l + k];
+                if (ref) {
+                    tt[b].dB = dB[l];
+                } else {
+                    const double wn = std::sqrt((tt[b].w[0] * tt[b].w[0] + tt[b].w[1] * tt[b].w[1]) + tt[b].w[2] * tt[b].w[2]);
+                    tt[b].dB = wiggle * (2.0 * 3.141592653589793115997963468544185161590576171875 * wn / (double)npoint[l]);
+                }
                 tt[b].gauss_off = gauss ? gsum : -1;
                 gsum += 4 * (int64_t)m + 3;
             }
@@ -497,7 +569,7 @@ hipError_t fbb_centerlines(wt_ctx *c, int ntube, const int32_t *npoint, const do
             if (e == hipSuccess) e = fft_batch(draws, scr, 2 * (int64_t)B, m, st, plans, twm);
             if (e == hipSuccess) {
                 hipLaunchKernelGGL(k_fbb_path, dim3(B), dim3(kPathThreads), 0, st, (const Z *)draws, (const FbbTube *)dt, m,
-                                   wiggle, (const double *)dg, seed, dxyz, dxyz + P, dxyz + 2 * P);
+                                   (const double *)dg, seed, dxyz, dxyz + P, dxyz + 2 * P);
                 e = hipGetLastError();
             }
             hipError_t f = hipStreamSynchronize(st);

@@ -1324,8 +1324,26 @@ int wt_fbb_centerlines(wt_ctx *c, int32_t ntube, const int32_t *npoint, const do
         if (!winding[3 * l] && !winding[3 * l + 1] && !winding[3 * l + 2])
             return set_err(c, WT_EINVAL, "tube %d: zero winding vector", l);
     }
-    hipError_t e = wt::fbb_centerlines(c, ntube, npoint, hurst, winding, wiggle, seed, keys, gauss, cx, cy, cz);
+    hipError_t e = wt::fbb_centerlines(c, ntube, npoint, hurst, winding, wiggle, nullptr, seed, keys, gauss, cx, cy, cz);
     if (e != hipSuccess) return hip_err(c, e, "fbb centerlines");
+    return WT_OK;
+}
+
+int wt_fbb_reference_centerlines(wt_ctx *c, int32_t ntube, const int32_t *npoint, const double *hurst,
+                                 const double *dB, uint64_t seed, const uint64_t *keys, const double *gauss, double *cx,
+                                 double *cy, double *cz) {
+    CHECK_CTX(c);
+    if (ntube <= 0 || !npoint || !hurst || !dB || !cx || !cy || !cz)
+        return set_err(c, WT_EINVAL, "need ntube > 0 and non-NULL arrays");
+    for (int l = 0; l < ntube; l++) {
+        if (npoint[l] < 3) return set_err(c, WT_EINVAL, "tube %d has %d points (at least 3)", l, npoint[l]);
+        if (wt::fbb_embedding(wt::fbb_generated_points(npoint[l], true)) > ((int64_t)1 << 20))
+            return set_err(c, WT_ERANGE, "tube %d: %d points exceed the generator's 2^19", l, npoint[l]);
+        if (!(hurst[l] > 0.0 && hurst[l] < 1.0)) return set_err(c, WT_EINVAL, "tube %d: Hurst index outside (0, 1)", l);
+        if (!(dB[l] >= 0.0) || !std::isfinite(dB[l])) return set_err(c, WT_EINVAL, "tube %d: dB must be finite and >= 0", l);
+    }
+    hipError_t e = wt::fbb_centerlines(c, ntube, npoint, hurst, nullptr, 0.0, dB, seed, keys, gauss, cx, cy, cz);
+    if (e != hipSuccess) return hip_err(c, e, "fbb reference centerlines");
     return WT_OK;
 }
 

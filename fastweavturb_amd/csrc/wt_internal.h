@@ -264,10 +264,13 @@ hipError_t launch_pdf(wt_ctx *c, int field, int nbins, float *val, float *pdf, i
 // x-increment structure functions of every instance in one launch: d_seps[pos*nsep + m] per arena
 // position, d_out[(pos*np + p)*nsep + m]
 hipError_t launch_sf_batch(wt_ctx *c, int pmin, int pmax, int nsep, const int32_t *d_seps, double *d_out);
-// fbb.hip: fBb centerlines of ntube tubes into host arrays (gauss: fixed random inputs or NULL)
+// fbb.hip: fBb centerlines of ntube tubes into host arrays (gauss: fixed random inputs or NULL);
+// winding mode with winding[3 ntube] and wiggle, reference mode (winding NULL) with dB[ntube]
 hipError_t fbb_centerlines(wt_ctx *c, int ntube, const int32_t *npoint, const double *hurst, const int32_t *winding,
-                           double wiggle, uint64_t seed, const uint64_t *keys, const double *gauss, double *cx,
-                           double *cy, double *cz);
+                           double wiggle, const double *dB, uint64_t seed, const uint64_t *keys, const double *gauss,
+                           double *cx, double *cy, double *cz);
+// points of a tube's generated path (128 for a reference-mode tube of fewer points)
+int32_t fbb_generated_points(int32_t npoint, bool ref);
 hipError_t fbb_normals(wt_ctx *c, uint64_t seed, uint64_t tube, int64_t count, double *host);
 int64_t fbb_embedding(int64_t npoint);
 // scan.hip: exclusive prefix sums with a caller-held workspace of scan_scratch_elems(n) int64, and

@@ -320,3 +320,157 @@ def woven_levels_device(g, N: int, levels: list, seed: int, gamma_jitter: float 
                   ntf_pre=ntf_pre, Rtube_ratio=Rtube_ratio, dzeta_sup=dzeta_sup)
     tb.validate()
     return p, tb
+
+
+# ---- the reference's own construction (FBB_with_parameters.m) ------------------------------------
+@dataclasses.dataclass
+class ReferenceFamily:
+    """The header of FBB_with_parameters.m (lines 5-22), with its values as defaults: grid N,
+    Kolmogorov scale eta, levels Nv, core-to-eta ratio c_e_s, target vortex volume fraction
+    rho_w_obj, tube-count ratio r_nL, sigma_ratio, step-to-core ratio c_B, scale ratio r_s, Hurst
+    index H, spectral exponent r_E, first circulation Gamma_1, first n_T."""
+    N: int = 256
+    eta: float = 0.00888684
+    Nv: int = 3
+    c_e_s: float = 1.7
+    rho_w_obj: float = 0.024
+    r_nL: float = 4.0
+    sigma_ratio: float = 4.0
+    c_B: float = 40.0
+    r_s: float = 0.5
+    H: float = 0.834
+    r_E: float = 5.0 / 3.0
+    Gamma_1: float = 1.0
+    n_T_1: int = 1
+
+
+def _mround(x: float) -> int:
+    """MATLAB's round: half away from zero."""
+    return int(math.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
+
+
+def reference_levels(f: ReferenceFamily) -> dict:
+    """Per level i = 1..Nv of FBB_with_parameters.m:25-48 and 188-193: tube count m_line, points Np,
+    step rms dB, core sigma, circulation Gamma and n_T, in the script's FP64 arithmetic."""
+    r_line = math.floor(f.r_nL / f.r_s + 0.5)
+    r_Np = math.floor(f.r_nL / f.r_s / r_line)
+    r_rho = f.r_s * f.r_s * f.r_nL
+    if r_rho != 1:
+        rho_Nv = f.rho_w_obj / (1 - r_rho ** f.Nv) * (1 - r_rho) * r_rho ** (f.Nv - 1)
+    else:
+        rho_Nv = f.rho_w_obj / f.Nv
+    sigma_Nv = f.c_e_s * f.eta
+    sigma_1 = sigma_Nv / (f.r_s ** (f.Nv - 1))
+    dB_Nv = f.c_B * sigma_Nv
+    dB_1 = dB_Nv / f.r_s ** (f.Nv - 1)
+    L_Nv_temp = rho_Nv * (2 * math.pi) ** 3 / sigma_Nv ** 2
+    L_1_temp = L_Nv_temp / f.r_nL ** (f.Nv - 1)
+    Np_1 = _mround(L_1_temp / (1.2 * dB_1))
+    r_Gamma = f.r_s ** ((f.r_E - 1) / 2) * f.r_nL ** (-1 / 2)
+    r_n_T = _mround(1 / f.r_s)
+    i = range(f.Nv)
+    return {"m_line": [int(r_line ** k) for k in i], "Np": [int(Np_1 * r_Np ** k) for k in i],
+            "dB": [float(dB_1 * f.r_s ** k) for k in i], "sigma": [float(sigma_1 * f.r_s ** k) for k in i],
+            "Gamma": [float(f.Gamma_1 * r_Gamma ** k) for k in i], "n_T": [int(f.n_T_1 * r_n_T ** k) for k in i]}
+
+
+def reference_centerline_from(npts: int, H: float, dB: float, zre: np.ndarray, zim: np.ndarray,
+                              u3: np.ndarray) -> np.ndarray:
+    """(3, npts) centerline as FBB_with_parameters.m:73-120 builds it, from given random inputs
+    (normals zre, zim of shape (2, m), m = embedding_size(max(npts, 128) - 1), and 3 uniforms u3):
+    the fBm path P (P_0 = 0) of max(npts, 128) points, for npts < 128 its points at
+    ceil(j 128 / npts) - 1, j = 1..npts (FBB:78-84); the bridge P_j - f_j P_npts (FBB:92-102); the
+    step rms scaled to dB (FBB:104-112); the shift (u - 1/2) 2 pi and mod(x, 2 pi) - pi (FBB:117-120)."""
+    ngen = 128 if npts < 128 else npts
+    g = fgn_from_normals(ngen - 1, H, zre, zim, 3)
+    P = np.concatenate([np.zeros((3, 1)), np.cumsum(g, axis=1)], axis=1)
+    if ngen != npts:
+        P = P[:, np.ceil(np.arange(1, npts + 1, dtype=np.float64) * (ngen / npts)).astype(np.int64) - 1]
+    j = np.arange(1, npts + 1, dtype=np.float64) / npts
+    f = 0.5 * ((j ** (2 * H) + 1.0) - (1.0 - j) ** (2 * H))
+    bb = P - f[None, :] * P[:, -1:]
+    steps = np.diff(bb, axis=1)
+    rms = float(np.sum(steps ** 2) / (npts - 1))
+    bb = bb * (dB / math.sqrt(rms) if rms > 0 else 0.0)
+    x = bb + ((np.asarray(u3, dtype=np.float64) - 0.5) * 2.0 * np.pi)[:, None]
+    return (x - np.floor(x / (2.0 * np.pi)) * (2.0 * np.pi)) - np.pi
+
+
+def philox_inputs(seed: int, key: int, m: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """A tube's random inputs from the stream the GPU generator uses: numpy's Philox4x64-10 keyed
+    (seed, key), uniforms (word >> 11) 2^-53, normals by Box-Muller on consecutive uniform pairs;
+    4m normals (zre: rows 0, 1; zim: rows 0, 1) and then 3 uniforms."""
+    raw = np.random.Philox(key=(int(key) << 64) | int(seed)).random_raw(4 * m + 3)
+    u = (raw >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    r = np.sqrt(-2.0 * np.log(1.0 - u[0:4 * m:2]))
+    z = np.empty(4 * m)
+    z[0::2] = r * np.cos(2 * np.pi * u[1:4 * m:2])
+    z[1::2] = r * np.sin(2 * np.pi * u[1:4 * m:2])
+    return z[:2 * m].reshape(2, m), z[2 * m:].reshape(2, m), u[4 * m:4 * m + 3]
+
+
+def reference_inputs(f: ReferenceFamily, seed: int, g=None) -> tuple[RunParams, Tubes]:
+    """The two input files' contents as FBB_with_parameters.m writes them for family f: tubes level
+    by level, tube l (0-based, in file order) with the random stream keyed (seed, l); centerlines on
+    the GPU of WovenGPU context g (wt_fbb_reference_centerlines) or, with g None, on the host from
+    the same stream. Parameters as the script writes them (FBB:198-243): N_downsample = N,
+    line_s = 1, line_e = all tubes, eta_t = 0, ntf_times 50, ntf_pre 1000, Rtube_ratio 3,
+    dzeta_sup 1e-4, if_filter 0."""
+    lv = reference_levels(f)
+    npoint, dB, sig, gam, nT = [], [], [], [], []
+    for i in range(f.Nv):
+        c = lv["m_line"][i]
+        npoint += [lv["Np"][i]] * c
+        dB += [lv["dB"][i]] * c
+        sig += [lv["sigma"][i]] * c
+        gam += [lv["Gamma"][i]] * c
+        nT += [lv["n_T"][i]] * c
+    nt = len(npoint)
+    if g is not None:
+        cx, cy, cz = g.fbb_reference_centerlines(npoint, [f.H] * nt, dB, seed)
+    else:
+        xs = []
+        for l in range(nt):
+            m = embedding_size(max(npoint[l], 128) - 1)
+            zre, zim, u3 = philox_inputs(seed, l, m)
+            xs.append(reference_centerline_from(npoint[l], f.H, dB[l], zre, zim, u3))
+        cx, cy, cz = np.concatenate(xs, axis=1)
+    t = Tubes(np.array(npoint, dtype=np.int32), np.ascontiguousarray(cx), np.ascontiguousarray(cy),
+              np.ascontiguousarray(cz), np.array(sig), np.array(gam), np.array(nT, dtype=np.int32),
+              np.zeros(nt, dtype=np.int32))
+    p = RunParams(N=f.N, n_d=f.N, line_s=1, line_e=nt, sigma_ratio=f.sigma_ratio, eta_t=0.0, ntf_times=50,
+                  ntf_pre=1000, Rtube_ratio=3.0, dzeta_sup=1e-4)
+    t.validate()
+    return p, t
+
+
+def main(argv=None) -> int:
+    """Writes centerline_input.dat and parameters_input.dat as FBB_with_parameters.m does, for the
+    family given by its header values (defaults: the script's)."""
+    import argparse
+    from .woven_io import write_inputs
+    d = ReferenceFamily()
+    ap = argparse.ArgumentParser(description="the reference's fBb input files (FBB_with_parameters.m)")
+    ap.add_argument("--out", required=True, help="directory for the two files")
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--host", action="store_true", help="draw the centerlines on the host instead of the GPU")
+    for f in dataclasses.fields(ReferenceFamily):
+        ap.add_argument("--" + f.name.replace("_", "-"), type=type(getattr(d, f.name)), default=getattr(d, f.name))
+    a = ap.parse_args(argv)
+    fam = ReferenceFamily(**{f.name: getattr(a, f.name) for f in dataclasses.fields(ReferenceFamily)})
+    g = None
+    if not a.host:
+        from .api import WovenGPU
+        g = WovenGPU(16, 64)
+    try:
+        p, t = reference_inputs(fam, a.seed, g)
+    finally:
+        if g is not None:
+            g.close()
+    write_inputs(a.out, p, t, reference_digits=True)
+    print(f"{t.nline} tubes, {t.npoint_all} points written to {a.out}")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -170,15 +170,19 @@ def read_inputs(directory) -> tuple[RunParams, Tubes]:
     return p, t
 
 
-def write_inputs(directory, p: RunParams, t: Tubes) -> None:
-    """Write both input files in the layout FBB_with_parameters.m produces."""
+def write_inputs(directory, p: RunParams, t: Tubes, reference_digits: bool = False) -> None:
+    """Write both input files in the layout FBB_with_parameters.m produces. With reference_digits
+    the numbers are printed as the script prints them (%18g: six significant digits; counts,
+    coordinates, sigma and Gamma, FBB_with_parameters.m:53-61, 122-126, 232-233); otherwise
+    coordinates, sigma and Gamma keep ten digits."""
     os.makedirs(directory, exist_ok=True)
     buf = io.StringIO()
-    buf.write(f"{t.nline:18d}\n")
+    fmt = "%18g" if reference_digits else "%18.10g"
+    buf.write((f"{t.nline:18g}" if reference_digits else f"{t.nline:18d}") + "\n")
     for n in t.npoint:
-        buf.write(f"{int(n):18d}\n")
+        buf.write((f"{int(n):18g}" if reference_digits else f"{int(n):18d}") + "\n")
     xyz = np.stack([t.cx, t.cy, t.cz], axis=1)
-    np.savetxt(buf, xyz, fmt="%18.10g", delimiter="")
+    np.savetxt(buf, xyz, fmt=fmt, delimiter="")
     with open(os.path.join(directory, "centerline_input.dat"), "w") as f:
         f.write(buf.getvalue())
     rows = [
@@ -189,8 +193,9 @@ def write_inputs(directory, p: RunParams, t: Tubes) -> None:
         f"{p.dzeta_sup:18g}", "---------------",
         'variables="i_line","sigma","Gamma","n_T","if_filter"',
     ]
+    sg = "18g" if reference_digits else "18.10g"
     for i in range(t.nline):
-        rows.append(f"{i + 1:8d}{t.sigma[i]:18.10g}{t.gamma[i]:18.10g}{int(t.n_T[i]):8d}{int(t.if_filter[i]):8d}")
+        rows.append(f"{i + 1:8d}{t.sigma[i]:{sg}}{t.gamma[i]:{sg}}{int(t.n_T[i]):8d}{int(t.if_filter[i]):8d}")
     with open(os.path.join(directory, "parameters_input.dat"), "w") as f:
         f.write("\n".join(rows) + "\n")
 

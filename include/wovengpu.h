@@ -273,6 +273,15 @@ int wt_stage_times(wt_ctx *ctx, double *ms, int32_t cap);
 int wt_fbb_centerlines(wt_ctx *ctx, int32_t ntube, const int32_t *npoint, const double *hurst,
                        const int32_t *winding, double wiggle, uint64_t seed, const uint64_t *keys,
                        const double *gauss, double *cx, double *cy, double *cz);
+/* The same construction as FBB_with_parameters.m:73-120 writes it (reference mode): no winding
+ * drift, the bridge scaled to the step rms dB[l] (the level's dB_1 r_s^(i-1), FBB:37, 70), a tube of
+ * npoint[l] < 128 points sampled from a 128-point path at ceil(j 128 / npoint), j = 1..npoint
+ * (FBB:78-84), the shift (u - 1/2) 2 pi added after the scaling and the wrap mod(x, 2 pi) - pi with
+ * mod(x, y) = x - floor(x / y) y (FBB:117-120). Random inputs as for wt_fbb_centerlines, with m the
+ * embedding of max(npoint[l], 128) points. */
+int wt_fbb_reference_centerlines(wt_ctx *ctx, int32_t ntube, const int32_t *npoint, const double *hurst,
+                                 const double *dB, uint64_t seed, const uint64_t *keys, const double *gauss,
+                                 double *cx, double *cy, double *cz);
 /* Embedding size m of a tube of npoint points: 2 x the smallest 2^a 3^b >= npoint - 1. */
 int32_t wt_fbb_embedding(int32_t npoint);
 /* The first count standard normals of tube `tube`'s random stream (inspection). */

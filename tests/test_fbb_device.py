@@ -107,3 +107,36 @@ def test_device_config_matches_its_host_twin_statistically():
     assert np.isfinite(up) and up > 0
     assert g.spectrum(1).sum() == pytest.approx(1.5, rel=0.02)
     g.close()
+
+
+def test_reference_mode_equals_host_construction():
+    """wt_fbb_reference_centerlines (the reference's construction: level dB, no drift, tubes below
+    128 points sampled from a 128-point path, mod(x, 2 pi) - pi) against
+    fbb.reference_centerline_from: on given inputs at 8..3000 points, and on the device's own
+    Philox stream for the family of the reference's shipped input pair (73 tubes)."""
+    npts = [8, 20, 64, 100, 127, 128, 148, 1000, 3000]
+    H = [0.834, 0.6, 0.75, 0.9, 0.55, 0.834, 0.834, 0.7, 0.65]
+    dB = [2.4, 1.2, 0.6, 0.3, 0.5, 1.0, 0.6043, 0.05, 0.01]
+    rng = np.random.default_rng(13)
+    gauss, host = [], []
+    for n, h, d in zip(npts, H, dB):
+        m = fbb.embedding_size(max(n, 128) - 1)
+        zre, zim, u3 = rng.standard_normal((2, m)), rng.standard_normal((2, m)), rng.random(3)
+        gauss.append(np.concatenate([zre.ravel(), zim.ravel(), u3]))
+        host.append(fbb.reference_centerline_from(n, h, d, zre, zim, u3))
+    g = ctx()
+    cx, cy, cz = g.fbb_reference_centerlines(npts, H, dB, seed=0, gauss=np.concatenate(gauss))
+    fam = fbb.ReferenceFamily(rho_w_obj=0.019)
+    _, td = fbb.reference_inputs(fam, seed=7, g=g)
+    g.close()
+    off = np.concatenate([[0], np.cumsum(npts)])
+    worst = 0.0
+    for i, x in enumerate(host):
+        dev = np.stack([cx[off[i]:off[i + 1]], cy[off[i]:off[i + 1]], cz[off[i]:off[i + 1]]])
+        assert np.all(dev >= -np.pi) and np.all(dev < np.pi)
+        worst = max(worst, float(np.abs(minimg(dev - x)).max()))
+    _, th = fbb.reference_inputs(fam, seed=7)
+    fam_worst = max(float(np.abs(minimg(a - b)).max()) for a, b in ((td.cx, th.cx), (td.cy, th.cy), (td.cz, th.cz)))
+    print(f"fbb reference mode, device vs host: given inputs {worst:.2e}, Philox family {fam_worst:.2e}", flush=True)
+    assert worst <= 1e-10 and fam_worst <= 1e-10
+    assert np.array_equal(td.npoint, th.npoint) and np.array_equal(td.sigma, th.sigma)
