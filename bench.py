@@ -433,7 +433,7 @@ def measured_traffic(workload: str, kernel: str):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=["c2", "c1", "small", "c4", "c3"])
     ap.add_argument("--cpu-budget", type=float, default=30.0,
@@ -503,14 +503,21 @@ def main():
     if not args.no_autotune:
         log("[bench] autotune: " + g.autotune(args.tile_cache).strip().replace("\n", "; "))
     _force_tiles(g, args.deposit_tiles)
+    cold_ms = None
     for i in range(args.warmup):
+        tc = time.perf_counter()
         g.generate()
+        if i == 0:   # the first generation also uploads the tube table and sizes the work lists
+            cold_ms = (time.perf_counter() - tc) * 1e3
         log(f"[bench] warmup {i} done, stages ms {np.round(g.stage_times()[:4], 2).tolist()}")
     # every library call is stream-synchronous on return, so the clock brackets finished work
     t0 = time.perf_counter()
     stage_ms = np.zeros(4)
+    step_ms = []
     for i in range(args.steps):
+        ts = time.perf_counter()
         g.generate()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
         stage_ms += g.stage_times()[:4]
     dt = time.perf_counter() - t0
     ms_per_step = dt / args.steps * 1e3
@@ -579,6 +586,7 @@ def main():
         "config": {"workload": wname, "N": p.N, "tubes": t.nline, "points": int(t.npoint_all),
                    "sub_segments": int(seg["nseg_all"]), "levels": 6 if args.config == "c2" else None},
         "stages_ms": dict(zip(["segments", "bin", "deposit", "biot_savart"], (stage_ms / args.steps).tolist())),
+        "step_ms": step_ms, "median_ms": float(np.median(step_ms)), "cold_ms": cold_ms,
         "kernels_ms": kernels, "graph_ms_per_step": graph_ms, "roofline": roof, "headline_roofline": headline, "cpu_baseline": cpu,
         "time_to_target": ttt,
         "fft_vs_rocfft": fftc,
