@@ -59,9 +59,10 @@ def algorithmic_cost(slot: str, N: int, seg: dict, dep: dict, p):
     if slot == "theta":
         # certified FP32 re-evaluation of the reference's ntf_pre+1 samples per section, in pairs
         # t = 1/2 +- s: per pair s^2, the even and odd parts of the 4 shifted polynomials (25 FMA),
-        # 4 products s*O and 8 sums (63 FLOP); per sample |w|^2 (1 mul + 2 FMA), sqrt, reciprocal,
-        # 1 mul, the FP64 running sum and 3 bound updates (12 FLOP): 43.5 FLOP per sample
-        return seg["nsec_all"] * (p.ntf_pre + 1) * 43.5, "TFLOP/s", "valu", PEAK_FP32_TFLOPS
+        # 4 products s*O and 8 sums (63 FLOP), the pair's sum, its FP64 accumulation and the smaller
+        # denominator (3 FLOP); per sample |w|^2 (1 mul + 2 FMA), sqrt, reciprocal, 1 mul (8 FLOP):
+        # 41 FLOP per sample
+        return seg["nsec_all"] * (p.ntf_pre + 1) * 41.0, "TFLOP/s", "valu", PEAK_FP32_TFLOPS
     if slot == "deposit":
         # FP64 on the vector ALU (its peak equals the published FP64 peak): 16 FLOP per grid column
         # walked (two plane positions and the interval), 33 per candidate cell (the reference's

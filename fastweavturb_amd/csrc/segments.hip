@@ -313,21 +313,21 @@ __global__ __launch_bounds__(128) void k_theta_fast(Geo g, Run, int32_t *__restr
         }
     }
     double acc = 0.0;           // sum of f
-    float sg = 0.0f, sfg = 0.0f, gmin = 3.0e38f;
-    auto sample = [&](float wx, float wy, float wz, float gg) {
-        float n2 = __builtin_fmaf(wx, wx, __builtin_fmaf(wy, wy, wz * wz));
-        float rg = __builtin_amdgcn_rcpf(gg);
-        float f = __builtin_amdgcn_sqrtf(n2) * rg;
-        acc += (double)f;
-        sg += rg;
-        sfg = __builtin_fmaf(f, rg, sfg);
-        gmin = fminf(gmin, gg);
+    float gmin = 3.0e38f;       // smallest denominator
+    auto fval = [](float wx, float wy, float wz, float gg) {
+        const float n2 = __builtin_fmaf(wx, wx, __builtin_fmaf(wy, wy, wz * wz));
+        return __builtin_amdgcn_sqrtf(n2) * __builtin_amdgcn_rcpf(gg);
     };
-    // samples j = 0..M at t = j/M: pairs t = 1/2 +- s with s = (2i + o)/(2M), plus t = 1/2 for even M
+    // samples j = 0..M at t = j/M: pairs t = 1/2 +- s with s = (2i + o)/(2M), plus t = 1/2 for even M;
+    // a pair's two values are added in FP32 before the FP64 sum, and the smaller of its two
+    // denominators, fl(gE + gO) and fl(gE - gO), is fl(gE - |gO|) (rounding is monotonic)
     const bool odd = (M & 1) != 0;
     const int npair = odd ? (M + 1) / 2 : M / 2, o = odd ? 1 : 2;
     const float h = 1.0f / (float)(2 * M);
-    if (!odd) sample(wE[0][0], wE[1][0], wE[2][0], gE[0]);
+    if (!odd) {
+        acc += (double)fval(wE[0][0], wE[1][0], wE[2][0], gE[0]);
+        gmin = gE[0];
+    }
     for (int i = 0; i < npair; i++) {
         const float sv = (float)(2 * i + o) * h, v = sv * sv;
         float wp[3], wm[3];
@@ -338,15 +338,20 @@ __global__ __launch_bounds__(128) void k_theta_fast(Geo g, Run, int32_t *__restr
             wm[c] = E - Os;
         }
         const float gEv = horner8(gE, 4, v), gOs = sv * horner8(gO, 3, v);
-        sample(wp[0], wp[1], wp[2], gEv + gOs);
-        sample(wm[0], wm[1], wm[2], gEv - gOs);
+        const float fp = fval(wp[0], wp[1], wp[2], gEv + gOs), fm = fval(wm[0], wm[1], wm[2], gEv - gOs);
+        acc += (double)(fp + fm);
+        gmin = fminf(gmin, gEv - fabsf(gOs));
     }
-    // error bound of the FP32 sum against the reference's FP64 sum (both times dz)
+    // error bound of the FP32 sum against the reference's FP64 sum (both times dz): per sample the
+    // numerator's error kw Wb over the denominator and the denominator's relative error kg Gb / g,
+    // summed with sum 1/g <= (M + 1) / gmin and sum f/g <= acc / gmin; the roundings of sqrt, the
+    // reciprocal and the product (6u) and of the pair sums (u)
     const double u = 5.9604644775390625e-08;             // 2^-24
     const double zrel = 4.0e-15 * LD / fmax(zM, 1e-300);  // the reference's sample points
     const double kw = 16.0 * u + zrel, kg = 20.0 * u + zrel;
     const double Wb = sqrt(Wb2);
-    const double sum_b = 1.1 * (kw * Wb * (double)sg + 6.0 * u * acc + kg * Gb * (double)sfg) * 1.01;
+    const double ig = 1.0 / (double)gmin;
+    const double sum_b = 1.1 * (kw * Wb * (double)(M + 1) * ig + 7.0 * u * acc + kg * Gb * acc * ig) * 1.01;
     const double theta = acc * dz;
     const double err = sum_b * dz + 1.0e-9 * theta;
     const double sc = (double)r.ntf_times / kPi2;
