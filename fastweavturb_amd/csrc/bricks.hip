@@ -23,23 +23,46 @@ __device__ __forceinline__ uint64_t lanemask_lt64() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Adds 1 to cnt[key] for every lane with `active`; one atomic per distinct key in the wave.
-// Returns the counter value before this lane's increment (its slot).
-__device__ __forceinline__ uint32_t wave_count_add(uint32_t *cnt, int64_t key, bool active) {
+// Adds 1 to cnt[key] for every lane with `active`; one atomic per distinct key in the wave, none
+// of them waited for.
+__device__ __forceinline__ void wave_count_add(uint32_t *cnt, int64_t key, bool active) {
     uint64_t pending = __ballot(active);
-    uint32_t slot = 0;
     const int lane = (int)__lane_id();
     while (pending) {
-        int leader = __ffsll((unsigned long long)pending) - 1;
-        int64_t lk = __shfl(key, leader, 64);
-        uint64_t m = __ballot(active && key == lk) & pending;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(cnt + lk, (uint32_t)__popcll(m));
-        base = __shfl(base, leader, 64);
-        if ((m >> lane) & 1ull) slot = base + (uint32_t)__popcll(m & lanemask_lt64());
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int64_t lk = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(key >> 32), leader) << 32) |
+                                     (uint32_t)__builtin_amdgcn_readlane((int)key, leader));
+        const uint64_t m = __ballot(active && key == lk) & pending;
+        if (lane == leader) atomicAdd(cnt + lk, (uint32_t)__popcll(m));
         pending &= ~m;
     }
-    return slot;
+}
+
+// Adds 1 to cnt[key] for every lane with `active` and returns the counter value before this lane's
+// increment (its slot). The wave's lanes are grouped by key first; then every group's leader adds
+// the group's size at once, so the wave waits for one round of returned atomics, not one per key.
+__device__ __forceinline__ uint32_t wave_slot_add(uint32_t *cnt, int64_t key, bool active) {
+    uint64_t pending = __ballot(active);
+    const int lane = (int)__lane_id();
+    int my_leader = lane;
+    uint64_t my_m = 0;
+    uint32_t lead_n = 0;
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int64_t lk = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(key >> 32), leader) << 32) |
+                                     (uint32_t)__builtin_amdgcn_readlane((int)key, leader));
+        const uint64_t m = __ballot(active && key == lk) & pending;
+        if ((m >> lane) & 1ull) {
+            my_leader = leader;
+            my_m = m;
+        }
+        if (lane == leader) lead_n = (uint32_t)__popcll(m);
+        pending &= ~m;
+    }
+    uint32_t base = 0;
+    if (lead_n) base = atomicAdd(cnt + key, lead_n);
+    base = (uint32_t)__shfl((int)base, my_leader, 64);
+    return base + (uint32_t)__popcll(my_m & lanemask_lt64());
 }
 
 // Bricks of one axis met by the extended index range [lo, hi] (lo >= 1 - N, hi <= 2N): for each
@@ -129,22 +152,76 @@ __device__ __forceinline__ bool pair_meets_slab(const SubSeg &sg, const int32_t 
 // steps while the rest of its wave waits.
 constexpr int32_t kBigPairs = 256;
 
-// Pass 0 (FILL = false): counts pairs per brick, flags boxes outside [1-N, 2N] and lists the
-// sub-segments with more than kBigPairs pairs in big[0 .. *nbig) (entries beyond big_cap are not
-// stored: the host then grows the list and repeats the pass).
-// Pass 1 (FILL = true): writes every pair (sub-point index, image code kx + 3ky + 9kz) into its
-// brick's list, after the brick's wide[key] pairs of wide boxes (k_pairs_big), which lead it: the
-// deposition spreads those expensive pairs over its waves before it takes the rest in runs.
-template <bool FILL>
-__global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restrict__ cnt,
-                                               const int64_t *__restrict__ off, const uint32_t *__restrict__ wide,
-                                               uint32_t *__restrict__ pairs,
-                                               uint8_t *__restrict__ pair_img, int32_t *__restrict__ err,
-                                               uint32_t *__restrict__ big, uint32_t *__restrict__ nbig,
-                                               uint32_t big_cap) {
-    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+// The walk over one sub-segment's (brick, image) entries, x fastest as AxisImages::at orders
+// them. With SLAB, the y and z slab terms are kept and change only when the walk leaves a row.
+template <bool SLAB>
+struct EntryWalk {
+    AxisWalk wk[3];
+    double h0y = 0.0, l1y = 0.0, h0z = 0.0, l1z = 0.0;
+    int32_t jx = 0, jy = 0;
+    int64_t rowkey = 0;
+    __device__ void start(const AxisImages *ab, const SubSeg &sg, const int32_t *bd, int64_t brick0, int32_t nbx,
+                          int32_t nby) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) wk[c].start(ab[c]);
+        if (SLAB) {
+            slab_terms(sg, 1, wk[1].bi, wk[1].im, bd[1], h0y, l1y);
+            slab_terms(sg, 2, wk[2].bi, wk[2].im, bd[2], h0z, l1z);
+        }
+        rowkey = brick0 + ((int64_t)wk[2].bi * nby + wk[1].bi) * nbx;
+    }
+    // the current entry's brick key, image code and (SLAB) slab test; then steps to the next entry
+    // (t: the current entry's index, n: the number of entries)
+    __device__ bool take(const AxisImages *ab, const SubSeg &sg, const int32_t *bd, int64_t brick0, int32_t nbx,
+                         int32_t nby, int32_t nx, int32_t ny, int32_t t, int32_t n, int64_t &key, int32_t &code) {
+        bool met = true;
+        if (SLAB) {
+            double h0x, l1x;
+            slab_terms(sg, 0, wk[0].bi, wk[0].im, bd[0], h0x, l1x);
+            met = slab_met((h0x + h0y) + h0z, (l1x + l1y) + l1z);
+        }
+        key = rowkey + wk[0].bi;
+        code = wk[0].im + 3 * wk[1].im + 9 * wk[2].im;
+        if (++jx < nx) {
+            wk[0].next(ab[0]);
+        } else if (t + 1 < n) {
+            jx = 0;
+            wk[0].start(ab[0]);
+            if (++jy < ny) {
+                wk[1].next(ab[1]);
+            } else {
+                jy = 0;
+                wk[1].start(ab[1]);
+                wk[2].next(ab[2]);
+                if (SLAB) slab_terms(sg, 2, wk[2].bi, wk[2].im, bd[2], h0z, l1z);
+            }
+            if (SLAB) slab_terms(sg, 1, wk[1].bi, wk[1].im, bd[1], h0y, l1y);
+            rowkey = brick0 + ((int64_t)wk[2].bi * nby + wk[1].bi) * nbx;
+        }
+        return met;
+    }
+};
+
+__device__ __forceinline__ int32_t wave_max(int32_t v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const int32_t w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Count pass: counts pairs per brick, flags boxes outside [1-N, 2N], lists the sub-segments with
+// more than kBigPairs pairs in big[0 .. *nbig) (entries beyond big_cap are not stored: the host
+// then grows the list and repeats the pass) and writes every sub-point's BinRec.
+__global__ __launch_bounds__(256) void k_pairs_count(Geo g, Run r0, uint32_t *__restrict__ cnt,
+                                                     BinRec *__restrict__ rec, int32_t *__restrict__ err,
+                                                     uint32_t *__restrict__ big, uint32_t *__restrict__ nbig,
+                                                     uint32_t big_cap) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool in = q < g.nsub_all;
     SubSeg sg;
-    bool ok = q < g.nsub_all && subseg_load(g, r0, q, sg);
+    bool ok = in && subseg_load(g, r0, q, sg);
     AxisImages ab[3];
     int32_t n = 0, nx = 1, ny = 1;
     int64_t brick0 = 0;
@@ -160,7 +237,7 @@ __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restri
         for (int c = 0; c < 3; c++)
             if (sg.lo[c] < 1 - sg.N || sg.hi[c] > 2 * sg.N) ok = false;
         if (!ok) {
-            if (!FILL) atomicOr(err, 2);
+            atomicOr(err, 2);
         } else {
 #pragma unroll
             for (int c = 0; c < 3; c++) ab[c] = axis_images(sg.lo[c], sg.hi[c], sg.N, r.lbd[c]);
@@ -168,63 +245,86 @@ __global__ __launch_bounds__(256) void k_pairs(Geo g, Run r0, uint32_t *__restri
             ny = ab[1].count();
             n = nx * ny * ab[2].count();
             if (n > kBigPairs) {
-                if (!FILL) {
-                    const uint32_t k = atomicAdd(nbig, 1u);
-                    if (k < big_cap) big[k] = (uint32_t)q;
-                }
+                const uint32_t k = atomicAdd(nbig, 1u);
+                if (k < big_cap) big[k] = (uint32_t)q;
                 n = 0;
             }
         }
     }
-    // wave-uniform loop over the lanes' brick lists
-    int32_t nmax = n;
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        int32_t w = __shfl_xor(nmax, o, 64);
-        nmax = w > nmax ? w : nmax;
+    const int32_t nmax = wave_max(n);
+    EntryWalk<true> ew;
+    if (n > 0) ew.start(ab, sg, bd, brick0, nbx, nby);
+    uint32_t mask = 0;
+    for (int32_t t = 0; t < nmax; t++) {
+        bool act = t < n;
+        int64_t key = 0;
+        int32_t code = 0;
+        if (act) act = ew.take(ab, sg, bd, brick0, nbx, nby, nx, ny, t, n, key, code);
+        if (act && t < kRecMaskBits) mask |= 1u << t;
+        wave_count_add(cnt, key, act);
     }
-    // each lane walks its box's entries x fastest, as AxisImages::at orders them; the y and z
-    // slab terms and the row's brick key change only when the walk leaves a row
-    AxisWalk wk[3];
-    double h0y = 0.0, l1y = 0.0, h0z = 0.0, l1z = 0.0;
-    int32_t jx = 0, jy = 0;
-    int64_t rowkey = 0;
-    if (n > 0) {
+    if (in) {
+        BinRec o;
 #pragma unroll
-        for (int c = 0; c < 3; c++) wk[c].start(ab[c]);
-        slab_terms(sg, 1, wk[1].bi, wk[1].im, bd[1], h0y, l1y);
-        slab_terms(sg, 2, wk[2].bi, wk[2].im, bd[2], h0z, l1z);
-        rowkey = brick0 + ((int64_t)wk[2].bi * nby + wk[1].bi) * nbx;
+        for (int c = 0; c < 3; c++) {
+            o.lo[c] = (int16_t)(n > 0 ? sg.lo[c] : 1);
+            o.hi[c] = (int16_t)(n > 0 ? sg.hi[c] : 0);
+        }
+        o.mask = mask;
+        rec[q] = o;
     }
+}
+
+// Fill pass: writes every pair (sub-point index, image code kx + 3ky + 9kz) into its brick's list,
+// after the brick's wide[key] pairs of wide boxes (k_pairs_big), which lead it: the deposition
+// spreads those expensive pairs over its waves before it takes the rest in runs. The boxes and
+// (up to kRecMaskBits entries) the slab tests come from the count pass's records; a box of more
+// entries repeats its slab tests.
+__global__ __launch_bounds__(256) void k_pairs_fill(Geo g, Run r0, uint32_t *__restrict__ cnt,
+                                                    const BinRec *__restrict__ rec, const int64_t *__restrict__ off,
+                                                    const uint32_t *__restrict__ wide, uint32_t *__restrict__ pairs,
+                                                    uint8_t *__restrict__ pair_img) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    BinRec rc;
+    int32_t n = 0, nx = 1, ny = 1;
+    int64_t brick0 = 0;
+    int32_t nbx = 1, nby = 1, bd[3] = {1, 1, 1};
+    AxisImages ab[3];
+    if (q < g.nsub_all) {
+        rc = rec[q];
+        if (rc.lo[0] <= rc.hi[0]) {
+            const Run &r = g.runs[g.tube_inst[g.sub_tube[q]]];
+            brick0 = r.brick0;
+            nbx = r.nb[0];
+            nby = r.nb[1];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                bd[c] = r.bd[c];
+                ab[c] = axis_images(rc.lo[c], rc.hi[c], r.N, r.lbd[c]);
+            }
+            nx = ab[0].count();
+            ny = ab[1].count();
+            n = nx * ny * ab[2].count();
+        }
+    }
+    const bool slow = n > kRecMaskBits;
+    SubSeg sg;
+    if (slow) subseg_load(g, r0, q, sg);   // a paired sub-segment: it loads
+    const int32_t nmax = wave_max(n);
+    EntryWalk<true> ews;     // slow lanes: slab tests again
+    EntryWalk<false> ewf;    // the rest: the recorded mask
+    if (slow) ews.start(ab, sg, bd, brick0, nbx, nby);
+    else if (n > 0) ewf.start(ab, sg, bd, brick0, nbx, nby);
     for (int32_t t = 0; t < nmax; t++) {
         bool act = t < n;
         int64_t key = 0;
         int32_t code = 0;
         if (act) {
-            double h0x, l1x;
-            slab_terms(sg, 0, wk[0].bi, wk[0].im, bd[0], h0x, l1x);
-            key = rowkey + wk[0].bi;
-            code = wk[0].im + 3 * wk[1].im + 9 * wk[2].im;
-            act = slab_met((h0x + h0y) + h0z, (l1x + l1y) + l1z);
-            if (++jx < nx) {
-                wk[0].next(ab[0]);
-            } else if (t + 1 < n) {
-                jx = 0;
-                wk[0].start(ab[0]);
-                if (++jy < ny) {
-                    wk[1].next(ab[1]);
-                } else {
-                    jy = 0;
-                    wk[1].start(ab[1]);
-                    wk[2].next(ab[2]);
-                    slab_terms(sg, 2, wk[2].bi, wk[2].im, bd[2], h0z, l1z);
-                }
-                slab_terms(sg, 1, wk[1].bi, wk[1].im, bd[1], h0y, l1y);
-                rowkey = brick0 + ((int64_t)wk[2].bi * nby + wk[1].bi) * nbx;
-            }
+            if (slow) act = ews.take(ab, sg, bd, brick0, nbx, nby, nx, ny, t, n, key, code);
+            else act = ewf.take(ab, sg, bd, brick0, nbx, nby, nx, ny, t, n, key, code) && ((rc.mask >> t) & 1u);
         }
-        uint32_t slot = wave_count_add(cnt, key, act);
-        if (FILL && act) {
+        const uint32_t slot = wave_slot_add(cnt, key, act);
+        if (act) {
             const int64_t at = off[key] + wide[key] + slot;
             pairs[at] = (uint32_t)q;
             pair_img[at] = (uint8_t)code;
@@ -295,8 +395,7 @@ static hipError_t bin_count_kernels(wt_ctx *c) {
     WT_HIP(fill_zero(w.err, sizeof(int32_t), st));
     WT_HIP(fill_zero(w.nbig, sizeof(uint32_t), st));
     kt_begin(c, kSlotBinCount);
-    hipLaunchKernelGGL(k_pairs<false>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, w.cnt,
-                       (const int64_t *)nullptr, (const uint32_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr,
+    hipLaunchKernelGGL(k_pairs_count, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, w.cnt, w.rec,
                        w.err, w.big, w.nbig, w.big_cap);
     hipLaunchKernelGGL(k_pairs_big<false>, dim3(w.big_grid), dim3(256), 0, st, g, c->run, w.cnt, b.brick_wide,
                        (const int64_t *)nullptr, (uint32_t *)nullptr, (uint8_t *)nullptr, (const uint32_t *)w.big,
@@ -315,9 +414,9 @@ static hipError_t bin_fill_kernels(wt_ctx *c) {
     WT_HIP(fill_zero(w.cnt, sizeof(uint32_t) * b.nbricks, st));
     WT_HIP(fill_zero(w.wslot, sizeof(uint32_t) * b.nbricks, st));
     kt_begin(c, kSlotBinFill);
-    hipLaunchKernelGGL(k_pairs<true>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, w.cnt,
-                       (const int64_t *)b.brick_off, (const uint32_t *)b.brick_wide, b.pairs, b.pair_img, w.err, w.big,
-                       w.nbig, w.big_cap);
+    hipLaunchKernelGGL(k_pairs_fill, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, w.cnt,
+                       (const BinRec *)w.rec, (const int64_t *)b.brick_off, (const uint32_t *)b.brick_wide, b.pairs,
+                       b.pair_img);
     hipLaunchKernelGGL(k_pairs_big<true>, dim3(w.big_grid), dim3(256), 0, st, g, c->run, w.cnt, w.wslot,
                        (const int64_t *)b.brick_off, b.pairs, b.pair_img, (const uint32_t *)w.big,
                        (const uint32_t *)w.nbig, w.big_cap);
@@ -351,6 +450,7 @@ hipError_t launch_bins(wt_ctx *c) {
     WT_HIP(balloc(c, &b.brick_tab, nbricks));
     WT_HIP(balloc(c, &b.brick_off, nbricks + 1));
     WT_HIP(balloc(c, &w.cnt, nbricks));
+    WT_HIP(balloc(c, &w.rec, g.nsub_all));
     WT_HIP(balloc(c, &w.wslot, nbricks));
     WT_HIP(balloc(c, &b.brick_wide, nbricks));
     WT_HIP(balloc(c, &w.err, 1));

@@ -102,7 +102,18 @@ struct SegWs {
     unsigned int *nredo = nullptr;      // [1]
     int64_t *scan = nullptr;            // scan workspace of the sub-point layout
 };
+// What the count pass of the binning learnt about one sub-point's sub-segment, so that the fill
+// pass needs no geometry: its box in extended indices and, for boxes of at most kRecMaskBits
+// (brick, image) entries, which entries meet the end-plane slab (bit t: entry t in the order of
+// the walk). A sub-point that starts no paired sub-segment (no deposit, or a box expanded by
+// k_pairs_big) holds an empty box (lo > hi).
+constexpr int kRecMaskBits = 32;
+struct BinRec {
+    int16_t lo[3], hi[3];
+    uint32_t mask;
+};
 struct BinWs {
+    BinRec *rec = nullptr;              // [nsub_all] count pass -> fill pass
     uint32_t *cnt = nullptr;            // [nbricks] pairs per brick (fill pass: slots after the wide ones)
     uint32_t *wslot = nullptr;          // [nbricks] fill pass: slots of the wide boxes' pairs
     int32_t *err = nullptr;             // [1]
