@@ -267,8 +267,8 @@ __global__ __launch_bounds__(256) void k_pairs_count(Geo g, Run r0, uint32_t *__
         BinRec o;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            o.lo[c] = (int16_t)(n > 0 ? sg.lo[c] : 1);
-            o.hi[c] = (int16_t)(n > 0 ? sg.hi[c] : 0);
+            o.lo[c] = (int16_t)(ok ? sg.lo[c] : 1);
+            o.hi[c] = (int16_t)(ok ? sg.hi[c] : 0);
         }
         o.mask = mask;
         rec[q] = o;
@@ -305,6 +305,7 @@ __global__ __launch_bounds__(256) void k_pairs_fill(Geo g, Run r0, uint32_t *__r
             nx = ab[0].count();
             ny = ab[1].count();
             n = nx * ny * ab[2].count();
+            if (n > kBigPairs) n = 0;   // k_pairs_big's
         }
     }
     const bool slow = n > kRecMaskBits;

@@ -38,6 +38,7 @@ struct DepArgs {
     Geo g;
     Run r;                        // instance 0 (only used for the Run argument of subseg_load)
     Bins b;
+    const BinRec *rec;            // the binning's boxes, per sub-point
     void *omega;                  // double* (FP64 mode) or float* (FP32 mode)
     double fx_limit;              // |value * scale| bound before overflow (FP32 mode; the scale
                                   // is the brick's instance's Run::fx_scale)
@@ -51,9 +52,9 @@ struct DepArgs {
 // tube l (sub_sec[q], sub_tube[q]), with a shorter chain of dependent loads: the four sub-points
 // q..q+3 (the last sub-segment of a section takes its second chord from the next section's first
 // two sub-points, which are q+2 and q+3 there) and the section's and tube's values are read at
-// once. Same values as subseg_load.
+// once; the box is the one the binning computed (rc). Same values as subseg_load.
 __device__ __forceinline__ void subseg_load_pair(const Geo &g, const Run &ri, int64_t q, int32_t s, int32_t l,
-                                                 SubSeg &o) {
+                                                 const BinRec &rc, SubSeg &o) {
     double p[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) p[k] = g.ci[3 * q + k];
@@ -101,10 +102,8 @@ __device__ __forceinline__ void subseg_load_pair(const Geo &g, const Run &ri, in
     o.R = (R2 > R) ? R2 : R;
 #pragma unroll
     for (int c = 0; c < 3; c++) {  // tube.f90:793-798
-        const double mn = o.ci0[c] < o.ci1[c] ? o.ci0[c] : o.ci1[c];
-        const double mx = o.ci0[c] > o.ci1[c] ? o.ci0[c] : o.ci1[c];
-        o.lo[c] = (int32_t)floor((mn - ri.start - o.R) / ri.dx + 1.0);
-        o.hi[c] = (int32_t)ceil((mx - ri.start + o.R) / ri.dx + 1.0);
+        o.lo[c] = rc.lo[c];
+        o.hi[c] = rc.hi[c];
     }
 }
 
@@ -349,8 +348,11 @@ struct RowSeg {
     uint16_t pad;
 };
 
+// A candidate cell in the axes of its pair's column walk: a along the column, b and c across it
+// (extended indices), and tag = owning lane | column axis << 6. The x, y, z indices are formed
+// when the candidate is tested, once per candidate instead of once per queue round.
 struct HitQ {
-    int16_t ip, jp, kp, owner;                       // extended indices, owning lane
+    int16_t a, tag, b, c;
 };
 
 // 1/x from the hardware reciprocal refined by one Newton step (within a few ulp, never exactly
@@ -550,8 +552,18 @@ struct Wave {
 template <bool F64, bool FAST>
 __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me, int cnt) {
     const int lane = W.lane;
-    const HitQ h = W.candq[W.candn - cnt + (lane < cnt ? lane : 0)];
+    const HitQ hq = W.candq[W.candn - cnt + (lane < cnt ? lane : 0)];
     wave_sync_lds();
+    struct {
+        int32_t ip, jp, kp, owner;
+    } h;
+    {
+        const int ax = hq.tag >> 6;
+        h.owner = hq.tag & 63;
+        h.ip = ax == 0 ? hq.a : (ax == 1 ? hq.c : hq.b);
+        h.jp = ax == 0 ? hq.b : (ax == 1 ? hq.a : hq.c);
+        h.kp = ax == 0 ? hq.c : (ax == 1 ? hq.b : hq.a);
+    }
     W.candn = __builtin_amdgcn_readfirstlane(W.candn - cnt);
     W.ncand += (uint32_t)cnt;
     SubSeg sg;
@@ -609,7 +621,7 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
         ax = w.ax;
         L = col_lines(w, ic);
     }
-    const int bx = ax == 2 ? 0 : ax + 1;
+    const int16_t tag = (int16_t)(own | (ax << 6));
     W.segn = __builtin_amdgcn_readfirstlane(W.segn - cnt);
     const int32_t cap = W.ri->colcap;
     const int32_t bl0 = ib, steps = len < cap ? len : cap;
@@ -625,12 +637,11 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
         for (int u = 0; (m = __ballot(u < nc)) != 0ull; u++) {
             if (u < nc) {
                 const bool second = u >= nc0;
-                const int32_t ia = second ? ia1 + (u - nc0) : ia0 + u, jb = second ? ib + 1 : ib;
                 HitQ q;
-                q.ip = (int16_t)(ax == 0 ? ia : (bx == 0 ? jb : ic));
-                q.jp = (int16_t)(ax == 1 ? ia : (bx == 1 ? jb : ic));
-                q.kp = (int16_t)(ax == 2 ? ia : (bx == 2 ? jb : ic));
-                q.owner = (int16_t)own;
+                q.a = (int16_t)((second ? ia1 - nc0 : ia0) + u);
+                q.tag = tag;
+                q.b = (int16_t)(second ? ib + 1 : ib);
+                q.c = (int16_t)ic;
                 W.candq[W.candn + lanes_below(m)] = q;
             }
             W.candn = __builtin_amdgcn_readfirstlane(W.candn + __popcll(m));
@@ -719,12 +730,14 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
         next = __shfl(next, 0, 64);
         uint32_t q_next = 0;
         int32_t code_next = 0, s_next = 0, l_next = 0;
+        BinRec rc_next{};
         int64_t pn = pair_of((int64_t)next);
         if ((int64_t)next < K && pn < P) {
             q_next = A.b.pairs[p0 + pn];
             code_next = A.b.pair_img[p0 + pn];
             s_next = g.sub_sec[q_next];
             l_next = g.sub_tube[q_next];
+            rc_next = A.rec[q_next];
         }
         for (;;) {
             const unsigned long long chunk = next;
@@ -732,6 +745,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
             const int64_t pi = pn;
             const uint32_t q_cur = q_next;
             const int32_t code = code_next, s_cur = s_next, l_cur = l_next;
+            const BinRec rc_cur = rc_next;
             if (lane == 0) next = atomicAdd(&s_misc[1], 1ull);
             next = __shfl(next, 0, 64);
             pn = pair_of((int64_t)next);
@@ -748,7 +762,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
             {
                 SubSeg sg;
                 ColWalk w;
-                if (have) subseg_load_pair(g, ri, (int64_t)q_cur, s_cur, l_cur, sg);
+                if (have) subseg_load_pair(g, ri, (int64_t)q_cur, s_cur, l_cur, rc_cur, sg);
                 if (have) {
 #pragma unroll
                     for (int c = 0; c < 3; c++) {
@@ -844,6 +858,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
             if (have_next) {   // q_next has arrived by now
                 s_next = g.sub_sec[q_next];
                 l_next = g.sub_tube[q_next];
+                rc_next = A.rec[q_next];
             }
             // ---- stage 1 (lane = pair): one row of the pair per step, non-empty bands queued ----
             for (int32_t ic = c0; __ballot(have && ic <= c1); ic++) {
@@ -930,6 +945,7 @@ static hipError_t deposit_kernels(wt_ctx *c, bool accumulate) {
     A.g = c->geo;
     A.r = c->run;
     A.b = c->bins;
+    A.rec = c->binws.rec;
     A.omega = c->omega;
     A.counters = cnt;
     A.next = (unsigned int *)(cnt + 4);

@@ -103,10 +103,10 @@ struct SegWs {
     int64_t *scan = nullptr;            // scan workspace of the sub-point layout
 };
 // What the count pass of the binning learnt about one sub-point's sub-segment, so that the fill
-// pass needs no geometry: its box in extended indices and, for boxes of at most kRecMaskBits
-// (brick, image) entries, which entries meet the end-plane slab (bit t: entry t in the order of
-// the walk). A sub-point that starts no paired sub-segment (no deposit, or a box expanded by
-// k_pairs_big) holds an empty box (lo > hi).
+// pass needs no geometry and the deposition takes the box as it is: its box in extended indices
+// (tube.f90:793-798) and, for boxes of at most kRecMaskBits (brick, image) entries, which entries
+// meet the end-plane slab (bit t: entry t in the order of the walk). A sub-point that starts no
+// deposited sub-segment holds an empty box (lo > hi).
 constexpr int kRecMaskBits = 32;
 struct BinRec {
     int16_t lo[3], hi[3];
