@@ -528,6 +528,14 @@ __device__ __forceinline__ void fetch_owner(const SegRec &me, int own, const Run
     sg.field_off = ri.field_off;
 }
 
+// LDS slot of brick cell (x, y, z) in each component's accumulator: rows along x, with x rotated by
+// y + z within its row. A batch of hits from walks along x shares nearly one x (the end-plane slab
+// is thinner than a cell); the rotation spreads those over the LDS banks, which a plain x-fastest
+// layout would map to the same two banks.
+__device__ __forceinline__ int acc_slot(int x, int y, int z, int lbx, int lby) {
+    return ((x + y + z) & ((1 << lbx) - 1)) + (((z << lby) + y) << lbx);
+}
+
 // Per-wave work state of the deposition pipeline: pairs (lanes) -> row segments -> columns ->
 // candidates (tested and deposited 64 at a time with every lane busy).
 template <bool F64>
@@ -579,8 +587,8 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
             if (FAST) hit_math_f32(hc, sg, h.ip, h.jp, h.kp, v);
             else hit_math_f64(W.A->g, sg, h.ip, h.jp, h.kp, v);
             const int N = W.ri->N;
-            const int cell = (wrap0(h.ip, N) - W.ox) +
-                             ((((wrap0(h.kp, N) - W.oz) << W.ri->lbd[1]) + (wrap0(h.jp, N) - W.oy)) << W.ri->lbd[0]);
+            const int x = wrap0(h.ip, N) - W.ox, y = wrap0(h.jp, N) - W.oy, z = wrap0(h.kp, N) - W.oz;
+            const int cell = acc_slot(x, y, z, W.ri->lbd[0], W.ri->lbd[1]);
             if (F64) {
 #pragma unroll
                 for (int c = 0; c < 3; c++) unsafeAtomicAdd((double *)W.acc + c * kCells + cell, v[c]);
@@ -886,11 +894,11 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                 for (int c = 0; c < 3; c++) {
                     if (F64) {
                         double *o = (double *)A.omega + idx + c * N3;
-                        const double v = (double)acc[c * kCells + i];
+                        const double v = (double)acc[c * kCells + acc_slot(x, y, z, lbx, lby)];
                         *o = A.accumulate ? *o + v : v;
                     } else {
                         float *o = (float *)A.omega + idx + c * N3;
-                        const double v = (double)(long long)acc[c * kCells + i] * ri.fx_inv;
+                        const double v = (double)(long long)acc[c * kCells + acc_slot(x, y, z, lbx, lby)] * ri.fx_inv;
                         *o = (float)(A.accumulate ? (double)*o + v : v);
                     }
                 }
