@@ -25,6 +25,27 @@
 
 namespace wt {
 
+// Global loads per thread issued before the first of them is waited for, in the passes' staging
+// loops (each loop step otherwise waits for its own load before storing it to LDS).
+constexpr int kLoadBatch = 8;
+
+// Quotient and remainder of i by d: shifts when d is a power of two (shift = log2 d, else -1),
+// otherwise a multiply-high by m = ceil(2^32 / d), exact for i * d < 2^32.
+struct Divider {
+    int d, shift;
+    uint32_t m;
+};
+__device__ __forceinline__ Divider make_divider(int d) {
+    Divider v;
+    v.d = d;
+    v.shift = (d & (d - 1)) == 0 ? __ffs(d) - 1 : -1;
+    v.m = (uint32_t)((0x100000000ull + (uint64_t)d - 1) / (uint64_t)d);
+    return v;
+}
+__device__ __forceinline__ int div_of(int i, const Divider &v) {
+    return v.shift >= 0 ? i >> v.shift : (int)__umulhi((uint32_t)i, v.m);
+}
+
 // ---- x pass, forward: two real lines -> two half spectra --------------------------------
 template <typename T>
 __global__ void k_fft_x_r2c(const T *__restrict__ f, C2<T> *__restrict__ spec, FftPlan pl_g, int pairs_per_block,
@@ -36,16 +57,32 @@ __global__ void k_fft_x_r2c(const T *__restrict__ f, C2<T> *__restrict__ spec, F
     __syncthreads();
     const int64_t pair0 = (int64_t)blockIdx.x * pairs_per_block;
     int np = (int)((npairs - pair0) < pairs_per_block ? (npairs - pair0) : pairs_per_block);
-    for (int i = threadIdx.x; i < np * N; i += blockDim.x) {
-        int pp = i / N, x = i - pp * N;
-        int64_t line = 2 * (pair0 + pp);
-        T a = f[line * N + x], b = f[(line + 1) * N + x];
-        buf[pp * P + pidx(pl.rev[x])] = C2<T>{a, b};
+    const Divider dN = make_divider(N), dH = make_divider(H);
+    const T *f0 = f + 2 * pair0 * N;   // the block's lines are contiguous
+    for (int i0 = threadIdx.x; i0 < np * N; i0 += kLoadBatch * blockDim.x) {
+        T a[kLoadBatch], b[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int i = i0 + u * blockDim.x;
+            if (i < np * N) {
+                const int pp = div_of(i, dN), x = i - pp * N;
+                a[u] = f0[(2 * pp) * N + x];
+                b[u] = f0[(2 * pp + 1) * N + x];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int i = i0 + u * blockDim.x;
+            if (i < np * N) {
+                const int pp = div_of(i, dN), x = i - pp * N;
+                buf[pp * P + pidx(pl.rev[x])] = C2<T>{a[u], b[u]};
+            }
+        }
     }
     __syncthreads();
     fft_lds<T, false>(buf, np, P, pl);
     for (int i = threadIdx.x; i < np * H; i += blockDim.x) {
-        int pp = i / H, k = i - pp * H;
+        int pp = div_of(i, dH), k = i - pp * H;
         C2<T> z = buf[pp * P + pidx(k == N / 2 ? N / 2 : k)];
         C2<T> zc = buf[pp * P + pidx((N - k) % N)];
         // A = (Z(k) + conj Z(N-k)) / 2, B = (Z(k) - conj Z(N-k)) / (2i)
@@ -70,29 +107,65 @@ __global__ void k_fft_x_c2r(const C2<T> *__restrict__ spec, T *__restrict__ u, T
     __syncthreads();
     const int64_t pair0 = (int64_t)blockIdx.x * pairs_per_block;
     int np = (int)((npairs - pair0) < pairs_per_block ? (npairs - pair0) : pairs_per_block);
-    for (int i = threadIdx.x; i < np * N; i += blockDim.x) {
-        int pp = i / N, k = i - pp * N;
-        int64_t line = 2 * (pair0 + pp);
-        int kk = k <= N / 2 ? k : N - k;
-        C2<T> A = spec[line * Hs + kk], B = spec[(line + 1) * Hs + kk];
-        if (k == 0 || k == N / 2) { A.y = 0; B.y = 0; }
-        if (k > N / 2) { A.y = -A.y; B.y = -B.y; }   // Hermitian extension
-        // Z = A + i B
-        buf[pp * P + pidx(k)] = C2<T>{A.x - B.y, A.y + B.x};
+    const Divider dN = make_divider(N);
+    const C2<T> *s0 = spec + 2 * pair0 * Hs;   // the block's lines are contiguous
+    for (int i0 = threadIdx.x; i0 < np * N; i0 += kLoadBatch * blockDim.x) {
+        C2<T> A[kLoadBatch], B[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int i = i0 + u * blockDim.x;
+            if (i < np * N) {
+                const int pp = div_of(i, dN), k = i - pp * N;
+                const int kk = k <= N / 2 ? k : N - k;
+                A[u] = s0[(2 * pp) * Hs + kk];
+                B[u] = s0[(2 * pp + 1) * Hs + kk];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int i = i0 + u * blockDim.x;
+            if (i < np * N) {
+                const int pp = div_of(i, dN), k = i - pp * N;
+                C2<T> a = A[u], b = B[u];
+                if (k == 0 || k == N / 2) { a.y = 0; b.y = 0; }
+                if (k > N / 2) { a.y = -a.y; b.y = -b.y; }   // Hermitian extension
+                // Z = A + i B
+                buf[pp * P + pidx(k)] = C2<T>{a.x - b.y, a.y + b.x};
+            }
+        }
     }
     __syncthreads();
     fft_lds_dif<T, true>(buf, np, P, pl);
     // a block's pairs may straddle two instances of the group
-    for (int i = threadIdx.x; i < np * N; i += blockDim.x) {
-        int pp = i / N, x = i - pp * N;
-        int64_t line = 2 * (pair0 + pp);
-        const T s = sc ? (T)sc[(pair0 + pp) / pairs_per_inst].inv_up : (T)1;
-        C2<T> z = buf[pp * P + pidx(pl.rev[x])];
-        u[line * N + x] = z.x * s;
-        u[(line + 1) * N + x] = z.y * s;
-        if (omega) {
-            omega[line * N + x] *= s;
-            omega[(line + 1) * N + x] *= s;
+    T *u0 = u + 2 * pair0 * N;
+    T *w0 = omega ? omega + 2 * pair0 * N : nullptr;
+    for (int i0 = threadIdx.x; i0 < np * N; i0 += kLoadBatch * blockDim.x) {
+        T oa[kLoadBatch], ob[kLoadBatch];
+        if (w0) {
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; u++) {
+                const int i = i0 + u * blockDim.x;
+                if (i < np * N) {
+                    const int pp = div_of(i, dN), x = i - pp * N;
+                    oa[u] = w0[(2 * pp) * N + x];
+                    ob[u] = w0[(2 * pp + 1) * N + x];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int i = i0 + u * blockDim.x;
+            if (i < np * N) {
+                const int pp = div_of(i, dN), x = i - pp * N;
+                const T s = sc ? (T)sc[(pair0 + pp) / pairs_per_inst].inv_up : (T)1;
+                C2<T> z = buf[pp * P + pidx(pl.rev[x])];
+                u0[(2 * pp) * N + x] = z.x * s;
+                u0[(2 * pp + 1) * N + x] = z.y * s;
+                if (w0) {
+                    w0[(2 * pp) * N + x] = oa[u] * s;
+                    w0[(2 * pp + 1) * N + x] = ob[u] * s;
+                }
+            }
         }
     }
 }
@@ -110,15 +183,27 @@ __global__ void k_fft_y(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
     int kx0 = chunk * W;
     int w_valid = H - kx0 < W ? H - kx0 : W;
     C2<T> *base = spec + plane * (int64_t)N * Hs;
-    for (int i = threadIdx.x; i < W * N; i += blockDim.x) {
-        int w = i % W, y = i / W;
-        if (w < w_valid) buf[w * P + pidx(INV ? y : pl.rev[y])] = base[(int64_t)y * Hs + kx0 + w];
+    const Divider dW = make_divider(W);
+    for (int i0 = threadIdx.x; i0 < W * N; i0 += kLoadBatch * blockDim.x) {
+        C2<T> v[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int i = i0 + u * blockDim.x;
+            const int y = div_of(i, dW), w = i - y * W;
+            if (i < W * N && w < w_valid) v[u] = base[(int64_t)y * Hs + kx0 + w];
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int i = i0 + u * blockDim.x;
+            const int y = div_of(i, dW), w = i - y * W;
+            if (i < W * N && w < w_valid) buf[w * P + pidx(INV ? y : pl.rev[y])] = v[u];
+        }
     }
     __syncthreads();
     if (INV) fft_lds_dif<T, true>(buf, w_valid, P, pl);
     else fft_lds<T, false>(buf, w_valid, P, pl);
     for (int i = threadIdx.x; i < W * N; i += blockDim.x) {
-        int w = i % W, y = i / W;
+        const int y = div_of(i, dW), w = i - y * W;
         if (w < w_valid) base[(int64_t)y * Hs + kx0 + w] = buf[w * P + pidx(INV ? pl.rev[y] : y)];
     }
 }
@@ -154,12 +239,24 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
     int kx0 = chunk * W;
     int w_valid = H - kx0 < W ? H - kx0 : W;
     for (int i = threadIdx.x; i < 2 * nek + 2; i += blockDim.x) shell[i] = 0.0;
-    for (int i = threadIdx.x; i < 3 * W * N; i += blockDim.x) {
-        int w = i % W, rest = i / W;
-        int z = rest % N, c = rest / N;
-        C2<T> v = {0, 0};
-        if (w < w_valid) v = spec[c * N3h + ((int64_t)z * N + y) * Hs + kx0 + w];
-        buf[(c * W + w) * P + pidx(pl.rev[z])] = v;
+    const Divider dW = make_divider(W), dN = make_divider(N);
+    for (int i0 = threadIdx.x; i0 < 3 * W * N; i0 += kLoadBatch * blockDim.x) {
+        C2<T> v[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int i = i0 + u * blockDim.x;
+            const int rest = div_of(i, dW), w = i - rest * W;
+            const int c = div_of(rest, dN), z = rest - c * N;
+            v[u] = C2<T>{0, 0};
+            if (i < 3 * W * N && w < w_valid) v[u] = spec[c * N3h + ((int64_t)z * N + y) * Hs + kx0 + w];
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int i = i0 + u * blockDim.x;
+            const int rest = div_of(i, dW), w = i - rest * W;
+            const int c = div_of(rest, dN), z = rest - c * N;
+            if (i < 3 * W * N) buf[(c * W + w) * P + pidx(pl.rev[z])] = v[u];
+        }
     }
     __syncthreads();
     fft_lds<T, false>(buf, 3 * W, P, pl);
@@ -250,8 +347,8 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
     __syncthreads();
     fft_lds_dif<T, true>(buf, 3 * W, P, pl);
     for (int i = threadIdx.x; i < 3 * W * N; i += blockDim.x) {
-        int w = i % W, rest = i / W;
-        int z = rest % N, c = rest / N;
+        const int rest = div_of(i, dW), w = i - rest * W;
+        const int c = div_of(rest, dN), z = rest - c * N;
         if (w < w_valid)
             spec[c * N3h + ((int64_t)z * N + y) * Hs + kx0 + w] = buf[(c * W + w) * P + pidx(pl.rev[z])];
     }
