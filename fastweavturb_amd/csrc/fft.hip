@@ -15,7 +15,7 @@
 // Nyquist index the partner -k has the same wavenumber -N/2, and the projection works out to
 // zeroing that wavenumber component in the numerator while keeping it in k^2.
 //
-// The 1-D FFTs run in LDS: iterative in-place decimation in time over radices 8/4/3/2, data
+// The 1-D FFTs run in LDS: iterative in-place decimation in time over radices 16/8/4/3/2, data
 // loaded in mixed-radix digit-reversed order, twiddles from a table exp(-2 pi i k / N).
 #include <algorithm>
 #include <cstdio>
@@ -418,10 +418,10 @@ bool fft_size_supported(int N) {
     return n == 1;
 }
 
-// Digit-reversal permutation matching fft_lds's pass order (radix[0] first).
-void fft_reversal(int N, std::vector<int32_t> &rev) {
+// Digit-reversal permutation matching fft_lds's pass order (radix[0] first) for make_plan(N, r16).
+void fft_reversal(int N, std::vector<int32_t> &rev, bool r16) {
     FftPlan pl;
-    make_plan(N, pl);
+    make_plan(N, pl, r16);
     rev.resize(N);
     for (int i = 0; i < N; i++) {
         int x = i, pos = 0, mult = N;
@@ -525,7 +525,7 @@ static hipError_t spectral_group(wt_ctx *c, int mode, int pos, int K, bool timed
     const wt::Instance &I0 = c->inst[c->order[pos]];
     const int N = I0.N, H = N / 2 + 1;
     FftPlan pl;
-    make_plan(N, pl);
+    make_plan(N, pl, true);
     pl.tw = twiddles_for(c, N);
     if (!pl.tw) return hipErrorInvalidValue;
     pl.rev = (const int32_t *)((const char *)pl.tw + sizeof(C2<T>) * N);
@@ -645,7 +645,7 @@ static hipError_t tune_group(wt_ctx *c, int pos, int K, FftTiles &best, std::str
     const wt::Instance &I0 = c->inst[c->order[pos]];
     const int N = I0.N, H = N / 2 + 1, nek = I0.nek;
     FftPlan pl;
-    make_plan(N, pl);
+    make_plan(N, pl, true);
     pl.tw = twiddles_for(c, N);
     if (!pl.tw) return hipErrorInvalidValue;
     pl.rev = (const int32_t *)((const char *)pl.tw + sizeof(C2<T>) * N);

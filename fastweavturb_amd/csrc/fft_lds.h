@@ -1,5 +1,5 @@
 // In-LDS 1-D FFTs shared by the spectral stage (fft.hip) and the fBb centerline generator
-// (fbb.hip): iterative in-place mixed-radix (8/4/3/2) transforms of lines staged in LDS, with
+// (fbb.hip): iterative in-place mixed-radix (16/8/4/3/2) transforms of lines staged in LDS, with
 // twiddles exp(-2 pi i k / N) and a digit-reversal table from a plan (make_plan, fft_reversal).
 #pragma once
 
@@ -109,6 +109,35 @@ __device__ __forceinline__ void dft_small(C2<T> *v) {
         v[6] = csub(e[2], o2);
         v[3] = cadd(e[3], o3);
         v[7] = csub(e[3], o3);
+    } else if (R == 16) {
+        // 16 = 4 x 4: DFTs of the four decimated sequences v[n2], v[n2 + 4], ..., their outputs
+        // k1 times w16^(n2 k1), then DFTs over n2; X[k1 + 4 k2] lands in v[k1 + 4 k2]
+        const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, h = (T)0.70710678118654752440;
+        const T sg = INV ? (T)1 : (T)-1;   // sign of the imaginary part of w16^m
+        C2<T> a[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; n2++) {
+#pragma unroll
+            for (int n1 = 0; n1 < 4; n1++) a[n2][n1] = v[4 * n1 + n2];
+            dft4<T, INV>(a[n2]);
+        }
+        // w16^m for m = n2 k1 in {1, 2, 3, 4, 6, 9}
+        a[1][1] = cmul(a[1][1], C2<T>{c1, sg * s1});
+        a[1][2] = cmul(a[1][2], C2<T>{h, sg * h});
+        a[1][3] = cmul(a[1][3], C2<T>{s1, sg * c1});
+        a[2][1] = cmul(a[2][1], C2<T>{h, sg * h});
+        a[2][2] = mul_mi<T, INV>(a[2][2]);
+        a[2][3] = cmul(a[2][3], C2<T>{-h, sg * h});
+        a[3][1] = cmul(a[3][1], C2<T>{s1, sg * c1});
+        a[3][2] = cmul(a[3][2], C2<T>{-h, sg * h});
+        a[3][3] = cmul(a[3][3], C2<T>{-c1, -sg * s1});
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            C2<T> b[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+            dft4<T, INV>(b);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = b[k2];
+        }
     }
 }
 
@@ -160,6 +189,7 @@ __device__ void fft_lds(C2<T> *buf, int nlines, int ldb, const FftPlan &pl) {
     for (int p = 0; p < pl.npass; p++) {
         int R = pl.radix[p];
         switch (R) {
+            case 16: fft_pass<T, 16, INV>(buf, nlines, ldb, pl.N, L, tw); break;
             case 8: fft_pass<T, 8, INV>(buf, nlines, ldb, pl.N, L, tw); break;
             case 4: fft_pass<T, 4, INV>(buf, nlines, ldb, pl.N, L, tw); break;
             case 3: fft_pass<T, 3, INV>(buf, nlines, ldb, pl.N, L, tw); break;
@@ -220,6 +250,7 @@ __device__ void fft_lds_dif(C2<T> *buf, int nlines, int ldb, const FftPlan &pl) 
         int R = pl.radix[p];
         L /= R;
         switch (R) {
+            case 16: fft_pass_dif<T, 16, INV>(buf, nlines, ldb, pl.N, L, tw); break;
             case 8: fft_pass_dif<T, 8, INV>(buf, nlines, ldb, pl.N, L, tw); break;
             case 4: fft_pass_dif<T, 4, INV>(buf, nlines, ldb, pl.N, L, tw); break;
             case 3: fft_pass_dif<T, 3, INV>(buf, nlines, ldb, pl.N, L, tw); break;
@@ -229,15 +260,27 @@ __device__ void fft_lds_dif(C2<T> *buf, int nlines, int ldb, const FftPlan &pl) 
     }
 }
 
-// Radices of an N-point transform, largest first (N = 2^a 3^b).
-inline void make_plan(int N, FftPlan &pl) {
-    pl.N = N;
-    pl.npass = 0;
-    int n = N;
-    while (n % 8 == 0) { pl.radix[pl.npass++] = 8; n /= 8; }
-    while (n % 4 == 0) { pl.radix[pl.npass++] = 4; n /= 4; }
-    while (n % 2 == 0) { pl.radix[pl.npass++] = 2; n /= 2; }
-    while (n % 3 == 0) { pl.radix[pl.npass++] = 3; n /= 3; }
+// Radices of an N-point transform, largest first (N = 2^a 3^b). With r16, radix-16 passes are
+// used where they make fewer passes than radix 8 (1024 = 16 16 4 instead of 8 8 8 2; 512 stays
+// 8 8 8): every pass reads and writes the whole line in LDS once.
+inline void make_plan(int N, FftPlan &pl, bool r16 = false) {
+    auto fill = [&](bool use16) {
+        pl.N = N;
+        pl.npass = 0;
+        int n = N;
+        if (use16)
+            while (n % 16 == 0) { pl.radix[pl.npass++] = 16; n /= 16; }
+        while (n % 8 == 0) { pl.radix[pl.npass++] = 8; n /= 8; }
+        while (n % 4 == 0) { pl.radix[pl.npass++] = 4; n /= 4; }
+        while (n % 2 == 0) { pl.radix[pl.npass++] = 2; n /= 2; }
+        while (n % 3 == 0) { pl.radix[pl.npass++] = 3; n /= 3; }
+    };
+    fill(false);
+    if (r16) {
+        const int n8 = pl.npass;
+        fill(true);
+        if (pl.npass >= n8) fill(false);
+    }
 }
 
 }  // namespace wt

@@ -214,7 +214,7 @@ int wt_create_batch(wt_ctx **out, int32_t ninst, const int32_t *n, int32_t preci
         int N = c->inst[c->order[pos]].N;
         if (std::find(c->tw_N.begin(), c->tw_N.end(), N) != c->tw_N.end()) continue;
         std::vector<int32_t> rev;
-        wt::fft_reversal(N, rev);
+        wt::fft_reversal(N, rev, true);
         size_t twb = 2 * (size_t)N * es + (size_t)N * sizeof(int32_t);
         std::vector<unsigned char> host(twb);
         for (int k = 0; k < N; k++) {

@@ -258,7 +258,7 @@ hipError_t launch_filter(wt_ctx *c, int inst, int64_t kc2);
 hipError_t spectral_autotune(wt_ctx *c, const std::vector<int32_t> &cached, std::string &report);
 bool fft_size_supported(int N);
 int64_t spec_elems_of(int N, int precision);   // complex elements of one instance's three half spectra
-void fft_reversal(int N, std::vector<int32_t> &rev);
+void fft_reversal(int N, std::vector<int32_t> &rev, bool r16 = false);   // for make_plan(N, r16)
 // stats.hip: structure functions of instance c->cur
 hipError_t launch_sf(wt_ctx *c, int mode, int pmin, int pmax, int nsep, const int32_t *d_seps,
                      double *d_out);
