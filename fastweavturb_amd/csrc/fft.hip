@@ -46,6 +46,16 @@ __device__ __forceinline__ int div_of(int i, const Divider &v) {
     return v.shift >= 0 ? i >> v.shift : (int)__umulhi((uint32_t)i, v.m);
 }
 
+// Logical tile of a block such that each XCD works through one contiguous range of tiles. The
+// hardware hands workgroup b to XCD b % 8, and each XCD has its own L2; the column tiles of the y
+// and z passes that share 128-byte lines (tiles narrower than a line) are neighbours in tile order,
+// so this keeps a line's readers on one L2 instead of fetching it once per XCD.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
+    const unsigned per = nb / kXcds, rem = nb % kXcds, x = b % kXcds, k = b / kXcds;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
 // ---- x pass, forward: two real lines -> two half spectra --------------------------------
 template <typename T>
 __global__ void k_fft_x_r2c(const T *__restrict__ f, C2<T> *__restrict__ spec, FftPlan pl_g, int pairs_per_block,
@@ -178,8 +188,9 @@ __global__ void k_fft_y(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
     const int N = pl_g.N, H = N / 2 + 1, P = lds_pitch(N), Hs = spec_row<T>(N);
     const FftPlan pl = stage_plan<T>(pl_g, smem_raw + sizeof(C2<T>) * (size_t)W * P);
     __syncthreads();
-    int chunk = blockIdx.x % nchunk;
-    int64_t plane = blockIdx.x / nchunk;  // c*N + z
+    const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+    int chunk = tile % nchunk;
+    int64_t plane = tile / nchunk;  // c*N + z
     int kx0 = chunk * W;
     int w_valid = H - kx0 < W ? H - kx0 : W;
     C2<T> *base = spec + plane * (int64_t)N * Hs;
@@ -233,9 +244,10 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
     const FftPlan pl = stage_plan<T>(pl_g, plan_area);
     __syncthreads();
     const int64_t N3h = (int64_t)N * N * Hs;
-    int chunk = blockIdx.x % nchunk;
-    int y = (blockIdx.x / nchunk) % N;
-    spec += (int64_t)3 * N3h * (blockIdx.x / ((int64_t)nchunk * N));  // instance of the group
+    const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+    int chunk = tile % nchunk;
+    int y = (tile / nchunk) % N;
+    spec += (int64_t)3 * N3h * (tile / ((int64_t)nchunk * N));  // instance of the group
     int kx0 = chunk * W;
     int w_valid = H - kx0 < W ? H - kx0 : W;
     for (int i = threadIdx.x; i < 2 * nek + 2; i += blockDim.x) shell[i] = 0.0;
@@ -340,7 +352,7 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
         shell[2 * nek + 1] = b;
     }
     __syncthreads();
-    double *row = partials + (int64_t)blockIdx.x * (2 * nek + 2);
+    double *row = partials + (int64_t)tile * (2 * nek + 2);
     for (int i = threadIdx.x; i < 2 * nek + 2; i += blockDim.x) row[i] = shell[i];
     if (mode != 0) return;
     }
