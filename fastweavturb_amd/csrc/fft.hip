@@ -232,17 +232,54 @@ __device__ __forceinline__ int shell_of(int64_t k2) {
 // ---- z pass: forward, Biot-Savart (mode 0), spectrum only (mode 1) or the sharp spectral
 // filter (mode 2: modes with k2 > kc2 set to zero, filter_spec, subroutine.f90:1225-1277),
 // inverse. Partials row per block (modes 0 and 1): [0, nek) shells of w_hat (mode 0) |
-// [nek, 2nek) shells of u_hat | [2nek] sum |w_hat|^2 weighted | [2nek+1] sum |u_hat|^2 weighted
+// [nek, 2nek) shells of u_hat | [2nek] sum |w_hat|^2 weighted | [2nek+1] sum |u_hat|^2 weighted.
+// One component's tile is in LDS at a time: each is loaded, transformed and copied to registers
+// (thread t holds items i = t + k blockDim, i = w N + z, of every component), so the pointwise
+// step has all three components at hand; the results go back through LDS one component at a
+// time for the inverse transform. kZItems bounds the items per thread (W N <= kZItems blockDim).
+template <typename T>
+constexpr int kZItems = sizeof(T) == 4 ? 16 : 4;
+
+template <typename T>
+__device__ __forceinline__ void z_load_tile(C2<T> *buf, const C2<T> *__restrict__ col, int64_t zstride, int W,
+                                            int w_valid, int N, int P, const int32_t *rev, const Divider &dW) {
+    for (int i0 = threadIdx.x; i0 < W * N; i0 += kLoadBatch * blockDim.x) {
+        C2<T> v[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int i = i0 + u * blockDim.x;
+            const int z = div_of(i, dW), w = i - z * W;
+            v[u] = C2<T>{0, 0};
+            if (i < W * N && w < w_valid) v[u] = col[z * zstride + w];
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int i = i0 + u * blockDim.x;
+            const int z = div_of(i, dW), w = i - z * W;
+            if (i < W * N) buf[w * P + pidx(rev[z])] = v[u];
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void z_store_tile(const C2<T> *buf, C2<T> *__restrict__ col, int64_t zstride, int W,
+                                             int w_valid, int N, int P, const int32_t *rev, const Divider &dW) {
+    for (int i = threadIdx.x; i < W * N; i += blockDim.x) {
+        const int z = div_of(i, dW), w = i - z * W;
+        if (w < w_valid) col[z * zstride + w] = buf[w * P + pidx(rev[z])];
+    }
+}
+
 template <typename T>
 __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchunk, int mode, int nek,
                         double *__restrict__ partials, int64_t kc2) {
+    constexpr int KM = kZItems<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    C2<T> *buf = (C2<T> *)smem_raw;                       // 3 * W * P
+    C2<T> *buf = (C2<T> *)smem_raw;                       // W * P
     const int N = pl_g.N, H = N / 2 + 1, P = lds_pitch(N), Hs = spec_row<T>(N);
-    unsigned char *plan_area = smem_raw + sizeof(C2<T>) * (size_t)3 * W * P;
+    unsigned char *plan_area = smem_raw + sizeof(C2<T>) * (size_t)W * P;
     double *shell = (double *)(plan_area + plan_lds_bytes(N, sizeof(C2<T>)));   // 2 * nek + 2
     const FftPlan pl = stage_plan<T>(pl_g, plan_area);
-    __syncthreads();
     const int64_t N3h = (int64_t)N * N * Hs;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     int chunk = tile % nchunk;
@@ -252,50 +289,57 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
     int w_valid = H - kx0 < W ? H - kx0 : W;
     for (int i = threadIdx.x; i < 2 * nek + 2; i += blockDim.x) shell[i] = 0.0;
     const Divider dW = make_divider(W), dN = make_divider(N);
-    for (int i0 = threadIdx.x; i0 < 3 * W * N; i0 += kLoadBatch * blockDim.x) {
-        C2<T> v[kLoadBatch];
-#pragma unroll
-        for (int u = 0; u < kLoadBatch; u++) {
-            const int i = i0 + u * blockDim.x;
-            const int rest = div_of(i, dW), w = i - rest * W;
-            const int c = div_of(rest, dN), z = rest - c * N;
-            v[u] = C2<T>{0, 0};
-            if (i < 3 * W * N && w < w_valid) v[u] = spec[c * N3h + ((int64_t)z * N + y) * Hs + kx0 + w];
-        }
-#pragma unroll
-        for (int u = 0; u < kLoadBatch; u++) {
-            const int i = i0 + u * blockDim.x;
-            const int rest = div_of(i, dW), w = i - rest * W;
-            const int c = div_of(rest, dN), z = rest - c * N;
-            if (i < 3 * W * N) buf[(c * W + w) * P + pidx(pl.rev[z])] = v[u];
-        }
-    }
-    __syncthreads();
-    fft_lds<T, false>(buf, 3 * W, P, pl);
+    const int64_t zstride = (int64_t)N * Hs;
+    C2<T> *col0 = spec + (int64_t)y * Hs + kx0;           // component c: col0 + c * N3h
     const int ky = wavenum(y, N);
-    if (mode == 2) {
-        for (int i = threadIdx.x; i < w_valid * N; i += blockDim.x) {
-            int w = i / N, z = i - w * N;
-            int kx = wavenum(kx0 + w, N), kz = wavenum(z, N);
-            if ((int64_t)kx * kx + (int64_t)ky * ky + (int64_t)kz * kz > kc2) {
-                const int zp = pidx(z);
+    __syncthreads();
+    if (mode == 2) {   // each component on its own: forward, mask, inverse
+        for (int c = 0; c < 3; c++) {
+            z_load_tile<T>(buf, col0 + c * N3h, zstride, W, w_valid, N, P, pl.rev, dW);
+            __syncthreads();
+            fft_lds<T, false>(buf, W, P, pl);
+            for (int i = threadIdx.x; i < w_valid * N; i += blockDim.x) {
+                const int w = div_of(i, dN), z = i - w * N;
+                const int kx = wavenum(kx0 + w, N), kz = wavenum(z, N);
+                if ((int64_t)kx * kx + (int64_t)ky * ky + (int64_t)kz * kz > kc2) buf[w * P + pidx(z)] = C2<T>{0, 0};
+            }
+            __syncthreads();
+            fft_lds_dif<T, true>(buf, W, P, pl);
+            z_store_tile<T>(buf, col0 + c * N3h, zstride, W, w_valid, N, P, pl.rev, dW);
+            __syncthreads();
+        }
+        return;
+    }
+    // forward transforms, one component at a time, into registers
+    C2<T> R[3][KM];
+    for (int c = 0; c < 3; c++) {
+        z_load_tile<T>(buf, col0 + c * N3h, zstride, W, w_valid, N, P, pl.rev, dW);
+        __syncthreads();
+        fft_lds<T, false>(buf, W, P, pl);
 #pragma unroll
-                for (int c = 0; c < 3; c++) buf[(c * W + w) * P + zp] = C2<T>{0, 0};
+        for (int k = 0; k < KM; k++) {
+            const int i = threadIdx.x + k * blockDim.x;
+            if (i < W * N) {
+                const int w = div_of(i, dN), z = i - w * N;
+                R[c][k] = buf[w * P + pidx(z)];
             }
         }
-    } else {
-    // pointwise: (w, kz) pairs
+        __syncthreads();
+    }
+    // pointwise: (w, kz) items
     double e_w = 0.0, e_u = 0.0;
-    for (int i = threadIdx.x; i < w_valid * N; i += blockDim.x) {
-        int w = i / N, z = i - w * N;
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int i = threadIdx.x + k * blockDim.x;
+        if (i >= w_valid * N) continue;
+        const int w = div_of(i, dN), z = i - w * N;
         int kxi = kx0 + w;
         int kx = wavenum(kxi, N), kz = wavenum(z, N);
         int64_t k2 = (int64_t)kx * kx + (int64_t)ky * ky + (int64_t)kz * kz;
         double wt = (kxi == 0 || kxi == N / 2) ? 1.0 : 2.0;   // half-spectrum multiplicity
-        const int zp = pidx(z);
-        C2<T> a = buf[(0 * W + w) * P + zp], b = buf[(1 * W + w) * P + zp], c = buf[(2 * W + w) * P + zp];
+        const C2<T> a = R[0][k], b = R[1][k], cc = R[2][k];
         double ew = (double)a.x * a.x + (double)a.y * a.y + (double)b.x * b.x + (double)b.y * b.y +
-                    (double)c.x * c.x + (double)c.y * c.y;
+                    (double)cc.x * cc.x + (double)cc.y * cc.y;
         int sh = shell_of(k2);
         if (mode == 0) {
             e_w += wt * ew;
@@ -310,8 +354,8 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
             } else {
                 T ik2 = (T)(1.0 / (double)k2);
                 // cross = q x w
-                C2<T> cx = {qy * c.x - qz * b.x, qy * c.y - qz * b.y};
-                C2<T> cy = {qz * a.x - qx * c.x, qz * a.y - qx * c.y};
+                C2<T> cx = {qy * cc.x - qz * b.x, qy * cc.y - qz * b.y};
+                C2<T> cy = {qz * a.x - qx * cc.x, qz * a.y - qx * cc.y};
                 C2<T> cz = {qx * b.x - qy * a.x, qx * b.y - qy * a.y};
                 ux = C2<T>{-cx.y * ik2, cx.x * ik2};
                 uy = C2<T>{-cy.y * ik2, cy.x * ik2};
@@ -321,9 +365,9 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
                         (double)uz.x * uz.x + (double)uz.y * uz.y;
             e_u += wt * eu;
             if (sh >= 1 && sh <= nek) atomicAdd(shell + nek + sh - 1, 0.5 * wt * eu);
-            buf[(0 * W + w) * P + zp] = ux;
-            buf[(1 * W + w) * P + zp] = uy;
-            buf[(2 * W + w) * P + zp] = uz;
+            R[0][k] = ux;
+            R[1][k] = uy;
+            R[2][k] = uz;
         } else {
             e_u += wt * ew;
             if (sh >= 1 && sh <= nek) atomicAdd(shell + nek + sh - 1, 0.5 * wt * ew);
@@ -355,14 +399,20 @@ __global__ void k_fft_z(C2<T> *__restrict__ spec, FftPlan pl_g, int W, int nchun
     double *row = partials + (int64_t)tile * (2 * nek + 2);
     for (int i = threadIdx.x; i < 2 * nek + 2; i += blockDim.x) row[i] = shell[i];
     if (mode != 0) return;
-    }
-    __syncthreads();
-    fft_lds_dif<T, true>(buf, 3 * W, P, pl);
-    for (int i = threadIdx.x; i < 3 * W * N; i += blockDim.x) {
-        const int rest = div_of(i, dW), w = i - rest * W;
-        const int c = div_of(rest, dN), z = rest - c * N;
-        if (w < w_valid)
-            spec[c * N3h + ((int64_t)z * N + y) * Hs + kx0 + w] = buf[(c * W + w) * P + pidx(pl.rev[z])];
+    // inverse transforms, one component at a time
+    for (int c = 0; c < 3; c++) {
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const int i = threadIdx.x + k * blockDim.x;
+            if (i < W * N) {
+                const int w = div_of(i, dN), z = i - w * N;
+                buf[w * P + pidx(z)] = R[c][k];
+            }
+        }
+        __syncthreads();
+        fft_lds_dif<T, true>(buf, W, P, pl);
+        z_store_tile<T>(buf, col0 + c * N3h, zstride, W, w_valid, N, P, pl.rev, dW);
+        __syncthreads();
     }
 }
 
@@ -470,7 +520,11 @@ static int lds_limit(wt_ctx *c) {
 static size_t lds_x(int N, int ppb, size_t cs) { return (size_t)ppb * lds_pitch(N) * cs + plan_lds_bytes(N, cs); }
 static size_t lds_y(int N, int wy, size_t cs) { return (size_t)wy * lds_pitch(N) * cs + plan_lds_bytes(N, cs); }
 static size_t lds_z(int N, int wz, size_t cs, int nek) {
-    return 3 * (size_t)wz * lds_pitch(N) * cs + plan_lds_bytes(N, cs) + sizeof(double) * (2 * nek + 2);
+    return (size_t)wz * lds_pitch(N) * cs + plan_lds_bytes(N, cs) + sizeof(double) * (2 * nek + 2);
+}
+// whether a z tile of wz columns fits the registers of tz threads (kZItems items per thread)
+static bool z_fits(int N, int wz, int tz, size_t cs) {
+    return (int64_t)wz * N <= (int64_t)(cs == sizeof(C2<float>) ? kZItems<float> : kZItems<double>) * tz;
 }
 
 // The shapes used without tuning: as many lines/columns as fit, up to 8192 points (x) or 16
@@ -486,6 +540,8 @@ static FftTiles default_tiles(wt_ctx *c, int N, int nek) {
     t.wz = 1;
     while (t.wz < 16 && lds_z(N, 2 * t.wz, cs, nek) + kZStatic <= lim) t.wz *= 2;
     t.tx = t.ty = t.tz = 256;
+    while (!z_fits(N, t.wz, t.tz, cs) && t.tz < 1024) t.tz *= 2;
+    while (!z_fits(N, t.wz, t.tz, cs) && t.wz > 1) t.wz /= 2;
     return t;
 }
 
@@ -493,7 +549,11 @@ static FftTiles tiles_for(wt_ctx *c, int N, int nek) {
     for (size_t i = 0; i + 7 <= c->fft_tiles.size(); i += 7)
         if (c->fft_tiles[i] == N) {
             const int32_t *v = &c->fft_tiles[i + 1];
-            return FftTiles{v[0], v[1], v[2], v[3], v[4], v[5]};
+            FftTiles t{v[0], v[1], v[2], v[3], v[4], v[5]};
+            const size_t cs = c->precision == 64 ? sizeof(C2<double>) : sizeof(C2<float>);
+            while (!z_fits(N, t.wz, t.tz, cs) && t.tz < 1024) t.tz *= 2;   // shapes cached for other kernels
+            while (!z_fits(N, t.wz, t.tz, cs) && t.wz > 1) t.wz /= 2;
+            return t;
         }
     return default_tiles(c, N, nek);
 }
@@ -582,6 +642,7 @@ static hipError_t spectral_group(wt_ctx *c, int mode, int pos, int K, bool timed
         if (need > c->partials_cap) return hipErrorInvalidValue;
     }
     double *colsum = c->partials ? c->partials + (size_t)zblocks * cols : nullptr;
+    if (!z_fits(N, Wz, tl.tz, cs)) return hipErrorInvalidConfiguration;
     WT_HIP(hipFuncSetAttribute((const void *)k_fft_z<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shz));
     if (timed) kt_begin(c, kSlotFftZ);
     hipLaunchKernelGGL(k_fft_z<T>, dim3((unsigned)zblocks), dim3(tl.tz), shz, st, spec, pl, Wz, ncz, mode, nek,
@@ -730,11 +791,15 @@ static hipError_t tune_group(wt_ctx *c, int pos, int K, FftTiles &best, std::str
     for (int wz = 1; wz <= 16 && err == hipSuccess; wz *= 2) {
         const size_t sh = lds_z(N, wz, cs, nek);
         if (sh + kZStatic > lim) continue;
+        bool any = false;
+        for (int tz : threads) any = any || z_fits(N, wz, tz, cs);
+        if (!any) continue;
         err = hipFuncSetAttribute((const void *)k_fft_z<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         const int ncz = (H + wz - 1) / wz;
         const int64_t zblocks = (int64_t)N * ncz * K;
         for (int tz : threads) {
             if (err != hipSuccess) break;
+            if (!z_fits(N, wz, tz, cs)) continue;
             err = timed([&] { hipLaunchKernelGGL(k_fft_z<T>, dim3((unsigned)zblocks), dim3(tz), sh, st, spec, pl, wz, ncz,
                                                  0, nek, c->partials, (int64_t)0); }, &t);
             if (err == hipSuccess && t < bz) { bz = t; best.wz = wz; best.tz = tz; }
