@@ -4,7 +4,7 @@
 // bricks at the upper edge when a brick edge does not divide N). A sub-segment is paired with every brick that its
 // integer bounding box (in extended indices, wrapped periodically) overlaps, once per periodic
 // image of the brick that the box meets, unless that part of the box lies entirely outside the
-// slab between the sub-segment's two end planes (pair_meets_slab). The deposition
+// slab between the sub-segment's two end planes or beyond the reach of its hits (pair_meets_slab). The deposition
 // kernel then owns one brick at a time and accumulates all of its pairs in LDS, so no global
 // atomics touch the field.
 //
@@ -126,8 +126,14 @@ struct AxisWalk {
 // rounding of either in the reference's expression for any cell), receive no contribution from
 // the sub-segment and get no pair. Both extremes are sums of one term per axis (slab_terms),
 // added in the order x, y, z.
+// A hit X also lies within reach of ci0: with rho < R its distance from the chord line and t0 its
+// offset along tci0, t1 = (t0 - l) cos(delta) + rho sin(delta) (e . f) < 0 (delta: the angle
+// between tci0 and tci1, l = ndci0) gives 0 <= t0 < l + R tan(delta), so
+// |X - ci0|^2 = rho^2 + t0^2 < R^2 + (l + R tan(delta))^2. Brick parts whose nearest point is
+// farther get no pair either (the squared distance is a sum of one term per axis, the bound
+// widened by 1e-9 relative and 1e-12 absolute; no bound when the chords turn by 60 degrees or more).
 __device__ __forceinline__ void slab_terms(const SubSeg &sg, int c, int32_t bi, int32_t im, int32_t bd, double &h0,
-                                           double &l1) {
+                                           double &l1, double &d2) {
     const int32_t org = bi * bd, ext = sg.N - org < bd ? sg.N - org : bd;
     const int32_t sh = (im - 1) * sg.N;
     const int32_t a = sg.lo[c] > org + 1 + sh ? sg.lo[c] : org + 1 + sh;
@@ -135,16 +141,27 @@ __device__ __forceinline__ void slab_terms(const SubSeg &sg, int c, int32_t bi, 
     const double xa = mesh(a, sg), xb = mesh(b, sg);
     h0 = ((sg.tci0[c] > 0.0 ? xb : xa) - sg.ci0[c]) * sg.tci0[c];
     l1 = ((sg.tci1[c] > 0.0 ? xa : xb) - sg.ci1[c]) * sg.tci1[c];
+    const double d = fmax(fmax(xa - sg.ci0[c], sg.ci0[c] - xb), 0.0);
+    d2 = d * d;
 }
 
 __device__ __forceinline__ bool slab_met(double t0, double t1) { return t0 >= -1e-12 && t1 < 1e-12; }
 
+// Squared reach of the sub-segment's hits from ci0 (above).
+__device__ __forceinline__ double hit_reach2(const SubSeg &sg) {
+    const double cd = (sg.tci0[0] * sg.tci1[0] + sg.tci0[1] * sg.tci1[1]) + sg.tci0[2] * sg.tci1[2];
+    if (!(cd > 0.5)) return INFINITY;
+    const double tand = sqrt(fmax(1.0 - cd * cd, 0.0)) / cd;
+    const double e = sg.ndci0 + sg.R * tand;
+    return (sg.R * sg.R + e * e) * (1.0 + 1e-9) + 1e-12;
+}
+
 __device__ __forceinline__ bool pair_meets_slab(const SubSeg &sg, const int32_t *bi, const int32_t *im,
-                                                const int32_t *bd) {
-    double h0[3], l1[3];
+                                                const int32_t *bd, double reach2) {
+    double h0[3], l1[3], d2[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) slab_terms(sg, c, bi[c], im[c], bd[c], h0[c], l1[c]);
-    return slab_met((h0[0] + h0[1]) + h0[2], (l1[0] + l1[1]) + l1[2]);
+    for (int c = 0; c < 3; c++) slab_terms(sg, c, bi[c], im[c], bd[c], h0[c], l1[c], d2[c]);
+    return slab_met((h0[0] + h0[1]) + h0[2], (l1[0] + l1[1]) + l1[2]) && (d2[0] + d2[1]) + d2[2] < reach2;
 }
 
 // A sub-segment with more pairs than this (a box of many bricks: the widest cores at high N) is
@@ -157,7 +174,7 @@ constexpr int32_t kBigPairs = 256;
 template <bool SLAB>
 struct EntryWalk {
     AxisWalk wk[3];
-    double h0y = 0.0, l1y = 0.0, h0z = 0.0, l1z = 0.0;
+    double h0y = 0.0, l1y = 0.0, h0z = 0.0, l1z = 0.0, d2y = 0.0, d2z = 0.0, reach2 = 0.0;
     int32_t jx = 0, jy = 0;
     int64_t rowkey = 0;
     __device__ void start(const AxisImages *ab, const SubSeg &sg, const int32_t *bd, int64_t brick0, int32_t nbx,
@@ -165,8 +182,9 @@ struct EntryWalk {
 #pragma unroll
         for (int c = 0; c < 3; c++) wk[c].start(ab[c]);
         if (SLAB) {
-            slab_terms(sg, 1, wk[1].bi, wk[1].im, bd[1], h0y, l1y);
-            slab_terms(sg, 2, wk[2].bi, wk[2].im, bd[2], h0z, l1z);
+            reach2 = hit_reach2(sg);
+            slab_terms(sg, 1, wk[1].bi, wk[1].im, bd[1], h0y, l1y, d2y);
+            slab_terms(sg, 2, wk[2].bi, wk[2].im, bd[2], h0z, l1z, d2z);
         }
         rowkey = brick0 + ((int64_t)wk[2].bi * nby + wk[1].bi) * nbx;
     }
@@ -176,9 +194,9 @@ struct EntryWalk {
                          int32_t nby, int32_t nx, int32_t ny, int32_t t, int32_t n, int64_t &key, int32_t &code) {
         bool met = true;
         if (SLAB) {
-            double h0x, l1x;
-            slab_terms(sg, 0, wk[0].bi, wk[0].im, bd[0], h0x, l1x);
-            met = slab_met((h0x + h0y) + h0z, (l1x + l1y) + l1z);
+            double h0x, l1x, d2x;
+            slab_terms(sg, 0, wk[0].bi, wk[0].im, bd[0], h0x, l1x, d2x);
+            met = slab_met((h0x + h0y) + h0z, (l1x + l1y) + l1z) && (d2x + d2y) + d2z < reach2;
         }
         key = rowkey + wk[0].bi;
         code = wk[0].im + 3 * wk[1].im + 9 * wk[2].im;
@@ -193,9 +211,9 @@ struct EntryWalk {
                 jy = 0;
                 wk[1].start(ab[1]);
                 wk[2].next(ab[2]);
-                if (SLAB) slab_terms(sg, 2, wk[2].bi, wk[2].im, bd[2], h0z, l1z);
+                if (SLAB) slab_terms(sg, 2, wk[2].bi, wk[2].im, bd[2], h0z, l1z, d2z);
             }
-            if (SLAB) slab_terms(sg, 1, wk[1].bi, wk[1].im, bd[1], h0y, l1y);
+            if (SLAB) slab_terms(sg, 1, wk[1].bi, wk[1].im, bd[1], h0y, l1y, d2y);
             rowkey = brick0 + ((int64_t)wk[2].bi * nby + wk[1].bi) * nbx;
         }
         return met;
@@ -354,12 +372,13 @@ __global__ __launch_bounds__(256) void k_pairs_big(Geo g, Run r0, uint32_t *__re
 #pragma unroll
         for (int c = 0; c < 3; c++) ab[c] = axis_images(sg.lo[c], sg.hi[c], sg.N, r.lbd[c]);
         const int32_t nx = ab[0].count(), ny = ab[1].count(), n = nx * ny * ab[2].count();
+        const double reach2 = hit_reach2(sg);
         for (int32_t t = lane; t < n; t += 64) {
             int32_t bi[3], im[3];
             ab[0].at(t % nx, bi[0], im[0]);
             ab[1].at((t / nx) % ny, bi[1], im[1]);
             ab[2].at(t / (nx * ny), bi[2], im[2]);
-            if (!pair_meets_slab(sg, bi, im, r.bd)) continue;
+            if (!pair_meets_slab(sg, bi, im, r.bd, reach2)) continue;
             const int64_t key = r.brick0 + ((int64_t)bi[2] * r.nb[1] + bi[1]) * r.nb[0] + bi[0];
             if (FILL) {
                 const uint32_t slot = atomicAdd(wide + key, 1u);   // wslot: the wide pairs lead the list
