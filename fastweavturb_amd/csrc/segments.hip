@@ -328,6 +328,7 @@ __global__ __launch_bounds__(128) void k_theta_fast(Geo g, Run, int32_t *__restr
         acc += (double)fval(wE[0][0], wE[1][0], wE[2][0], gE[0]);
         gmin = gE[0];
     }
+#pragma unroll 2
     for (int i = 0; i < npair; i++) {
         const float sv = (float)(2 * i + o) * h, v = sv * sv;
         float wp[3], wm[3];
