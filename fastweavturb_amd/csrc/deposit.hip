@@ -719,13 +719,17 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
         __syncthreads();
         const int64_t p0 = A.b.brick_off[bi], p1 = A.b.brick_off[bi + 1];
         // The brick's list starts with its H pairs of wide boxes (each a large share of the brick's
-        // work; neighbours along a tube sit next to each other). They come first, in Kh = ceil(H / 64)
-        // chunks whose chunk k holds pairs k, k + Kh, k + 2 Kh, ... (lane j: pair k + j Kh), so that
-        // they spread over the waves instead of holding one wave while the others wait at the brick's
-        // barrier; the other pairs follow in runs of 64 (neighbouring thin sub-segments of about the
-        // same cost share a wave). Chunks are claimed one step ahead, and the next chunk's pair index
-        // and image read while the current one is processed.
-        const int64_t P = p1 - p0, H = A.b.brick_wide[bi], Kh = (H + 63) >> 6, K = Kh + ((P - H + 63) >> 6);
+        // work; neighbours along a tube sit next to each other). They come first, in Kh chunks whose
+        // chunk k holds pairs k, k + Kh, k + 2 Kh, ... (lane j: pair k + j Kh), so that they spread
+        // over the waves instead of holding one wave while the others wait at the brick's barrier:
+        // Kh is ceil(H / 64) rounded up to a multiple of the workgroup's waves (every wave gets wide
+        // work; a chunk's lanes share it through the wave's queues), at most H. The other pairs follow
+        // in runs of 64 (neighbouring thin sub-segments of about the same cost share a wave). Chunks
+        // are claimed one step ahead, and the next chunk's pair index and image read while the
+        // current one is processed.
+        const int64_t P = p1 - p0, H = A.b.brick_wide[bi];
+        const int64_t Kw = (((H + 63) >> 6) + kDepWaves - 1) / kDepWaves * kDepWaves;
+        const int64_t Kh = H < Kw ? H : Kw, K = Kh + ((P - H + 63) >> 6);
         auto pair_of = [&](int64_t k) -> int64_t {
             if (k < Kh) {
                 const int64_t i = k + (int64_t)lane * Kh;
