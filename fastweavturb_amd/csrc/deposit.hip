@@ -550,7 +550,7 @@ struct Wave {
     int ox, oy, oz;
     int lane;
     int segn, candn;
-    uint32_t ncand, nhit, novf;   // wave-uniform counts (scalar registers; below 2^32 per wave)
+    uint32_t *stat;               // LDS: candidates tested, hits, overflows of this wave
     uint32_t ncols;               // columns walked by this lane
 };
 
@@ -573,7 +573,6 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
         h.kp = ax == 0 ? hq.c : (ax == 1 ? hq.b : hq.a);
     }
     W.candn = __builtin_amdgcn_readfirstlane(W.candn - cnt);
-    W.ncand += (uint32_t)cnt;
     SubSeg sg;
     fetch_owner<FAST>(me, lane < cnt ? h.owner : lane, *W.ri, sg);
     bool hit = false, ovf = false;
@@ -603,8 +602,13 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
             }
         }
     }
-    W.nhit = __builtin_amdgcn_readfirstlane(W.nhit + (uint32_t)__popcll(__ballot(hit)));
-    W.novf = __builtin_amdgcn_readfirstlane(W.novf + (uint32_t)__popcll(__ballot(ovf)));
+    // the wave's counts live in LDS, so that no loop around this stage carries them in registers
+    const uint32_t nh = (uint32_t)__popcll(__ballot(hit)), no = (uint32_t)__popcll(__ballot(ovf));
+    if (lane == 0) {
+        __hip_atomic_fetch_add(W.stat + 0, (uint32_t)cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __hip_atomic_fetch_add(W.stat + 1, nh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        if (no) __hip_atomic_fetch_add(W.stat + 2, no, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
 }
 
 // Columns walked per row segment and pass (Run::colcap): a longer segment's remainder is queued
@@ -633,7 +637,6 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
     W.segn = __builtin_amdgcn_readfirstlane(W.segn - cnt);
     const int32_t cap = W.ri->colcap;
     const int32_t bl0 = ib, steps = len < cap ? len : cap;
-    W.ncols += (uint32_t)steps;
     // two columns per lane and step (about a third of the columns hold a candidate, so the
     // queue rounds of both are shared)
     for (int32_t col = 0; __ballot(col < steps) != 0ull; col += 2, ib += 2) {
@@ -673,6 +676,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
     unsigned char *wbase = smem + sizeof(Acc) * 3 * kCells;
     constexpr size_t kWaveBytes = sizeof(ColWalk) * 64 + sizeof(RowSeg) * 128 + sizeof(HitQ) * 128;
     unsigned long long *s_misc = (unsigned long long *)(wbase + kWaveBytes * kDepWaves);  // brick, cursor
+    uint32_t *s_stat = (uint32_t *)(s_misc + 2);                                         // [kDepWaves][4]
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     Wave<F64> W;
@@ -688,7 +692,9 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
     }
     W.lane = lane;
     W.segn = W.candn = 0;
-    W.ncols = W.ncand = W.nhit = W.novf = 0;
+    W.ncols = 0;
+    W.stat = s_stat + 4 * wv;
+    if (lane < 4) W.stat[lane] = 0u;
     const Geo &g = A.g;
     for (;;) {
         if (threadIdx.x == 0) {
@@ -876,6 +882,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
             for (int32_t ic = c0; __ballot(have && ic <= c1); ic++) {
                 int32_t bl = 0, bh = -1;
                 const bool sgok = have && ic <= c1 && row_band(rb, ic, bl, bh);
+                W.ncols += sgok ? (uint32_t)(bh - bl + 1) : 0u;
                 const unsigned long long m = __ballot(sgok);
                 if (sgok) W.segs[W.segn + lanes_below(m)] =
                     RowSeg{(int16_t)ic, (int16_t)bl, (uint8_t)(bh - bl + 1), (uint8_t)lane, 0};
@@ -914,11 +921,12 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
     unsigned long long ncols = W.ncols;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) ncols += __shfl_xor(ncols, o, 64);
+    wave_sync_lds();
     if (lane == 0) {
         atomicAdd(A.counters + 0, ncols);
-        atomicAdd(A.counters + 1, (unsigned long long)W.ncand);
-        atomicAdd(A.counters + 2, (unsigned long long)W.nhit);
-        if (W.novf) atomicAdd(A.counters + 3, (unsigned long long)W.novf);
+        atomicAdd(A.counters + 1, (unsigned long long)W.stat[0]);
+        atomicAdd(A.counters + 2, (unsigned long long)W.stat[1]);
+        if (W.stat[2]) atomicAdd(A.counters + 3, (unsigned long long)W.stat[2]);
     }
 }
 
@@ -927,7 +935,8 @@ static hipError_t deposit_t(wt_ctx *c, DepArgs &A) {
     hipStream_t st = c->stream;
     using Acc = typename AccT<F64>::type;
     const size_t wave_bytes = sizeof(ColWalk) * 64 + sizeof(RowSeg) * 128 + sizeof(HitQ) * 128;
-    const size_t lds = sizeof(Acc) * 3 * kCells + wave_bytes * kDepWaves + 2 * sizeof(unsigned long long);
+    const size_t lds = sizeof(Acc) * 3 * kCells + wave_bytes * kDepWaves + 2 * sizeof(unsigned long long) +
+                       4 * sizeof(uint32_t) * kDepWaves;
     const void *fn = (const void *)k_deposit<F64, FAST>;
     WT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int cus = 256, per_cu = 0;
