@@ -107,9 +107,9 @@ __device__ __forceinline__ void subseg_load_pair(const Geo &g, const Run &ri, in
     }
 }
 
-// Number of lanes below this one whose bit is set in the wave mask m.
-__device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+// Number of lanes below this one whose bit is set in the wave mask m, plus base.
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long m, uint32_t base = 0u) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, base));
 }
 
 
@@ -349,11 +349,14 @@ struct RowSeg {
 };
 
 // A candidate cell in the axes of its pair's column walk: a along the column, b and c across it
-// (extended indices), and tag = owning lane | column axis << 6. The x, y, z indices are formed
-// when the candidate is tested, once per candidate instead of once per queue round.
+// (extended indices, stored + 32768 so that a column step adds to them without borrowing from
+// the neighbouring field), and tag = owning lane | column axis << 6: w0 = (a + 32768) | tag << 16,
+// w1 = (b + 32768) | (c + 32768) << 16. The x, y, z indices are formed when the candidate is
+// tested, once per candidate instead of once per queue round.
 struct HitQ {
-    int16_t a, tag, b, c;
+    uint32_t w0, w1;
 };
+constexpr int32_t kIdxBias = 32768;
 
 // 1/x from the hardware reciprocal refined by one Newton step (within a few ulp, never exactly
 // rounded): only for quantities that feed single-precision bounds carrying their own margins.
@@ -463,12 +466,13 @@ __device__ __forceinline__ bool row_band(const RowBand &rb, int32_t ic, int32_t 
 // db, so each plane gives one line q - B db that bounds a from below or from above, with the
 // margin already applied (q = (P - C dc) -/+ m, rounded twice before the fma that evaluates it:
 // within 1.8e-7 of |P| + |B| Wb + |C| Wc, under the margin's 3e-7). Unused slots hold B = 0 and
-// q = -/+1e9.
+// q = -/+1e9. The lower bounds are stored minus one, so that a column's count is
+// floor(hi) - ceil(lo - 1) (a rounding of lo - 1 can only lower it by one more: one extra candidate).
 struct ColLines {
-    float qL0, BL0, qL1, BL1;   // lower bounds
+    float qL0, BL0, qL1, BL1;   // lower bounds, minus one
     float qU0, BU0, qU1, BU1;   // upper bounds
-    float A0, A1;               // the pair's a-range, relative to lo_a
-    int32_t lo_a, lo_b;
+    float A0, A1;               // the pair's a-range, relative to lo_a (A0 minus one)
+    int32_t lo_a1, lo_b;        // lo_a + 1
 };
 
 __device__ __forceinline__ ColLines col_lines(const ColWalk &w, int32_t ic) {
@@ -476,28 +480,29 @@ __device__ __forceinline__ ColLines col_lines(const ColWalk &w, int32_t ic) {
     const float dcn = (float)(ic - w.lo_c);
     const float p0 = __builtin_fmaf(-w.C0, dcn, w.P0), p1 = __builtin_fmaf(-w.C1, dcn, w.P1);
     const bool up0 = w.up0, lo1 = w.use1 && !w.up1, hi1 = w.use1 && w.up1;
-    L.qL0 = up0 ? p0 - w.m0 : -1e9f;
+    L.qL0 = up0 ? (p0 - w.m0) - 1.0f : -1e9f;
     L.BL0 = up0 ? w.B0 : 0.0f;
     L.qU0 = up0 ? 1e9f : p0 + w.m0;
     L.BU0 = up0 ? 0.0f : w.B0;
-    L.qL1 = lo1 ? p1 - w.m1 : -1e9f;
+    L.qL1 = lo1 ? (p1 - w.m1) - 1.0f : -1e9f;
     L.BL1 = lo1 ? w.B1 : 0.0f;
     L.qU1 = hi1 ? p1 + w.m1 : 1e9f;
     L.BU1 = hi1 ? w.B1 : 0.0f;
-    L.A0 = (float)(w.a0 - w.lo_a);
+    L.A0 = (float)(w.a0 - w.lo_a - 1);
     L.A1 = (float)(w.a1 - w.lo_a);
-    L.lo_a = w.lo_a;
+    L.lo_a1 = w.lo_a + 1;
     L.lo_b = w.lo_b;
     return L;
 }
 
-// Candidate a-range of column ib of a row segment: returns the count, first index in *ia0.
+// Candidate a-range of column ib of a row segment: returns the count (at most one more than the
+// cells between the bounds, never fewer), the first index in *ia0.
 __device__ __forceinline__ int32_t column_range(const ColLines &L, int32_t ib, int32_t *ia0) {
     const float db = (float)(ib - L.lo_b);
     const float flo = ceilf(fmaxf(fmaxf(L.A0, __builtin_fmaf(-L.BL0, db, L.qL0)), __builtin_fmaf(-L.BL1, db, L.qL1)));
     const float fhi = floorf(fminf(fminf(L.A1, __builtin_fmaf(-L.BU0, db, L.qU0)), __builtin_fmaf(-L.BU1, db, L.qU1)));
-    *ia0 = L.lo_a + (int32_t)flo;
-    return flo <= fhi ? (int32_t)(fhi - flo) + 1 : 0;
+    *ia0 = L.lo_a1 + (int32_t)flo;
+    return (int32_t)fmaxf(fhi - flo, 0.0f);
 }
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -566,11 +571,13 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
         int32_t ip, jp, kp, owner;
     } h;
     {
-        const int ax = hq.tag >> 6;
-        h.owner = hq.tag & 63;
-        h.ip = ax == 0 ? hq.a : (ax == 1 ? hq.c : hq.b);
-        h.jp = ax == 0 ? hq.b : (ax == 1 ? hq.a : hq.c);
-        h.kp = ax == 0 ? hq.c : (ax == 1 ? hq.b : hq.a);
+        const int32_t tag = (int32_t)(hq.w0 >> 16), ax = tag >> 6;
+        const int32_t a = (int32_t)(hq.w0 & 0xffffu) - kIdxBias, b = (int32_t)(hq.w1 & 0xffffu) - kIdxBias,
+                      c = (int32_t)(hq.w1 >> 16) - kIdxBias;
+        h.owner = tag & 63;
+        h.ip = ax == 0 ? a : (ax == 1 ? c : b);
+        h.jp = ax == 0 ? b : (ax == 1 ? a : c);
+        h.kp = ax == 0 ? c : (ax == 1 ? b : a);
     }
     W.candn = __builtin_amdgcn_readfirstlane(W.candn - cnt);
     SubSeg sg;
@@ -622,7 +629,7 @@ template <bool F64, bool FAST>
 __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, int cnt) {
     const int lane = W.lane;
     int32_t len = 0, ib = 0, ic = 0, own = 0, ax = 0;
-    ColLines L;
+    ColLines L{};   // idle lanes: zero lines, their counts are masked
     if (lane < cnt) {
         const RowSeg r = W.segs[W.segn - cnt + lane];
         len = r.len;
@@ -633,27 +640,29 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
         ax = w.ax;
         L = col_lines(w, ic);
     }
-    const int16_t tag = (int16_t)(own | (ax << 6));
+    // the candidate's first word is its a-index plus wb (no carry: a + bias < 2^16)
+    const uint32_t wb = ((uint32_t)(own | (ax << 6)) << 16) + (uint32_t)kIdxBias;
     W.segn = __builtin_amdgcn_readfirstlane(W.segn - cnt);
     const int32_t cap = W.ri->colcap;
     const int32_t bl0 = ib, steps = len < cap ? len : cap;
+    // the candidate's second word for column ib: (b + bias) | (c + bias) << 16
+    uint32_t w1 = (uint32_t)(ib + kIdxBias) | ((uint32_t)(ic + kIdxBias) << 16);
     // two columns per lane and step (about a third of the columns hold a candidate, so the
     // queue rounds of both are shared)
-    for (int32_t col = 0; __ballot(col < steps) != 0ull; col += 2, ib += 2) {
-        int32_t nc0 = 0, ia0 = 0, nc1 = 0, ia1 = 0;
-        if (col < steps) nc0 = column_range(L, ib, &ia0);
-        if (col + 1 < steps) nc1 = column_range(L, ib + 1, &ia1);
-        const int32_t nc = nc0 + nc1;
+    for (int32_t col = 0; __ballot(col < steps) != 0ull; col += 2, ib += 2, w1 += 2) {
+        int32_t ia0, ia1;
+        int32_t nc0 = column_range(L, ib, &ia0), nc1 = column_range(L, ib + 1, &ia1);
+        nc0 = col < steps ? nc0 : 0;
+        nc1 = col + 1 < steps ? nc1 : 0;
+        const int32_t nc = nc0 + nc1, ia1s = ia1 - nc0;
         unsigned long long m;
         for (int u = 0; (m = __ballot(u < nc)) != 0ull; u++) {
             if (u < nc) {
                 const bool second = u >= nc0;
                 HitQ q;
-                q.a = (int16_t)((second ? ia1 - nc0 : ia0) + u);
-                q.tag = tag;
-                q.b = (int16_t)(second ? ib + 1 : ib);
-                q.c = (int16_t)ic;
-                W.candq[W.candn + lanes_below(m)] = q;
+                q.w0 = (uint32_t)((second ? ia1s : ia0) + u) + wb;
+                q.w1 = second ? w1 + 1u : w1;
+                W.candq[lanes_below(m, (uint32_t)W.candn)] = q;
             }
             W.candn = __builtin_amdgcn_readfirstlane(W.candn + __popcll(m));
             wave_sync_lds();
