@@ -214,22 +214,22 @@ __device__ __forceinline__ void hit_const_load(const Geo &g, const SubSeg &sg, H
     hc.Gamma = (float)g.gamma[sg.tube];
 }
 
-// sg.s0, sg.s1 hold zA, zB (SegRec).
-__device__ void hit_math_f32(const HitConst &hc, const SubSeg &sg, int32_t ip, int32_t jp, int32_t kp, double *v) {
+// sg.s0, sg.s1 hold zA, zB (SegRec). Returns tg and the tangent dcf; the contribution is tg dcf.
+__device__ float hit_math_f32(const HitConst &hc, const SubSeg &sg, int32_t ip, int32_t jp, int32_t kp, float *dcf) {
     const double *coe = hc.coe;
     double X[3] = {mesh(ip, sg), mesh(jp, sg), mesh(kp, sg)};
     double a[3] = {X[0] - sg.ci0[0], X[1] - sg.ci0[1], X[2] - sg.ci0[2]};
     double t0 = (a[0] * sg.tci0[0] + a[1] * sg.tci0[1]) + a[2] * sg.tci0[2];
     double z = sg.s0 + sg.s1 * t0;
-    // spline point and tangent together (Horner with derivative, FP64 FMA), point wrapped to
-    // [-pi, pi) as in coo_curve_spline_5th
+    // spline point and tangent together (Horner with derivative, FP64 FMA; the derivative's first
+    // step is the leading coefficient), point wrapped to [-pi, pi) as in coo_curve_spline_5th
     double cz3[3], dc[3], xcd[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const double *a = coe + 6 * c;
-        double pv = a[5], dv = 0.0;
+        double dv = a[5], pv = __builtin_fma(a[5], z, a[4]);
 #pragma unroll
-        for (int k = 4; k >= 0; k--) {
+        for (int k = 3; k >= 0; k--) {
             dv = __builtin_fma(dv, z, pv);
             pv = __builtin_fma(pv, z, a[k]);
         }
@@ -241,7 +241,7 @@ __device__ void hit_math_f32(const HitConst &hc, const SubSeg &sg, int32_t ip, i
 #pragma unroll
     for (int c = 0; c < 3; c++) if (cz3[c] < -kPi) cz3[c] = cz3[c] + kPi2;
     cminus(X, cz3, xcd);
-    float xc[3], dcf[3];
+    float xc[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         xc[c] = (float)xcd[c];
@@ -258,9 +258,7 @@ __device__ void hit_math_f32(const HitConst &hc, const SubSeg &sg, int32_t ip, i
         float R = (float)sg.R;
         float norm = 0.15915494309189535f * is2;
         float tp = (__expf(-0.5f * rho2 * is2) - __expf(-0.5f * R * R * is2)) * norm;
-        float tg = tp * Gamma * indc;
-#pragma unroll
-        for (int c = 0; c < 3; c++) v[c] = (double)(tg * dcf[c]);
+        return tp * Gamma * indc;
     }
 }
 
@@ -281,14 +279,6 @@ __device__ __forceinline__ bool pair_hits_fast(const SubSeg &sg, int32_t ip, int
     if (v >= 0.0 && v < R2lo) return true;
     if (!(v < R2hi)) return false;
     return sqrt(v) < sg.R;
-}
-
-// 0-based wrapped index of an extended index in [1 - N, 2N]
-__device__ __forceinline__ int32_t wrap0(int32_t ip, int32_t N) {
-    int32_t x = ip - 1;
-    x += x < 0 ? N : 0;
-    x -= x >= N ? N : 0;
-    return x;
 }
 
 // Element i of a 3-vector by selects (a runtime index into a private array would live in scratch).
@@ -333,7 +323,8 @@ struct ColWalk {
     float P0, B0, C0, P1, B1, C1;
     float m0, m1;
     int16_t lo_a, lo_b, lo_c, a0, a1;   // a0..a1: the pair's a-range in the brick image
-    int8_t ax, up0, up1, use1;
+    int16_t tag;                        // the candidates' tag (HitQ)
+    int8_t up0, up1, use1;
 };
 
 // Margin of a single-precision plane position P - B db - C dc over db, dc in [0, W].
@@ -350,9 +341,10 @@ struct RowSeg {
 
 // A candidate cell in the axes of its pair's column walk: a along the column, b and c across it
 // (extended indices, stored + 32768 so that a column step adds to them without borrowing from
-// the neighbouring field), and tag = owning lane | column axis << 6: w0 = (a + 32768) | tag << 16,
-// w1 = (b + 32768) | (c + 32768) << 16. The x, y, z indices are formed when the candidate is
-// tested, once per candidate instead of once per queue round.
+// the neighbouring field), and tag = owning lane | column axis << 6 | the pair's brick image on
+// x, y, z (0..2 each) << 8, 10, 12: w0 = (a + 32768) | tag << 16, w1 = (b + 32768) | (c + 32768)
+// << 16. The x, y, z indices are formed when the candidate is tested, once per candidate instead
+// of once per queue round, and the cell within the brick from the image without a wrap.
 struct HitQ {
     uint32_t w0, w1;
 };
@@ -568,10 +560,11 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
     const HitQ hq = W.candq[W.candn - cnt + (lane < cnt ? lane : 0)];
     wave_sync_lds();
     struct {
-        int32_t ip, jp, kp, owner;
+        int32_t ip, jp, kp, owner, tag;
     } h;
     {
-        const int32_t tag = (int32_t)(hq.w0 >> 16), ax = tag >> 6;
+        const int32_t tag = (int32_t)(hq.w0 >> 16), ax = (tag >> 6) & 3;
+        h.tag = tag;
         const int32_t a = (int32_t)(hq.w0 & 0xffffu) - kIdxBias, b = (int32_t)(hq.w1 & 0xffffu) - kIdxBias,
                       c = (int32_t)(hq.w1 >> 16) - kIdxBias;
         h.owner = tag & 63;
@@ -589,22 +582,43 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
         const double R2 = sg.R * sg.R;
         hit = pair_hits_fast(sg, h.ip, h.jp, h.kp, R2 * (1.0 - 1e-14), R2 * (1.0 + 1e-14));
         if (hit) {
-            double v[3];
-            if (FAST) hit_math_f32(hc, sg, h.ip, h.jp, h.kp, v);
-            else hit_math_f64(W.A->g, sg, h.ip, h.jp, h.kp, v);
+            // the cell within the brick: extended index - (origin + 1) - (image - 1) N
             const int N = W.ri->N;
-            const int x = wrap0(h.ip, N) - W.ox, y = wrap0(h.jp, N) - W.oy, z = wrap0(h.kp, N) - W.oz;
+            const int x = h.ip - (W.ox + 1 - N) - (int)__umul24(__builtin_amdgcn_ubfe((uint32_t)h.tag, 8, 2), N);
+            const int y = h.jp - (W.oy + 1 - N) - (int)__umul24(__builtin_amdgcn_ubfe((uint32_t)h.tag, 10, 2), N);
+            const int z = h.kp - (W.oz + 1 - N) - (int)__umul24(__builtin_amdgcn_ubfe((uint32_t)h.tag, 12, 2), N);
             const int cell = acc_slot(x, y, z, W.ri->lbd[0], W.ri->lbd[1]);
-            if (F64) {
-#pragma unroll
-                for (int c = 0; c < 3; c++) unsafeAtomicAdd((double *)W.acc + c * kCells + cell, v[c]);
-            } else {
+            if (FAST) {
+                // fixed point: (tg 2^k) dcf is 2^k (tg dcf) in single precision (a power-of-two
+                // scale), rounded to an integer; the 64-bit integer is assembled from its halves
+                float dcf[3];
+                const float tgs = hit_math_f32(hc, sg, h.ip, h.jp, h.kp, dcf) * (float)W.ri->fx_scale;
+                const float lim = (float)W.A->fx_limit;
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    double s = v[c] * W.ri->fx_scale;
-                    if (!(fabs(s) < W.A->fx_limit)) { ovf = true; s = 0.0; }
-                    long long fx = (long long)rint(s);
-                    atomicAdd((unsigned long long *)W.acc + c * kCells + cell, (unsigned long long)fx);
+                    float s = tgs * dcf[c];
+                    if (!(fabsf(s) < lim)) { ovf = true; s = 0.0f; }
+                    const double r = (double)__builtin_rintf(s);
+                    const double hi = __builtin_floor(r * 0x1p-32);
+                    const double lo = __builtin_fma(hi, -0x1p32, r);
+                    const unsigned long long fx =
+                        ((unsigned long long)(uint32_t)(int32_t)hi << 32) | (unsigned long long)(uint32_t)lo;
+                    atomicAdd((unsigned long long *)W.acc + c * kCells + cell, fx);
+                }
+            } else {
+                double v[3];
+                hit_math_f64(W.A->g, sg, h.ip, h.jp, h.kp, v);
+                if (F64) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) unsafeAtomicAdd((double *)W.acc + c * kCells + cell, v[c]);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        double s = v[c] * W.ri->fx_scale;
+                        if (!(fabs(s) < W.A->fx_limit)) { ovf = true; s = 0.0; }
+                        long long fx = (long long)rint(s);
+                        atomicAdd((unsigned long long *)W.acc + c * kCells + cell, (unsigned long long)fx);
+                    }
                 }
             }
         }
@@ -628,7 +642,7 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
 template <bool F64, bool FAST>
 __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, int cnt) {
     const int lane = W.lane;
-    int32_t len = 0, ib = 0, ic = 0, own = 0, ax = 0;
+    int32_t len = 0, ib = 0, ic = 0, own = 0, wtag = 0;
     ColLines L{};   // idle lanes: zero lines, their counts are masked
     if (lane < cnt) {
         const RowSeg r = W.segs[W.segn - cnt + lane];
@@ -637,11 +651,11 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
         ic = r.c;
         own = r.pair;
         const ColWalk w = W.walks[own];
-        ax = w.ax;
+        wtag = w.tag;
         L = col_lines(w, ic);
     }
     // the candidate's first word is its a-index plus wb (no carry: a + bias < 2^16)
-    const uint32_t wb = ((uint32_t)(own | (ax << 6)) << 16) + (uint32_t)kIdxBias;
+    const uint32_t wb = ((uint32_t)wtag << 16) + (uint32_t)kIdxBias;
     W.segn = __builtin_amdgcn_readfirstlane(W.segn - cnt);
     const int32_t cap = W.ri->colcap;
     const int32_t bl0 = ib, steps = len < cap ? len : cap;
@@ -825,7 +839,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                     }
                     const int ax = dominant_axis(sg.tci0);
                     const int bx = (ax + 1) % 3, cx = (ax + 2) % 3;
-                    w.ax = (int8_t)ax;
+                    w.tag = (int16_t)(lane | (ax << 6) | (img[0] << 8) | (img[1] << 10) | (img[2] << 12));
                     w.lo_a = (int16_t)pick3(sg.lo, ax);
                     w.lo_b = (int16_t)pick3(sg.lo, bx);
                     w.lo_c = (int16_t)pick3(sg.lo, cx);
