@@ -198,24 +198,37 @@ __device__ void hit_math_f64(const Geo &g, const SubSeg &sg, int32_t ip, int32_t
 // 1/(1 - kappa rho cos), which amplifies rounding near its zero, so the whole reference formula
 // runs in FP64 (hit_math_f64).
 
-// Per-hit values that live in global memory (the section's spline and the tube's constants):
-// loaded before the candidate test so that their latency overlaps it.
+// Per-hit values that live in global memory (the section's spline): loaded before the candidate
+// test so that their latency overlaps it.
 struct HitConst {
     double coe[18];
-    double smin;
-    float Gamma;
 };
 
 __device__ __forceinline__ void hit_const_load(const Geo &g, const SubSeg &sg, HitConst &hc) {
     const double *coe = g.coe + 18 * (int64_t)sg.sec;
 #pragma unroll
     for (int k = 0; k < 18; k++) hc.coe[k] = coe[k];
-    hc.smin = g.sigma[sg.tube];
-    hc.Gamma = (float)g.gamma[sg.tube];
+}
+
+// The Gaussian core of a pair's tube in single precision (constant core radius): the contribution
+// is tp Gamma |c'|^-1 c' with tp Gamma = exp2(e rho^2) g - gc, e = -log2(e) / (2 sigma^2),
+// g = Gamma / (2 pi sigma^2) and gc = g exp(-R^2 / (2 sigma^2)) (the cut-off term).
+struct GaussK {
+    float e, g, gc;
+};
+
+__device__ __forceinline__ GaussK gauss_consts(double sigma, double gamma, double R) {
+    GaussK k;
+    const float s = (float)sigma, is2 = __builtin_amdgcn_rcpf(s * s), Rf = (float)R;
+    k.e = -0.5f * 1.4426950408889634f * is2;
+    k.g = 0.15915494309189535f * is2 * (float)gamma;
+    k.gc = __expf(-0.5f * Rf * Rf * is2) * k.g;
+    return k;
 }
 
 // sg.s0, sg.s1 hold zA, zB (SegRec). Returns tg and the tangent dcf; the contribution is tg dcf.
-__device__ float hit_math_f32(const HitConst &hc, const SubSeg &sg, int32_t ip, int32_t jp, int32_t kp, float *dcf) {
+__device__ float hit_math_f32(const HitConst &hc, const GaussK &gk, const SubSeg &sg, int32_t ip, int32_t jp, int32_t kp,
+                              float *dcf) {
     const double *coe = hc.coe;
     double X[3] = {mesh(ip, sg), mesh(jp, sg), mesh(kp, sg)};
     double a[3] = {X[0] - sg.ci0[0], X[1] - sg.ci0[1], X[2] - sg.ci0[2]};
@@ -247,18 +260,12 @@ __device__ float hit_math_f32(const HitConst &hc, const SubSeg &sg, int32_t ip, 
         xc[c] = (float)xcd[c];
         dcf[c] = (float)dc[c];
     }
-    const float Gamma = hc.Gamma;
-    const double smin = hc.smin;
     {
 #pragma clang fp contract(fast)
         float rho2 = xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2];
         float indc = __builtin_amdgcn_rsqf(dcf[0] * dcf[0] + dcf[1] * dcf[1] + dcf[2] * dcf[2]);
-        float sigma = (float)smin;
-        float is2 = __builtin_amdgcn_rcpf(sigma * sigma);
-        float R = (float)sg.R;
-        float norm = 0.15915494309189535f * is2;
-        float tp = (__expf(-0.5f * rho2 * is2) - __expf(-0.5f * R * R * is2)) * norm;
-        return tp * Gamma * indc;
+        float tpg = __builtin_fmaf(__builtin_amdgcn_exp2f(rho2 * gk.e), gk.g, -gk.gc);
+        return tpg * indc;
     }
 }
 
@@ -308,6 +315,7 @@ struct SegRec {
     double ci0[3], ci1[3], tci0[3], tci1[3];
     double s0, s1, ndci0, R;
     int32_t sec, tube;
+    GaussK gk;   // FP32 mode
 };
 
 // Column walk of one (sub-segment, brick image) pair, kept in LDS for the column stage. Columns
@@ -505,7 +513,7 @@ __device__ __forceinline__ void wave_sync_lds() {
 
 // The sub-segment of lane `own` (every lane of the wave must call this).
 template <bool FAST>
-__device__ __forceinline__ void fetch_owner(const SegRec &me, int own, const Run &ri, SubSeg &sg) {
+__device__ __forceinline__ void fetch_owner(const SegRec &me, int own, const Run &ri, SubSeg &sg, GaussK &gk) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         sg.ci0[c] = __shfl(me.ci0[c], own, 64);
@@ -518,7 +526,13 @@ __device__ __forceinline__ void fetch_owner(const SegRec &me, int own, const Run
     if (!FAST) sg.ndci0 = __shfl(me.ndci0, own, 64);
     sg.R = __shfl(me.R, own, 64);
     sg.sec = __shfl(me.sec, own, 64);
-    sg.tube = __shfl(me.tube, own, 64);
+    if (FAST) {
+        gk.e = __shfl(me.gk.e, own, 64);
+        gk.g = __shfl(me.gk.g, own, 64);
+        gk.gc = __shfl(me.gk.gc, own, 64);
+    } else {
+        sg.tube = __shfl(me.tube, own, 64);
+    }
     sg.N = ri.N;
     sg.dx = ri.dx;
     sg.start = ri.start;
@@ -574,7 +588,8 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
     }
     W.candn = __builtin_amdgcn_readfirstlane(W.candn - cnt);
     SubSeg sg;
-    fetch_owner<FAST>(me, lane < cnt ? h.owner : lane, *W.ri, sg);
+    GaussK gk;
+    fetch_owner<FAST>(me, lane < cnt ? h.owner : lane, *W.ri, sg, gk);
     bool hit = false, ovf = false;
     if (lane < cnt) {
         HitConst hc;
@@ -592,7 +607,7 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
                 // fixed point: (tg 2^k) dcf is 2^k (tg dcf) in single precision (a power-of-two
                 // scale), rounded to an integer; the 64-bit integer is assembled from its halves
                 float dcf[3];
-                const float tgs = hit_math_f32(hc, sg, h.ip, h.jp, h.kp, dcf) * (float)W.ri->fx_scale;
+                const float tgs = hit_math_f32(hc, gk, sg, h.ip, h.jp, h.kp, dcf) * (float)W.ri->fx_scale;
                 const float lim = (float)W.A->fx_limit;
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
@@ -823,6 +838,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                         const double LD = g.Len_Disc[sg.tube], ind = 1.0 / sg.ndci0;
                         me.s0 = (sg.s0 * L * ind - g.czl[sg.sec]) * LD;
                         me.s1 = (sg.s1 - sg.s0) * LD * ind;
+                        me.gk = gauss_consts(g.sigma[sg.tube], g.gamma[sg.tube], sg.R);
                     }
                     me.R = sg.R;
                     me.sec = sg.sec;
@@ -892,6 +908,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
 #pragma unroll
                     for (int c = 0; c < 3; c++) me.ci0[c] = me.ci1[c] = me.tci0[c] = me.tci1[c] = 0.0;
                     me.s0 = me.s1 = me.ndci0 = me.R = 0.0;
+                    me.gk = GaussK{0.0f, 0.0f, 0.0f};
                     me.sec = me.tube = 0;
                 }
             }
