@@ -472,7 +472,7 @@ struct ColLines {
     float qL0, BL0, qL1, BL1;   // lower bounds, minus one
     float qU0, BU0, qU1, BU1;   // upper bounds
     float A0, A1;               // the pair's a-range, relative to lo_a (A0 minus one)
-    int32_t lo_a1, lo_b;        // lo_a + 1
+    int32_t lo_a1, lo_b;        // lo_a + 1 (a-index of the count's origin), lo_b
 };
 
 __device__ __forceinline__ ColLines col_lines(const ColWalk &w, int32_t ic) {
@@ -495,13 +495,12 @@ __device__ __forceinline__ ColLines col_lines(const ColWalk &w, int32_t ic) {
     return L;
 }
 
-// Candidate a-range of column ib of a row segment: returns the count (at most one more than the
-// cells between the bounds, never fewer), the first index in *ia0.
-__device__ __forceinline__ int32_t column_range(const ColLines &L, int32_t ib, int32_t *ia0) {
-    const float db = (float)(ib - L.lo_b);
+// Candidate a-range of the column at db = ib - lo_b of a row segment: returns the count (at most
+// one more than the cells between the bounds, never fewer), the first index minus lo_a1 in *ia0.
+__device__ __forceinline__ int32_t column_range(const ColLines &L, float db, int32_t *ia0) {
     const float flo = ceilf(fmaxf(fmaxf(L.A0, __builtin_fmaf(-L.BL0, db, L.qL0)), __builtin_fmaf(-L.BL1, db, L.qL1)));
     const float fhi = floorf(fminf(fminf(L.A1, __builtin_fmaf(-L.BU0, db, L.qU0)), __builtin_fmaf(-L.BU1, db, L.qU1)));
-    *ia0 = L.lo_a1 + (int32_t)flo;
+    *ia0 = (int32_t)flo;
     return (int32_t)fmaxf(fhi - flo, 0.0f);
 }
 
@@ -669,8 +668,9 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
         wtag = w.tag;
         L = col_lines(w, ic);
     }
-    // the candidate's first word is its a-index plus wb (no carry: a + bias < 2^16)
-    const uint32_t wb = ((uint32_t)wtag << 16) + (uint32_t)kIdxBias;
+    // the candidate's first word is column_range's index plus wb (a = index + lo_a1; no carry:
+    // a + bias < 2^16)
+    const uint32_t wb = ((uint32_t)wtag << 16) + (uint32_t)(kIdxBias + L.lo_a1);
     W.segn = __builtin_amdgcn_readfirstlane(W.segn - cnt);
     const int32_t cap = W.ri->colcap;
     const int32_t bl0 = ib, steps = len < cap ? len : cap;
@@ -678,11 +678,12 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
     uint32_t w1 = (uint32_t)(ib + kIdxBias) | ((uint32_t)(ic + kIdxBias) << 16);
     // two columns per lane and step (about a third of the columns hold a candidate, so the
     // queue rounds of both are shared)
-    for (int32_t col = 0; __ballot(col < steps) != 0ull; col += 2, ib += 2, w1 += 2) {
+    float db = (float)(ib - L.lo_b);   // exact: |db| < 2^24
+    for (int32_t rem = steps; __ballot(rem > 0) != 0ull; rem -= 2, db += 2.0f, w1 += 2) {
         int32_t ia0, ia1;
-        int32_t nc0 = column_range(L, ib, &ia0), nc1 = column_range(L, ib + 1, &ia1);
-        nc0 = col < steps ? nc0 : 0;
-        nc1 = col + 1 < steps ? nc1 : 0;
+        int32_t nc0 = column_range(L, db, &ia0), nc1 = column_range(L, db + 1.0f, &ia1);
+        nc0 = rem > 0 ? nc0 : 0;
+        nc1 = rem > 1 ? nc1 : 0;
         const int32_t nc = nc0 + nc1, ia1s = ia1 - nc0;
         unsigned long long m;
         for (int u = 0; (m = __ballot(u < nc)) != 0ull; u++) {
