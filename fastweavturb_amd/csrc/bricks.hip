@@ -296,8 +296,9 @@ __global__ __launch_bounds__(256) void k_pairs_count(Geo g, Run r0, uint32_t *__
 // Fill pass: writes every pair (sub-point index, image code kx + 3ky + 9kz) into its brick's list,
 // after the brick's wide[key] pairs of wide boxes (k_pairs_big), which lead it: the deposition
 // spreads those expensive pairs over its waves before it takes the rest in runs. The boxes and
-// (up to kRecMaskBits entries) the slab tests come from the count pass's records; a box of more
-// entries repeats its slab tests.
+// (up to kRecMaskBits entries) the slab tests come from the count pass's records: such a lane
+// steps through the set bits of its mask only, forming each entry's brick and image from its
+// index; a box of more entries walks all of them and repeats its slab tests.
 __global__ __launch_bounds__(256) void k_pairs_fill(Geo g, Run r0, uint32_t *__restrict__ cnt,
                                                     const BinRec *__restrict__ rec, const int64_t *__restrict__ off,
                                                     const uint32_t *__restrict__ wide, uint32_t *__restrict__ pairs,
@@ -329,18 +330,31 @@ __global__ __launch_bounds__(256) void k_pairs_fill(Geo g, Run r0, uint32_t *__r
     const bool slow = n > kRecMaskBits;
     SubSeg sg;
     if (slow) subseg_load(g, r0, q, sg);   // a paired sub-segment: it loads
-    const int32_t nmax = wave_max(n);
+    uint32_t m = (!slow && n > 0) ? rc.mask : 0u;
+    const int32_t nmax = wave_max(slow ? n : __popc(m));
     EntryWalk<true> ews;     // slow lanes: slab tests again
-    EntryWalk<false> ewf;    // the rest: the recorded mask
     if (slow) ews.start(ab, sg, bd, brick0, nbx, nby);
-    else if (n > 0) ewf.start(ab, sg, bd, brick0, nbx, nby);
+    // entry e = jx + nx (jy + ny jz): quotients by nx, ny from single-precision reciprocals
+    // ((e + 1/2) / nx is at least 1/64 from an integer for e < 32)
+    const float inx = 1.0f / (float)nx, iny = 1.0f / (float)ny;
     for (int32_t t = 0; t < nmax; t++) {
-        bool act = t < n;
+        bool act = false;
         int64_t key = 0;
         int32_t code = 0;
-        if (act) {
-            if (slow) act = ews.take(ab, sg, bd, brick0, nbx, nby, nx, ny, t, n, key, code);
-            else act = ewf.take(ab, sg, bd, brick0, nbx, nby, nx, ny, t, n, key, code) && ((rc.mask >> t) & 1u);
+        if (slow) {
+            act = t < n && ews.take(ab, sg, bd, brick0, nbx, nby, nx, ny, t, n, key, code);
+        } else if (m != 0u) {
+            const int32_t e = __builtin_ctz(m);
+            m &= m - 1u;
+            const int32_t q1 = (int32_t)__builtin_floorf(((float)e + 0.5f) * inx);
+            const int32_t q2 = (int32_t)__builtin_floorf(((float)q1 + 0.5f) * iny);
+            int32_t bx, by, bz, ix, iy, iz;
+            ab[0].at(e - q1 * nx, bx, ix);
+            ab[1].at(q1 - q2 * ny, by, iy);
+            ab[2].at(q2, bz, iz);
+            key = brick0 + ((int64_t)bz * nby + by) * nbx + bx;
+            code = ix + 3 * iy + 9 * iz;
+            act = true;
         }
         const uint32_t slot = wave_slot_add(cnt, key, act);
         if (act) {
