@@ -138,23 +138,37 @@ __device__ __forceinline__ SecEnds section_ends(const Geo &g, int l, int64_t s) 
 }
 
 // curve_spline_5th (subroutine.f90:463-488) with abscissae cze_pl*Len_Disc (tube.f90:598)
-__global__ void k_spline(Geo g) {
-    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s >= g.nsec_all) return;
-    int l = g.sec_tube[s];
-    SecEnds e = section_ends(g, l, s);
-    double LD = g.Len_Disc[l];
-    double zr = e.cze1 * LD - e.cze0 * LD;
-    const double *X[3] = {g.cx, g.cy, g.cz};
-    double coe[18];
+// One section per thread; the block's coefficients (18 per section, contiguous in memory) are
+// staged in LDS (rows padded to 19 doubles) and written out with consecutive threads on
+// consecutive doubles.
+constexpr int kSplineBlock = 256;
+__global__ __launch_bounds__(kSplineBlock) void k_spline(Geo g) {
+    __shared__ double st[kSplineBlock * 19];
+    const int64_t s0 = blockIdx.x * (int64_t)kSplineBlock;
+    const int64_t s = s0 + threadIdx.x;
+    if (s < g.nsec_all) {
+        int l = g.sec_tube[s];
+        SecEnds e = section_ends(g, l, s);
+        double LD = g.Len_Disc[l];
+        double zr = e.cze1 * LD - e.cze0 * LD;
+        const double *X[3] = {g.cx, g.cy, g.cz};
+        double coe[18];
 #pragma unroll
-    for (int c = 0; c < 3; c++)
-        spline5_coe(coe + 6 * c, zr, X[c][e.p0], X[c][e.p1], g.ck[3 * e.p0 + c], g.ck[3 * e.p1 + c],
-                    g.ckk[3 * e.p0 + c], g.ckk[3 * e.p1 + c]);
+        for (int c = 0; c < 3; c++)
+            spline5_coe(coe + 6 * c, zr, X[c][e.p0], X[c][e.p1], g.ck[3 * e.p0 + c], g.ck[3 * e.p1 + c],
+                        g.ckk[3 * e.p0 + c], g.ckk[3 * e.p1 + c]);
 #pragma unroll
-    for (int k = 0; k < 18; k++) g.coe[18 * s + k] = coe[k];
-    g.czl[s] = e.cze0;
-    g.czr[s] = e.cze1;
+        for (int k = 0; k < 18; k++) st[19 * threadIdx.x + k] = coe[k];
+        g.czl[s] = e.cze0;
+        g.czr[s] = e.cze1;
+    }
+    __syncthreads();
+    const int64_t ns = g.nsec_all - s0 < kSplineBlock ? g.nsec_all - s0 : kSplineBlock;
+    double *out = g.coe + 18 * s0;
+    for (int i = threadIdx.x; i < 18 * ns; i += kSplineBlock) {
+        const int t = i / 18;
+        out[i] = st[19 * t + (i - 18 * t)];
+    }
 }
 
 // Integrand of the turning angle: |c' x c''| / |c'|^2 (tube.f90:629-639).
@@ -378,24 +392,8 @@ __global__ void k_theta_listed(Geo g, Run r, const int32_t *__restrict__ redo,
         if (!theta_exact(g, r, redo[i])) atomicOr(err, 1);
 }
 
-__global__ void k_fill_sub_sec(Geo g) {
-    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s >= g.nsec_all) return;
-    const int32_t l = g.sec_tube[s];
-    for (int64_t q = g.sp_base[s]; q < g.sp_base[s + 1]; q++) {
-        g.sub_sec[q] = (int32_t)s;
-        g.sub_tube[q] = l;
-    }
-}
-
-// tube.f90:706-713 sub-points; tube.f90:719-723 seam: the last sub-point of a real section is
-// the first sub-point of the next section (evaluated with that section's spline at 0).
-__global__ void k_subpoints(Geo g) {
-    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q >= g.nsub_all) return;
-    int32_t s = g.sub_sec[q];
-    int l = g.sec_tube[s];
-    int32_t j = (int32_t)(q - g.sp_base[s]);
+// The sub-point q = sp_base[s] + j of section s of tube l.
+__device__ __forceinline__ void subpoint_at(const Geo &g, int64_t q, int32_t s, int l, int32_t j) {
     bool real = (int64_t)s < g.sec_base[l + 1] - 1;
     int32_t src = s;
     int32_t jj = j;
@@ -410,6 +408,41 @@ __global__ void k_subpoints(Geo g) {
     spline_pos((sv - czl) * LD, coe, c3);
 #pragma unroll
     for (int c = 0; c < 3; c++) g.ci[3 * q + c] = c3[c];
+}
+
+// tube.f90:706-713 sub-points; tube.f90:719-723 seam: the last sub-point of a real section is
+// the first sub-point of the next section (evaluated with that section's spline at 0).
+// Each wave takes 64 consecutive sections (lane i: section s0 + i) and then their sub-points, 64
+// at a time with consecutive lanes on consecutive sub-points: a sub-point's section is the last
+// lane whose first sub-point is not after it (a binary search over the lanes), so sub_sec,
+// sub_tube and ci are written in order.
+constexpr int kSubBlock = 256;
+__global__ __launch_bounds__(kSubBlock) void k_subpoints(Geo g) {
+    const int lane = threadIdx.x & 63;
+    const int64_t s0 = (blockIdx.x * (int64_t)(kSubBlock / 64) + (threadIdx.x >> 6)) * 64;
+    if (s0 >= g.nsec_all) return;   // whole wave
+    const int64_t qa = g.sp_base[s0];
+    const int64_t se = s0 + 64 < g.nsec_all ? s0 + 64 : g.nsec_all;
+    const int64_t nq = g.sp_base[se] - qa;
+    const int32_t rel = s0 + lane < se ? (int32_t)(g.sp_base[s0 + lane] - qa) : INT32_MAX;
+    for (int64_t k0 = 0; k0 < nq; k0 += 64) {
+        const int32_t qr = (int32_t)(k0 + lane);
+        int32_t pos = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1) {
+            const int32_t v = __shfl(rel, pos + step, 64);
+            if (pos + step < 64 && v <= qr) pos += step;
+        }
+        const int32_t first = __shfl(rel, pos, 64);   // every lane takes part in the shuffles
+        if (qr >= nq) continue;
+        const int64_t q = qa + qr;
+        const int32_t s = (int32_t)(s0 + pos);
+        const int32_t j = qr - first;
+        const int l = g.sec_tube[s];
+        g.sub_sec[q] = s;
+        g.sub_tube[q] = l;
+        subpoint_at(g, q, s, l, j);
+    }
 }
 
 // Per sub-point: number of grid columns of its sub-segment's box (0 if it starts none).
@@ -463,7 +496,7 @@ static hipError_t segment_kernels(wt_ctx *c, bool replay) {
     hipLaunchKernelGGL(k_derivative<false>, dim3(grid_for(g.npoint_all, bs)), dim3(bs), 0, st, g);
     hipLaunchKernelGGL(k_derivative<true>, dim3(grid_for(g.npoint_all, bs)), dim3(bs), 0, st, g);
     // per section
-    hipLaunchKernelGGL(k_spline, dim3(grid_for(g.nsec_all, bs)), dim3(bs), 0, st, g);
+    hipLaunchKernelGGL(k_spline, dim3(grid_for(g.nsec_all, kSplineBlock)), dim3(kSplineBlock), 0, st, g);
     WT_HIP(fill_zero(w.err, sizeof(int32_t), st));
     kt_begin(c, kSlotTheta);
     if (c->theta_exact_all) {
@@ -478,9 +511,8 @@ static hipError_t segment_kernels(wt_ctx *c, bool replay) {
     // sub-point layout
     WT_HIP(exclusive_scan_i32_to_i64(g.ntf, 1, g.sp_base, g.nsec_all, st, w.scan));
     if (!replay) return hipGetLastError();
-    hipLaunchKernelGGL(k_fill_sub_sec, dim3(grid_for(g.nsec_all, bs)), dim3(bs), 0, st, g);
     kt_begin(c, kSlotSubpoints);
-    hipLaunchKernelGGL(k_subpoints, dim3(grid_for(g.nsub_all, bs)), dim3(bs), 0, st, g);
+    hipLaunchKernelGGL(k_subpoints, dim3(grid_for(g.nsec_all, kSubBlock)), dim3(kSubBlock), 0, st, g);
     kt_end(c, kSlotSubpoints);
     return hipGetLastError();
 }
@@ -491,7 +523,6 @@ hipError_t launch_segments(wt_ctx *c) {
     hipStream_t st = c->stream;
     dfree(c->seg_allocs);
     w = SegWs{};
-    const int bs = 256;
     WT_HIP(dalloc(c, c->seg_allocs, &w.tot, g.nline));
     WT_HIP(dalloc(c, c->seg_allocs, &w.err, 1));
     WT_HIP(dalloc(c, c->seg_allocs, &w.redo, g.nsec_all));
@@ -512,9 +543,8 @@ hipError_t launch_segments(wt_ctx *c) {
     WT_HIP(dalloc(c, c->seg_allocs, &g.sub_sec, nsub));
     WT_HIP(dalloc(c, c->seg_allocs, &g.sub_tube, nsub));
     WT_HIP(dalloc(c, c->seg_allocs, &g.ci, 3 * nsub));
-    hipLaunchKernelGGL(k_fill_sub_sec, dim3(grid_for(g.nsec_all, bs)), dim3(bs), 0, st, g);
     kt_begin(c, kSlotSubpoints);
-    hipLaunchKernelGGL(k_subpoints, dim3(grid_for(nsub, bs)), dim3(bs), 0, st, g);
+    hipLaunchKernelGGL(k_subpoints, dim3(grid_for(g.nsec_all, kSubBlock)), dim3(kSubBlock), 0, st, g);
     kt_end(c, kSlotSubpoints);
     return hipGetLastError();
 }
