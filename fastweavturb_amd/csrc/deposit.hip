@@ -249,11 +249,15 @@ __device__ float hit_math_f32(const HitConst &hc, const GaussK &gk, const SubSeg
         cz3[c] = pv;
         dc[c] = dv;
     }
+    // the wrap of the spline point and cminus's correction, each as one fma with an integer
+    // multiple of 2 pi (fma(-1, 2 pi, c) rounds like c - 2 pi, and a zero multiple is exact)
 #pragma unroll
-    for (int c = 0; c < 3; c++) if (cz3[c] >= kPi) cz3[c] = cz3[c] - kPi2;
-#pragma unroll
-    for (int c = 0; c < 3; c++) if (cz3[c] < -kPi) cz3[c] = cz3[c] + kPi2;
-    cminus(X, cz3, xcd);
+    for (int c = 0; c < 3; c++) {
+        const int k1 = cz3[c] >= kPi ? 1 : (cz3[c] < -kPi ? -1 : 0);
+        const double d = X[c] - __builtin_fma(-(double)k1, kPi2, cz3[c]);
+        const int k2 = d > kPi ? 1 : (d < -kPi ? -1 : 0);
+        xcd[c] = __builtin_fma(-(double)k2, kPi2, d);
+    }
     float xc[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
