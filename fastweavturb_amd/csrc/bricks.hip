@@ -239,7 +239,7 @@ __global__ __launch_bounds__(256) void k_pairs_count(Geo g, Run r0, uint32_t *__
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const bool in = q < g.nsub_all;
     SubSeg sg;
-    bool ok = in && subseg_load(g, r0, q, sg);
+    bool ok = in && subseg_load_fast(g, q, sg);
     AxisImages ab[3];
     int32_t n = 0, nx = 1, ny = 1;
     int64_t brick0 = 0;
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(256) void k_pairs_fill(Geo g, Run r0, uint32_t *__r
     }
     const bool slow = n > kRecMaskBits;
     SubSeg sg;
-    if (slow) subseg_load(g, r0, q, sg);   // a paired sub-segment: it loads
+    if (slow) subseg_load_fast(g, q, sg);   // a paired sub-segment: it loads
     uint32_t m = (!slow && n > 0) ? rc.mask : 0u;
     const int32_t nmax = wave_max(slow ? n : __popc(m));
     EntryWalk<true> ews;     // slow lanes: slab tests again
@@ -380,7 +380,7 @@ __global__ __launch_bounds__(256) void k_pairs_big(Geo g, Run r0, uint32_t *__re
     for (uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < nb; w += gridDim.x * (blockDim.x >> 6)) {
         const int64_t q = big[w];
         SubSeg sg;
-        if (!subseg_load(g, r0, q, sg)) continue;   // listed sub-segments always load
+        if (!subseg_load_fast(g, q, sg)) continue;   // listed sub-segments always load
         const Run &r = g.runs[g.tube_inst[sg.tube]];
         AxisImages ab[3];
 #pragma unroll

@@ -155,6 +155,77 @@ __device__ __forceinline__ bool subseg_load(const Geo &g, const Run &, int64_t q
     return true;
 }
 
+// subseg_load with a shorter chain of dependent loads (same result): the sub-point's section and
+// tube come from sub_sec and sub_tube together, and the four sub-points q..q+3 that its chords can
+// use are read at once (a section's last sub-segment takes its second chord from the next
+// section's first two sub-points, q+2 and q+3 there; indices past the end are clamped, their
+// values unused).
+__device__ __forceinline__ bool subseg_load_fast(const Geo &g, int64_t q, SubSeg &o) {
+    const int32_t s = g.sub_sec[q];
+    const int32_t l = g.sub_tube[q];
+    double p[12];
+    const int64_t last = g.nsub_all - 1;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int64_t qk = q + k <= last ? q + k : last;
+#pragma unroll
+        for (int c = 0; c < 3; c++) p[3 * k + c] = g.ci[3 * qk + c];
+    }
+    const int32_t j = (int32_t)(q - g.sp_base[s]);
+    const int32_t nt = g.ntf[s];
+    if ((int64_t)s == g.sec_base[l + 1] - 1) return false;
+    if (j >= nt) return false;
+    const Run &r = g.runs[g.tube_inst[l]];
+    if (l - r.tube0 < r.line_s || l - r.tube0 > r.line_e) return false;
+    o.N = r.N;
+    o.dx = r.dx;
+    o.start = r.start;
+    o.field_off = r.field_off;
+    o.sec = s;
+    o.tube = l;
+    o.j = j;
+    auto chord_of = [](const double *b, const double *a, double *t, double *nd) {
+        double d[3];
+        cminus(b, a, d);
+        const double n = norm3(d);
+        *nd = n;
+#pragma unroll
+        for (int c = 0; c < 3; c++) t[c] = d[c] / n;
+    };
+    double nd1;
+    chord_of(p + 3, p, o.tci0, &o.ndci0);
+    if (j + 1 < nt) chord_of(p + 6, p + 3, o.tci1, &nd1);
+    else chord_of(p + 9, p + 6, o.tci1, &nd1);
+    double czl = g.czl[s], dtf = g.dtf[s];
+    o.s0 = czl + dtf * (double)j;
+    o.s1 = czl + dtf * (double)(j + 1);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        o.ci0[c] = p[c];
+        o.ci1[c] = p[3 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {  // mark1 (tube.f90:779-785)
+        if (o.ci1[c] - o.ci0[c] > kPi) o.ci0[c] = o.ci0[c] + kPi2;
+        else if (o.ci1[c] - o.ci0[c] < -kPi) o.ci1[c] = o.ci1[c] + kPi2;
+    }
+    double smin = g.sigma[l], smax = r.sigma_ratio * smin, len = g.length[l];
+    int nT = g.n_T[l];
+    double sig = sigma_func(o.s0, len, smax, smin, nT);
+    double R = r.Rtube_ratio * sig;
+    sig = sigma_func(o.s1, len, smax, smin, nT);
+    double R2 = r.Rtube_ratio * sig;
+    o.R = (R2 > R) ? R2 : R;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {  // tube.f90:793-798
+        double mn = o.ci0[c] < o.ci1[c] ? o.ci0[c] : o.ci1[c];
+        double mx = o.ci0[c] > o.ci1[c] ? o.ci0[c] : o.ci1[c];
+        o.lo[c] = (int32_t)floor((mn - r.start - o.R) / r.dx + 1.0);
+        o.hi[c] = (int32_t)ceil((mx - r.start + o.R) / r.dx + 1.0);
+    }
+    return true;
+}
+
 // Mesh coordinate of an extended 1-based index (tube.f90:331-339).
 __device__ __forceinline__ double mesh(int32_t ip, const SubSeg &sg) {
     return ((double)ip - 1.0) * sg.dx + sg.start;
