@@ -789,7 +789,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
         unsigned long long next = 0;
         if (lane == 0) next = atomicAdd(&s_misc[1], 1ull);
         next = __shfl(next, 0, 64);
-        uint32_t q_next = 0;
+        uint32_t q_next = 0, warm = 0;
         int32_t code_next = 0, s_next = 0, l_next = 0;
         BinRec rc_next{};
         int64_t pn = pair_of((int64_t)next);
@@ -823,6 +823,7 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
             {
                 SubSeg sg;
                 ColWalk w;
+                asm volatile("" ::"v"(warm));
                 if (have) subseg_load_pair(g, ri, (int64_t)q_cur, s_cur, l_cur, rc_cur, sg);
                 if (have) {
 #pragma unroll
@@ -922,6 +923,9 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                 s_next = g.sub_sec[q_next];
                 l_next = g.sub_tube[q_next];
                 rc_next = A.rec[q_next];
+                // one word of the next pair's sub-points, so that their cache lines are on their way
+                // while this chunk runs (the value is only waited for at the next set-up)
+                warm = ((const uint32_t *)(g.ci + 3 * (int64_t)q_next))[0];
             }
             // ---- stage 1 (lane = pair): one row of the pair per step, non-empty bands queued ----
             for (int32_t ic = c0; __ballot(have && ic <= c1); ic++) {
