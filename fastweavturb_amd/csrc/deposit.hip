@@ -346,10 +346,15 @@ __device__ __forceinline__ float plane_margin(double P, double B, double C, doub
 
 // A row of columns of one pair: columns b = bl .. bl + len - 1 of row c.
 struct RowSeg {
-    int16_t c, bl;
-    uint8_t len, pair;
-    uint16_t pad;
+    uint32_t w0, w1;   // c | bl << 16 (16-bit two's complement each), len | pair << 8
+    __device__ int32_t c() const { return (int32_t)(int16_t)(w0 & 0xffffu); }
+    __device__ int32_t bl() const { return (int32_t)w0 >> 16; }
+    __device__ int32_t len() const { return (int32_t)(w1 & 0xffu); }
+    __device__ int32_t pair() const { return (int32_t)((w1 >> 8) & 0xffu); }
 };
+__device__ __forceinline__ RowSeg row_seg(int32_t c, int32_t bl, int32_t len, int32_t pair) {
+    return RowSeg{((uint32_t)c & 0xffffu) | ((uint32_t)bl << 16), (uint32_t)len | ((uint32_t)pair << 8)};
+}
 
 // A candidate cell in the axes of its pair's column walk: a along the column, b and c across it
 // (extended indices, stored + 32768 so that a column step adds to them without borrowing from
@@ -375,11 +380,11 @@ __device__ __forceinline__ double rcp_nr(double x) {
 // half-width quadratic. The constants are formed in FP64 from the single-precision walk and
 // rounded once; a row then costs two plane lines, one square root and the clipping.
 struct RowBand {
-    float al0, be0, al1, be1;      // plane bounds: db >= / <= al + be dc (lower where lo0 / lo1)
+    float aL0, bL0, aL1, bL1;      // lower bounds on db: aL + bL dc (a plane's line, or b0r)
+    float aU0, bU0, aU1, bU1;      // upper bounds on db: aU + bU dc (a plane's line, or b1r)
     float cm, cp, g, k0, k1, uc;   // ellipse: db in [cm + g Dc - hw, cp + g Dc + hw], hw^2 = k0 - k1 Dc^2
     float b0r, b1r;                // the pair's b-range relative to lo_b
     int32_t lo_b, lo_c;
-    bool lo0, lo1;
 };
 
 // One plane's condition B db >= Q - C dc (s = +1) or B db <= Q - C dc (s = -1) as a line
@@ -422,19 +427,27 @@ __device__ __forceinline__ RowBand row_setup(const ColWalk &w, double ub, double
     rb.b1r = (float)(b1 - w.lo_b);
     const double dmax = fmax(fabs((double)(c0 - w.lo_c)), fabs((double)(c1 - w.lo_c)));
     const double A0 = w.a0 - w.lo_a, A1 = w.a1 - w.lo_a;
+    float al0, be0, al1 = -1e9f, be1 = 0.0f;
+    bool lo0, lo1 = true;
     {   // plane 0: up0 needs B0 db >= P0 - C0 dc - 2 m0 - A1, else B0 db <= P0 - C0 dc + 2 m0 - A0
         const double m2 = 2.0 * (double)w.m0;
         const double Q = w.up0 ? (double)w.P0 - m2 - A1 : (double)w.P0 + m2 - A0;
-        plane_line(Q, w.B0, w.C0, w.up0 ? 1.0 : -1.0, dmax, w.lo_c, rb.al0, rb.be0, rb.lo0, c0, c1);
+        plane_line(Q, w.B0, w.C0, w.up0 ? 1.0 : -1.0, dmax, w.lo_c, al0, be0, lo0, c0, c1);
     }
-    rb.lo1 = true;
-    rb.al1 = -1e9f;
-    rb.be1 = 0.0f;
     if (w.use1) {   // plane 1: up1 needs B1 db <= P1 - C1 dc + 2 m1 - A0, else B1 db >= P1 - C1 dc - 2 m1 - A1
         const double m2 = 2.0 * (double)w.m1;
         const double Q = w.up1 ? (double)w.P1 + m2 - A0 : (double)w.P1 - m2 - A1;
-        plane_line(Q, w.B1, w.C1, w.up1 ? -1.0 : 1.0, dmax, w.lo_c, rb.al1, rb.be1, rb.lo1, c0, c1);
+        plane_line(Q, w.B1, w.C1, w.up1 ? -1.0 : 1.0, dmax, w.lo_c, al1, be1, lo1, c0, c1);
     }
+    // each plane's line bounds db from one side; the other side's slot holds the b-range end
+    rb.aL0 = lo0 ? al0 : rb.b0r;
+    rb.bL0 = lo0 ? be0 : 0.0f;
+    rb.aU0 = lo0 ? rb.b1r : al0;
+    rb.bU0 = lo0 ? 0.0f : be0;
+    rb.aL1 = lo1 ? al1 : rb.b0r;
+    rb.bL1 = lo1 ? be1 : 0.0f;
+    rb.aU1 = lo1 ? rb.b1r : al1;
+    rb.bU1 = lo1 ? 0.0f : be1;
     // ellipse (1 + B0^2) Db^2 + 2 B0 C0 Db Dc + (1 + C0^2) Dc^2 < Rp2, Db = db - ub: for each Dc,
     // Db = -B0 C0 Dc / eb +- sqrt(Rp2 / eb - (1 + B0^2 + C0^2) Dc^2 / eb^2) with eb = 1 + B0^2
     const double B0 = w.B0, C0 = w.C0, eb = 1.0 + B0 * B0, ieb = rcp_nr(eb);
@@ -452,13 +465,12 @@ __device__ __forceinline__ RowBand row_setup(const ColWalk &w, double ub, double
 // the ellipse of radius Rp. A superset, clipped to b0..b1; false if empty.
 __device__ __forceinline__ bool row_band(const RowBand &rb, int32_t ic, int32_t &bl, int32_t &bh) {
     const float dcn = (float)(ic - rb.lo_c);
-    const float x0 = __builtin_fmaf(rb.be0, dcn, rb.al0), x1 = __builtin_fmaf(rb.be1, dcn, rb.al1);
     const float Dc = dcn - rb.uc;
     const float h2 = __builtin_fmaf(-rb.k1 * Dc, Dc, rb.k0);
     const float hw = __builtin_amdgcn_sqrtf(fmaxf(h2, 0.0f));
-    const float lo = fmaxf(fmaxf(rb.lo0 ? x0 : rb.b0r, rb.lo1 ? x1 : rb.b0r),
+    const float lo = fmaxf(fmaxf(__builtin_fmaf(rb.bL0, dcn, rb.aL0), __builtin_fmaf(rb.bL1, dcn, rb.aL1)),
                            fmaxf(__builtin_fmaf(rb.g, Dc, rb.cm) - hw, rb.b0r));
-    const float hi = fminf(fminf(rb.lo0 ? rb.b1r : x0, rb.lo1 ? rb.b1r : x1),
+    const float hi = fminf(fminf(__builtin_fmaf(rb.bU0, dcn, rb.aU0), __builtin_fmaf(rb.bU1, dcn, rb.aU1)),
                            fminf(__builtin_fmaf(rb.g, Dc, rb.cp) + hw, rb.b1r));
     if (!(h2 >= 0.0f) || !(lo <= hi)) return false;
     bl = rb.lo_b + (int32_t)ceilf(lo);
@@ -664,10 +676,10 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
     ColLines L{};   // idle lanes: zero lines, their counts are masked
     if (lane < cnt) {
         const RowSeg r = W.segs[W.segn - cnt + lane];
-        len = r.len;
-        ib = r.bl;
-        ic = r.c;
-        own = r.pair;
+        len = r.len();
+        ib = r.bl();
+        ic = r.c();
+        own = r.pair();
         const ColWalk w = W.walks[own];
         wtag = w.tag;
         L = col_lines(w, ic);
@@ -705,8 +717,8 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
     }
     const bool more = len > cap;
     const unsigned long long mr = __ballot(more);
-    if (more) W.segs[W.segn + lanes_below(mr)] =
-        RowSeg{(int16_t)ic, (int16_t)(bl0 + cap), (uint8_t)(len - cap), (uint8_t)own, 0};
+    if (more) W.segs[lanes_below(mr, (uint32_t)W.segn)] =
+        row_seg(ic, bl0 + cap, len - cap, own);
     W.segn = __builtin_amdgcn_readfirstlane(W.segn + __popcll(mr));
     wave_sync_lds();
 }
@@ -933,8 +945,8 @@ __global__ __launch_bounds__(kDepThreads) void k_deposit(DepArgs A) {
                 const bool sgok = have && ic <= c1 && row_band(rb, ic, bl, bh);
                 W.ncols += sgok ? (uint32_t)(bh - bl + 1) : 0u;
                 const unsigned long long m = __ballot(sgok);
-                if (sgok) W.segs[W.segn + lanes_below(m)] =
-                    RowSeg{(int16_t)ic, (int16_t)bl, (uint8_t)(bh - bl + 1), (uint8_t)lane, 0};
+                if (sgok) W.segs[lanes_below(m, (uint32_t)W.segn)] =
+                    row_seg(ic, bl, bh - bl + 1, lane);
                 W.segn = __builtin_amdgcn_readfirstlane(W.segn + __popcll(m));
                 wave_sync_lds();
                 while (W.segn >= 64) stage_columns<F64, FAST>(W, me, 64);
