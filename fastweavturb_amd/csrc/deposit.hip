@@ -481,13 +481,16 @@ __device__ __forceinline__ bool row_band(const RowBand &rb, int32_t ic, int32_t 
 // The a-range bounds of one row segment's columns: on row dc the plane positions are linear in
 // db, so each plane gives one line q - B db that bounds a from below or from above, with the
 // margin already applied (q = (P - C dc) -/+ m, rounded twice before the fma that evaluates it:
-// within 1.8e-7 of |P| + |B| Wb + |C| Wc, under the margin's 3e-7). Unused slots hold B = 0 and
-// q = -/+1e9. The lower bounds are stored minus one, so that a column's count is
-// floor(hi) - ceil(lo - 1) (a rounding of lo - 1 can only lower it by one more: one extra candidate).
+// within 1.8e-7 of |P| + |B| Wb + |C| Wc, under the margin's 3e-7). One lower and one upper line
+// are kept: the end planes usually bound a from opposite sides; where both bound it from the
+// same side (the chord turns by more than 90 degrees) only plane 0's line is kept there, a looser
+// bound that can only add candidates. An unused slot holds B = 0 and q = -/+1e9. The lower bound
+// is stored minus one, so that a column's count is floor(hi) - ceil(lo - 1) (a rounding of lo - 1
+// can only lower it by one more: one extra candidate).
 struct ColLines {
-    float qL0, BL0, qL1, BL1;   // lower bounds, minus one
-    float qU0, BU0, qU1, BU1;   // upper bounds
-    float A0, A1;               // the pair's a-range, relative to lo_a (A0 minus one)
+    float qL, BL;      // lower bound, minus one
+    float qU, BU;      // upper bound
+    int32_t A0, A1;    // the pair's a-range, relative to lo_a (A0 minus one)
     int32_t lo_a1, lo_b;        // lo_a + 1 (a-index of the count's origin), lo_b
 };
 
@@ -496,16 +499,19 @@ __device__ __forceinline__ ColLines col_lines(const ColWalk &w, int32_t ic) {
     const float dcn = (float)(ic - w.lo_c);
     const float p0 = __builtin_fmaf(-w.C0, dcn, w.P0), p1 = __builtin_fmaf(-w.C1, dcn, w.P1);
     const bool up0 = w.up0, lo1 = w.use1 && !w.up1, hi1 = w.use1 && w.up1;
-    L.qL0 = up0 ? (p0 - w.m0) - 1.0f : -1e9f;
-    L.BL0 = up0 ? w.B0 : 0.0f;
-    L.qU0 = up0 ? 1e9f : p0 + w.m0;
-    L.BU0 = up0 ? 0.0f : w.B0;
-    L.qL1 = lo1 ? (p1 - w.m1) - 1.0f : -1e9f;
-    L.BL1 = lo1 ? w.B1 : 0.0f;
-    L.qU1 = hi1 ? p1 + w.m1 : 1e9f;
-    L.BU1 = hi1 ? w.B1 : 0.0f;
-    L.A0 = (float)(w.a0 - w.lo_a - 1);
-    L.A1 = (float)(w.a1 - w.lo_a);
+    if (up0) {   // plane 0 bounds a from below
+        L.qL = (p0 - w.m0) - 1.0f;
+        L.BL = w.B0;
+        L.qU = hi1 ? p1 + w.m1 : 1e9f;
+        L.BU = hi1 ? w.B1 : 0.0f;
+    } else {     // from above
+        L.qU = p0 + w.m0;
+        L.BU = w.B0;
+        L.qL = lo1 ? (p1 - w.m1) - 1.0f : -1e9f;
+        L.BL = lo1 ? w.B1 : 0.0f;
+    }
+    L.A0 = w.a0 - w.lo_a - 1;
+    L.A1 = w.a1 - w.lo_a;
     L.lo_a1 = w.lo_a + 1;
     L.lo_b = w.lo_b;
     return L;
@@ -514,10 +520,10 @@ __device__ __forceinline__ ColLines col_lines(const ColWalk &w, int32_t ic) {
 // Candidate a-range of the column at db = ib - lo_b of a row segment: returns the count (at most
 // one more than the cells between the bounds, never fewer), the first index minus lo_a1 in *ia0.
 __device__ __forceinline__ int32_t column_range(const ColLines &L, float db, int32_t *ia0) {
-    const float flo = ceilf(fmaxf(fmaxf(L.A0, __builtin_fmaf(-L.BL0, db, L.qL0)), __builtin_fmaf(-L.BL1, db, L.qL1)));
-    const float fhi = floorf(fminf(fminf(L.A1, __builtin_fmaf(-L.BU0, db, L.qU0)), __builtin_fmaf(-L.BU1, db, L.qU1)));
-    *ia0 = (int32_t)flo;
-    return (int32_t)fmaxf(fhi - flo, 0.0f);
+    const int32_t flo = max((int32_t)ceilf(__builtin_fmaf(-L.BL, db, L.qL)), L.A0);
+    const int32_t fhi = min((int32_t)floorf(__builtin_fmaf(-L.BU, db, L.qU)), L.A1);
+    *ia0 = flo;
+    return max(fhi - flo, 0);
 }
 
 __device__ __forceinline__ void wave_sync_lds() {
