@@ -380,10 +380,9 @@ __device__ __forceinline__ double rcp_nr(double x) {
 // half-width quadratic. The constants are formed in FP64 from the single-precision walk and
 // rounded once; a row then costs two plane lines, one square root and the clipping.
 struct RowBand {
-    float aL0, bL0, aL1, bL1;      // lower bounds on db: aL + bL dc (a plane's line, or b0r)
-    float aU0, bU0, aU1, bU1;      // upper bounds on db: aU + bU dc (a plane's line, or b1r)
+    float aL, bL, aU, bU;          // bounds on db: db >= aL + bL dc, db <= aU + bU dc (plane lines)
     float cm, cp, g, k0, k1, uc;   // ellipse: db in [cm + g Dc - hw, cp + g Dc + hw], hw^2 = k0 - k1 Dc^2
-    float b0r, b1r;                // the pair's b-range relative to lo_b
+    int32_t b0r, b1r;              // the pair's b-range relative to lo_b
     int32_t lo_b, lo_c;
 };
 
@@ -423,8 +422,8 @@ __device__ __forceinline__ RowBand row_setup(const ColWalk &w, double ub, double
     RowBand rb;
     rb.lo_b = w.lo_b;
     rb.lo_c = w.lo_c;
-    rb.b0r = (float)(b0 - w.lo_b);
-    rb.b1r = (float)(b1 - w.lo_b);
+    rb.b0r = b0 - w.lo_b;
+    rb.b1r = b1 - w.lo_b;
     const double dmax = fmax(fabs((double)(c0 - w.lo_c)), fabs((double)(c1 - w.lo_c)));
     const double A0 = w.a0 - w.lo_a, A1 = w.a1 - w.lo_a;
     float al0, be0, al1 = -1e9f, be1 = 0.0f;
@@ -439,15 +438,12 @@ __device__ __forceinline__ RowBand row_setup(const ColWalk &w, double ub, double
         const double Q = w.up1 ? (double)w.P1 + m2 - A0 : (double)w.P1 - m2 - A1;
         plane_line(Q, w.B1, w.C1, w.up1 ? -1.0 : 1.0, dmax, w.lo_c, al1, be1, lo1, c0, c1);
     }
-    // each plane's line bounds db from one side; the other side's slot holds the b-range end
-    rb.aL0 = lo0 ? al0 : rb.b0r;
-    rb.bL0 = lo0 ? be0 : 0.0f;
-    rb.aU0 = lo0 ? rb.b1r : al0;
-    rb.bU0 = lo0 ? 0.0f : be0;
-    rb.aL1 = lo1 ? al1 : rb.b0r;
-    rb.bL1 = lo1 ? be1 : 0.0f;
-    rb.aU1 = lo1 ? rb.b1r : al1;
-    rb.bU1 = lo1 ? 0.0f : be1;
+    // one line per side: plane 0's on its side, plane 1's on the other if it bounds that side
+    // (where both bound the same side only plane 0's is kept, a looser band); none: -/+1e9
+    rb.aL = lo0 ? al0 : (lo1 ? al1 : -1e9f);
+    rb.bL = lo0 ? be0 : (lo1 ? be1 : 0.0f);
+    rb.aU = !lo0 ? al0 : (!lo1 ? al1 : 1e9f);
+    rb.bU = !lo0 ? be0 : (!lo1 ? be1 : 0.0f);
     // ellipse (1 + B0^2) Db^2 + 2 B0 C0 Db Dc + (1 + C0^2) Dc^2 < Rp2, Db = db - ub: for each Dc,
     // Db = -B0 C0 Dc / eb +- sqrt(Rp2 / eb - (1 + B0^2 + C0^2) Dc^2 / eb^2) with eb = 1 + B0^2
     const double B0 = w.B0, C0 = w.C0, eb = 1.0 + B0 * B0, ieb = rcp_nr(eb);
@@ -468,14 +464,12 @@ __device__ __forceinline__ bool row_band(const RowBand &rb, int32_t ic, int32_t 
     const float Dc = dcn - rb.uc;
     const float h2 = __builtin_fmaf(-rb.k1 * Dc, Dc, rb.k0);
     const float hw = __builtin_amdgcn_sqrtf(fmaxf(h2, 0.0f));
-    const float lo = fmaxf(fmaxf(__builtin_fmaf(rb.bL0, dcn, rb.aL0), __builtin_fmaf(rb.bL1, dcn, rb.aL1)),
-                           fmaxf(__builtin_fmaf(rb.g, Dc, rb.cm) - hw, rb.b0r));
-    const float hi = fminf(fminf(__builtin_fmaf(rb.bU0, dcn, rb.aU0), __builtin_fmaf(rb.bU1, dcn, rb.aU1)),
-                           fminf(__builtin_fmaf(rb.g, Dc, rb.cp) + hw, rb.b1r));
-    if (!(h2 >= 0.0f) || !(lo <= hi)) return false;
-    bl = rb.lo_b + (int32_t)ceilf(lo);
-    bh = rb.lo_b + (int32_t)floorf(hi);
-    return bl <= bh;
+    const float lo = fmaxf(__builtin_fmaf(rb.bL, dcn, rb.aL), __builtin_fmaf(rb.g, Dc, rb.cm) - hw);
+    const float hi = fminf(__builtin_fmaf(rb.bU, dcn, rb.aU), __builtin_fmaf(rb.g, Dc, rb.cp) + hw);
+    // the b-range clip in integers (ceil(max(x, b)) = max(ceil(x), b) for an integer b)
+    bl = rb.lo_b + max((int32_t)ceilf(lo), rb.b0r);
+    bh = rb.lo_b + min((int32_t)floorf(hi), rb.b1r);
+    return h2 >= 0.0f && bl <= bh;
 }
 
 // The a-range bounds of one row segment's columns: on row dc the plane positions are linear in
