@@ -298,7 +298,10 @@ __global__ __launch_bounds__(256) void k_pairs_count(Geo g, Run r0, uint32_t *__
 // spreads those expensive pairs over its waves before it takes the rest in runs. The boxes and
 // (up to kRecMaskBits entries) the slab tests come from the count pass's records: such a lane
 // steps through the set bits of its mask only, forming each entry's brick and image from its
-// index; a box of more entries walks all of them and repeats its slab tests.
+// index; a box of more entries walks all of them and repeats its slab tests. The two kinds run as
+// two launches (SLOW), so that the first, which has nearly all the pairs, keeps few registers
+// and many waves in flight for its returned atomics.
+template <bool SLOW>
 __global__ __launch_bounds__(256) void k_pairs_fill(Geo g, Run r0, uint32_t *__restrict__ cnt,
                                                     const BinRec *__restrict__ rec, const int64_t *__restrict__ off,
                                                     const uint32_t *__restrict__ wide, uint32_t *__restrict__ pairs,
@@ -328,12 +331,13 @@ __global__ __launch_bounds__(256) void k_pairs_fill(Geo g, Run r0, uint32_t *__r
         }
     }
     const bool slow = n > kRecMaskBits;
+    if (SLOW != slow) n = 0;   // the other launch's
     SubSeg sg;
-    if (slow) subseg_load_fast(g, q, sg);   // a paired sub-segment: it loads
-    uint32_t m = (!slow && n > 0) ? rc.mask : 0u;
-    const int32_t nmax = wave_max(slow ? n : __popc(m));
+    if (SLOW && n > 0) subseg_load_fast(g, q, sg);   // a paired sub-segment: it loads
+    uint32_t m = (!SLOW && n > 0) ? rc.mask : 0u;
+    const int32_t nmax = wave_max(SLOW ? n : __popc(m));
     EntryWalk<true> ews;     // slow lanes: slab tests again
-    if (slow) ews.start(ab, sg, bd, brick0, nbx, nby);
+    if (SLOW && n > 0) ews.start(ab, sg, bd, brick0, nbx, nby);
     // entry e = jx + nx (jy + ny jz): quotients by nx, ny from single-precision reciprocals
     // ((e + 1/2) / nx is at least 1/64 from an integer for e < 32)
     const float inx = 1.0f / (float)nx, iny = 1.0f / (float)ny;
@@ -341,7 +345,7 @@ __global__ __launch_bounds__(256) void k_pairs_fill(Geo g, Run r0, uint32_t *__r
         bool act = false;
         int64_t key = 0;
         int32_t code = 0;
-        if (slow) {
+        if (SLOW) {
             act = t < n && ews.take(ab, sg, bd, brick0, nbx, nby, nx, ny, t, n, key, code);
         } else if (m != 0u) {
             const int32_t e = __builtin_ctz(m);
@@ -448,7 +452,12 @@ static hipError_t bin_fill_kernels(wt_ctx *c) {
     WT_HIP(fill_zero(w.cnt, sizeof(uint32_t) * b.nbricks, st));
     WT_HIP(fill_zero(w.wslot, sizeof(uint32_t) * b.nbricks, st));
     kt_begin(c, kSlotBinFill);
-    hipLaunchKernelGGL(k_pairs_fill, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, w.cnt,
+    // the boxes of more than 32 entries first: their pairs, costlier than most, then follow the
+    // wide boxes' at the head of each brick's list, and the cheap pairs fill the list's end
+    hipLaunchKernelGGL(k_pairs_fill<true>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, w.cnt,
+                       (const BinRec *)w.rec, (const int64_t *)b.brick_off, (const uint32_t *)b.brick_wide, b.pairs,
+                       b.pair_img);
+    hipLaunchKernelGGL(k_pairs_fill<false>, dim3(grid_for(g.nsub_all, 256)), dim3(256), 0, st, g, c->run, w.cnt,
                        (const BinRec *)w.rec, (const int64_t *)b.brick_off, (const uint32_t *)b.brick_wide, b.pairs,
                        b.pair_img);
     hipLaunchKernelGGL(k_pairs_big<true>, dim3(w.big_grid), dim3(256), 0, st, g, c->run, w.cnt, w.wslot,
