@@ -701,6 +701,7 @@ __device__ __forceinline__ void stage_columns(Wave<F64> &W, const SegRec &me, in
         nc0 = rem > 0 ? nc0 : 0;
         nc1 = rem > 1 ? nc1 : 0;
         const int32_t nc = nc0 + nc1, ia1s = ia1 - nc0;
+        W.candn = __builtin_amdgcn_readfirstlane(W.candn);   // a scalar through the rounds
         unsigned long long m;
         for (int u = 0; (m = __ballot(u < nc)) != 0ull; u++) {
             if (u < nc) {
