@@ -608,7 +608,11 @@ __device__ __forceinline__ void stage_candidates(Wave<F64> &W, const SegRec &me,
     bool hit = false, ovf = false;
     if (lane < cnt) {
         HitConst hc;
-        if (FAST) hit_const_load(W.A->g, sg, hc);
+        if (FAST) {
+            hit_const_load(W.A->g, sg, hc);
+            // keeps the spline loads here, ahead of the test, rather than sunk into the hit branch
+            asm volatile("" ::: "memory");
+        }
         const double R2 = sg.R * sg.R;
         hit = pair_hits_fast(sg, h.ip, h.jp, h.kp, R2 * (1.0 - 1e-14), R2 * (1.0 + 1e-14));
         if (hit) {
