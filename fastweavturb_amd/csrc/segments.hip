@@ -348,14 +348,15 @@ __global__ __launch_bounds__(128) void k_theta_fast(Geo g, Run, int32_t *__restr
         float wp[3], wm[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float E = horner8(wE[c], 3, v), Os = sv * horner8(wO[c], 3, v);
-            wp[c] = E + Os;
-            wm[c] = E - Os;
+            const float E = horner8(wE[c], 3, v), O = horner8(wO[c], 3, v);
+            wp[c] = __builtin_fmaf(sv, O, E);
+            wm[c] = __builtin_fmaf(-sv, O, E);
         }
-        const float gEv = horner8(gE, 4, v), gOs = sv * horner8(gO, 3, v);
-        const float fp = fval(wp[0], wp[1], wp[2], gEv + gOs), fm = fval(wm[0], wm[1], wm[2], gEv - gOs);
+        const float gEv = horner8(gE, 4, v), gOv = horner8(gO, 3, v);
+        const float gp = __builtin_fmaf(sv, gOv, gEv), gm = __builtin_fmaf(-sv, gOv, gEv);
+        const float fp = fval(wp[0], wp[1], wp[2], gp), fm = fval(wm[0], wm[1], wm[2], gm);
         acc += (double)(fp + fm);
-        gmin = fminf(gmin, gEv - fabsf(gOs));
+        gmin = fminf(gmin, fminf(gp, gm));
     }
     // error bound of the FP32 sum against the reference's FP64 sum (both times dz): per sample the
     // numerator's error kw Wb over the denominator and the denominator's relative error kg Gb / g,
